@@ -1,0 +1,176 @@
+"""Benchmark: fused var-coefficient diffusion r <<= div(interp(k) * grad(phi)) on a
+256^3 fp64 SVolField with one periodic ghost layer, on one MI355X.
+
+One step = one evaluation of the fused assignment over the 256^3 interior with
+inputs resident in HBM.  Prints ONE JSON line (the contract of this repository):
+value = GCell-updates/s, plus the roofline of the dominant kernel (HIP events
+around each launch on the launch stream) and a CPU baseline (the oracle's
+restatement of the reference's serial and z-slab thread-parallel walks, timed on
+this host).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 256] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spatialops-threadfix_amd", "python"))
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_CELL = 24        # read phi, read k, write r (fp64)
+METRIC = ("GCell-updates/s & %HBM peak, div(k∇φ) 256³ fp64; "
+          "batched and adjoint vs plain")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    return p.parse_args()
+
+
+def cpu_baseline(L, kh, ph, coefs, budget_s):
+    """The oracle (CPU restatement of the reference walk) on this host: serial and
+    thread-parallel, each run repeatedly on the full workload for about budget_s/2."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import nebo_hip as nh
+    import oracle_lib as orc
+    n3 = L.interior[0] * L.interior[1] * L.interior[2]
+    r = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+    k = nh.Field(L, host=kh)
+    phi = nh.Field(L, host=ph)
+    cores = min(16, os.cpu_count() or 1)
+    out = {}
+    for label, thr in (("serial", 0), ("threads", cores)):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            orc.divkgrad(r, k, phi, coefs, 0, threads=thr)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s / 2 or reps >= 50:
+                break
+        out[label] = (n3 * reps / el / 1e9, reps, el)
+    gs, reps_s, el_s = out["serial"]
+    gt, reps_t, el_t = out["threads"]
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(gt, 4), "unit": "GCell-updates/s", "cores": cores, "kind": "port",
+            "sample": (f"full {L.interior[0]}^3 div(k grad phi) evaluations: {reps_t} on {cores} "
+                       f"threads (z-slab split, {el_t:.1f} s); serial {reps_s} in {el_s:.1f} s = "
+                       f"{gs:.4f} GCell/s; host {cpu_model}"),
+            "serial_value": round(gs, 4)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import nebo_hip as nh
+    from nebo_hip import inputs
+
+    n = a.n
+    L = nh.Layout.build((n, n, n), 1)
+    kh = inputs.coefficient_field(n, 2).reshape(-1)
+    ph = inputs.fourier_field(n, 3).reshape(-1)
+    k = nh.Field.from_numpy(L, kh, where="gpu")
+    phi = nh.Field.from_numpy(L, ph, where="gpu")
+    r = nh.Field.empty(L, "gpu")
+    r.gpu.zero_()
+    db = nh.build_stencils(n, n, n, 1.0, 1.0, 1.0)
+    ops = nh.basic_op_types(db)
+    coefs = nh.axis_coefs(db)
+    tree = (ops["DivX"](ops["InterpC2FX"](k) * ops["GradX"](phi))
+            + ops["DivY"](ops["InterpC2FY"](k) * ops["GradY"](phi))
+            + ops["DivZ"](ops["InterpC2FZ"](k) * ops["GradZ"](phi)))
+    assert nh.assign_plan(r, tree) == nh._abi.KERNEL_DIVKGRAD
+    # the serialised descriptor is reused across steps, as a caller's plan would be
+    ser = nh.serialize(tree, "gpu")
+    import ctypes as C
+    rd = r.desc("gpu")
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    lib = nh.lib()
+
+    def step():
+        rc = lib.nebo_assign(C.byref(rd), ser.ref, 1, sp)
+        if rc:
+            nh._abi.check(rc)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    # per-launch durations (HIP events on the launch stream)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(a.steps)]
+    for e0, e1 in evs:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = sorted(e0.elapsed_time(e1) for e0, e1 in evs)
+    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+    # whole-job timing: K back-to-back steps between two synchronisations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g0.record(stream)
+    for _ in range(a.steps):
+        step()
+    g1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = g0.elapsed_time(g1) / a.steps
+    cells = n ** 3
+    gcells = cells / (ms_per_step * 1e-3) / 1e9
+    achieved = BYTES_PER_CELL * cells / (avg_kernel_ms * 1e-3)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"divkgrad_{n}_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(L, kh, ph, coefs, a.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(gcells, 3), "unit": "GCell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded k ~ U[0.5,2), 8-mode Fourier phi, periodic ghosts)",
+        "config": {"workload": f"div(k grad phi) {n}^3 fp64, 1 ghost layer, fused <<=",
+                   "n": n, "cells": cells, "kernel": "divkgrad (hand-written HIP, gfx950)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "bytes_per_launch_algorithmic": BYTES_PER_CELL * cells,
+                     "avg_kernel_ms": round(avg_kernel_ms, 5),
+                     "median_kernel_ms": round(kernel_ms[len(kernel_ms) // 2], 5)},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": round(wall, 4),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,339 @@
+/*
+ * nebo_hip.h -- C-ABI of the MI355X (gfx950) evaluation engine for SpatialOps/Nebo
+ * field expressions.
+ *
+ * Every entry point takes plain pointers, sizes and POD descriptors; none of them
+ * owns field memory passed in a descriptor.  Every call returns an int status
+ * (NEBO_OK == 0); on failure nebo_last_error() returns a thread-local message that
+ * starts with "Nebo error in <context>:" so that a C++ wrapper can rethrow it as
+ * std::runtime_error exactly like the reference does
+ * (spatialops/NeboLhs.h:107-153, spatialops/NeboBasic.h:65-93).
+ *
+ * Memory layout (identical to the reference, spatialops/structured/MemoryWindow.h:121-131):
+ * a field is one x-fastest block of glob[0]*glob[1]*glob[2] doubles; the window is
+ * (offset, extent) inside that block; flat index = x + glob[0]*(y + glob[1]*z).
+ *
+ * Streams are passed as `void*` holding a hipStream_t (NULL = the null stream).
+ * No call synchronises implicitly unless its comment says so.  Calls on different
+ * streams may run concurrently from different host threads.
+ */
+#ifndef NEBO_HIP_H
+#define NEBO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* status codes                                                              */
+/* ------------------------------------------------------------------------- */
+#define NEBO_OK               0
+#define NEBO_ERR_INVALID      1   /* malformed descriptor / argument            */
+#define NEBO_ERR_GHOST        2   /* not enough valid ghost / extra cells       */
+#define NEBO_ERR_DEVICE       3   /* operand not resident on the LHS device     */
+#define NEBO_ERR_HIP          4   /* HIP runtime error                          */
+#define NEBO_ERR_TOO_LARGE    5   /* expression exceeds the evaluator's limits  */
+#define NEBO_ERR_UNSUPPORTED  6   /* operation not provided on this device      */
+
+/* device index semantics of spatialops/structured/MemoryTypes.h:14-19 */
+#define NEBO_CPU_INDEX (-1)
+
+/* GHOST_MAX of spatialops/structured/GhostData.h:43 ("infinite" ghost count) */
+#define NEBO_GHOST_MAX 9001
+
+/* ------------------------------------------------------------------------- */
+/* descriptors                                                               */
+/* ------------------------------------------------------------------------- */
+
+/* One SpatialField: SpatialField.h:128-603 + FieldInfo.h + BoundaryCellInfo.h. */
+typedef struct nebo_field {
+  void*   values;          /* first element of the allocation (glob block)      */
+  int32_t glob[3];         /* MemoryWindow::glob_dim                            */
+  int32_t offset[3];       /* MemoryWindow::offset (window with ghosts)         */
+  int32_t extent[3];       /* MemoryWindow::extent (window with ghosts)         */
+  int32_t ghost_minus[3];  /* get_ghost_data()       (allocated ghost layers)   */
+  int32_t ghost_plus[3];
+  int32_t valid_minus[3];  /* get_valid_ghost_data() (valid ghost layers)       */
+  int32_t valid_plus[3];
+  int32_t has_bc[3];       /* boundary_info().has_bc()                          */
+  int32_t bc_extra[3];     /* boundary_info().has_extra()                       */
+  int32_t device;          /* active device: NEBO_CPU_INDEX or a GPU ordinal    */
+  int32_t pad_;
+} nebo_field_t;
+
+/* One SpatialMask (SpatialMask.h:99-181, BitField.h:291-333).  Bit b of the
+ * mask is set when the point of flat index b (over the mask's glob block) is in
+ * the mask: word b/32, bit b%32. */
+typedef struct nebo_mask {
+  const uint32_t* bits;    /* device copy of the bit field (add_consumer)       */
+  int32_t glob[3];
+  int32_t offset[3];       /* window offset (with ghosts)                       */
+  int32_t extent[3];       /* window extent (with ghosts)                       */
+  int32_t ghost_minus[3];  /* get_ghost_data()                                  */
+  int32_t ghost_plus[3];
+  int32_t valid_minus[3];  /* get_valid_ghost_data()                            */
+  int32_t valid_plus[3];
+  int32_t has_bc[3];
+  int32_t bc_extra[3];
+} nebo_mask_t;
+
+/* Expression node opcodes.  Leaves, pointwise operators (NeboOperators.h),
+ * stencils (NeboStencils.h), cond (NeboCond.h) and masks (NeboMask.h). */
+enum nebo_opcode {
+  NEBO_SCALAR = 0,     /* value                                  NeboRhs.h:29-158  */
+  NEBO_FIELD,          /* idx = field table index                NeboRhs.h:160-328 */
+  NEBO_MASK,           /* idx = mask table index (boolean)       NeboMask.h:29-213 */
+  NEBO_ADD, NEBO_SUB, NEBO_MUL, NEBO_DIV,          /* a (op) b                     */
+  NEBO_NEG,                                         /* -a                           */
+  NEBO_MAX, NEBO_MIN,   /* (a > b) ? a : b  /  (a < b) ? a : b  NeboOperators.h:13237,13992 */
+  NEBO_ABS, NEBO_SQRT, NEBO_SQUARE, NEBO_CUBE,
+  NEBO_POW,             /* std::pow(a, b)                                        */
+  NEBO_POWINT,          /* std::pow(a, idx)                     NeboOperators.h:14710 */
+  NEBO_EXP, NEBO_LOG, NEBO_LOG10, NEBO_SIN, NEBO_COS, NEBO_TAN, NEBO_TANH,
+  NEBO_ERF, NEBO_ERFC,
+  NEBO_EQ, NEBO_NE, NEBO_LT, NEBO_LE, NEBO_GT, NEBO_GE,   /* boolean results       */
+  NEBO_AND, NEBO_OR, NEBO_NOT,                             /* boolean operands     */
+  NEBO_STENCIL,         /* a = argument node, idx = stencil table index          */
+  NEBO_COND,            /* test a ? value b : otherwise c       NeboCond.h:408-412 */
+  NEBO_NUM_OPCODES
+};
+
+typedef struct nebo_node {
+  int32_t op;            /* enum nebo_opcode                                    */
+  int32_t a, b, c;       /* child node indices; every child precedes its parent */
+  int32_t idx;           /* field / mask / stencil table index, or exponent     */
+  int32_t pad_;
+  double  value;         /* NEBO_SCALAR literal                                 */
+} nebo_node_t;
+
+#define NEBO_MAX_STENCIL_POINTS 27
+
+/* Stencil kinds (NeboStencils.h) */
+enum nebo_stencil_kind {
+  NEBO_STENCIL_WEIGHTED = 0,  /* fold_p arg(x+p)*c_p, left fold in point order  :407-448 */
+  NEBO_STENCIL_SUM      = 1   /* fold_p arg(x+p), left fold in point order      :940-975 */
+};
+
+typedef struct nebo_stencil {
+  int32_t kind;                               /* enum nebo_stencil_kind     */
+  int32_t npts;
+  int32_t pts[NEBO_MAX_STENCIL_POINTS][3];    /* offsets, definition order  */
+  double  coefs[NEBO_MAX_STENCIL_POINTS];     /* one per point (WEIGHTED)   */
+} nebo_stencil_t;
+
+/* A whole right-hand side: node list in post-order (root = last node). */
+typedef struct nebo_expr {
+  int32_t n_nodes;    const nebo_node_t*    nodes;
+  int32_t n_fields;   const nebo_field_t*   fields;
+  int32_t n_stencils; const nebo_stencil_t* stencils;
+  int32_t n_masks;    const nebo_mask_t*    masks;
+} nebo_expr_t;
+
+/* Loop limits in the LHS frame (relative to the first valid-ghost-excluded
+ * point): [lo[d], hi[d]) per axis.  NeboLhs.h:79-84. */
+typedef struct nebo_limits { int32_t lo[3]; int32_t hi[3]; } nebo_limits_t;
+
+/* ------------------------------------------------------------------------- */
+/* runtime                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* Last error message of the calling thread ("" when none). */
+const char* nebo_last_error(void);
+
+/* Number of visible GPUs (CUDADeviceInterface::get_device_count,
+ * spatialops/structured/ExternalAllocators.h:59-106). */
+int nebo_device_count(int* count);
+
+/* Select `device` and warm the runtime (CUDADeviceInterface::self()). */
+int nebo_init(int device);
+
+/* Device memory through a size-bucketed pool that keeps freed blocks for
+ * reuse (replaces Pool<T>::get/put, spatialops/structured/MemoryPool.cpp:102-186). */
+int nebo_alloc(int device, size_t bytes, void** out);
+int nebo_free(int device, void* ptr);
+/* Release every pooled block of `device` back to the runtime. */
+int nebo_pool_trim(int device);
+/* Bytes currently handed out / high-water mark (MemoryPool.h:49,82-90). */
+int nebo_pool_stats(int device, size_t* in_use, size_t* high_water);
+
+/* Copies.  kind: 0 host->device, 1 device->host, 2 device->device,
+ * 3 peer (dst on dst_dev, src on src_dev).  stream NULL => synchronous
+ * (CUDADeviceInterface::memcpy_to/from/peer, CudaMemoryAllocator.cu:143-230). */
+int nebo_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int nebo_memcpy_peer(void* dst, int dst_dev, const void* src, int src_dev,
+                     size_t bytes, void* stream);
+int nebo_memset(void* dst, int value, size_t bytes, void* stream);
+int nebo_stream_sync(void* stream);
+int nebo_device_sync(void);
+
+/* ------------------------------------------------------------------------- */
+/* assignment: the seam NeboField<Initial,F>::assign -> gpu_assign            */
+/* ------------------------------------------------------------------------- */
+
+/* Host-only: the loop limits `lhs <<= rhs` (use_ghost=1) or
+ * `interior_assign(lhs, rhs)` (use_ghost=0) would cover: calculate_actual_ghost
+ * (NeboBasic.h:61-100) + NeboLhs.h:67-88.  Sets *empty when the range has zero
+ * volume (no kernel is launched).  Throws the reference's ghost errors as
+ * NEBO_ERR_GHOST. */
+int nebo_assign_limits(const nebo_field_t* lhs, const nebo_expr_t* rhs,
+                       int use_ghost, nebo_limits_t* limits, int* empty);
+
+/* lhs <<= rhs on the LHS's GPU: one fused kernel over the whole tree
+ * (replaces gpu_assign + gpu_assign_kernel, NeboLhs.h:29-40,400-478).
+ * Recognised trees run on hand-written kernels; every other tree runs on the
+ * generic fused evaluator.  Results equal the reference CPU backend bit for bit
+ * for + - * / neg max min abs sqrt square cube, comparisons, cond, masks and
+ * stencils; pow and the transcendental functions (exp log log10 sin cos tan
+ * tanh erf erfc) use the device math library and stay within the ulp budgets
+ * of the reference's own tests (test/NeboTest.cpp:497-523); pow(x, 2) is
+ * evaluated as the correctly rounded x*x. */
+int nebo_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
+                void* stream);
+
+/* Like nebo_assign but forces the generic fused evaluator (no pattern match). */
+int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
+                        int use_ghost, void* stream);
+
+/* Which kernel nebo_assign would pick for this tree: 0 generic evaluator,
+ * 1 var-coefficient diffusion, 2 Laplacian+Euler, 3 pointwise copy/scale.
+ * Host-only. */
+int nebo_assign_plan(const nebo_field_t* lhs, const nebo_expr_t* rhs,
+                     int use_ghost, int* kernel_id);
+
+/* masked_assign(mask, lhs, rhs): lhs(p) = rhs(p) for each listed point p
+ * (interior-relative ijk triples, device array of 3*n_points int32).  The
+ * reference evaluates this on CPU only (NeboLhs.h:164-283, NeboAssignment.h:246-313);
+ * this is its GPU counterpart, with one evaluation per point. */
+int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points_dev,
+                       int32_t n_points, const nebo_expr_t* rhs, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* reductions (NeboReductions.h:45-240; CPU-only in the reference)           */
+/* ------------------------------------------------------------------------- */
+enum nebo_reduce_op { NEBO_RED_MAX = 0, NEBO_RED_MIN = 1, NEBO_RED_SUM = 2, NEBO_RED_NORM = 3 };
+
+/* field_max/min/sum/norm (use_ghost=1) and the _interior variants (use_ghost=0)
+ * of an expression.  Synchronises `stream` and writes the host scalar *out.
+ * max/min are exact; sum/norm accumulate in double-double (error-free
+ * transformations) and round once, so they are within one rounding of the
+ * exact sum. */
+int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
+                void* stream);
+
+/* Same, asynchronous: writes one double to device memory out_dev. */
+int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
+                      double* out_dev, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* ghost cells                                                               */
+/* ------------------------------------------------------------------------- */
+
+/* Periodic copy-fill of every allocated ghost layer of the field from the
+ * opposite interior side (the reference leaves ghost filling to its caller;
+ * this is the fill the periodic workloads use). */
+int nebo_fill_ghosts_periodic(const nebo_field_t* field, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* direct entry points of the hand-written stencil kernels                    */
+/* ------------------------------------------------------------------------- */
+
+/* Per-axis coefficients of the staggered operators of build_stencils
+ * (spatialops/structured/stencil/StencilBuilder.cpp:60-72). */
+typedef struct nebo_axis_coefs {
+  double interp[3][2];   /* InterpC2F{X,Y,Z}: {0.5, 0.5}           */
+  double grad[3][2];     /* Grad{X,Y,Z}:      {-1/dx, +1/dx}       */
+  double div[3][2];      /* Div{X,Y,Z}:       {-1/dx, +1/dx}       */
+} nebo_axis_coefs_t;
+
+/* r <<= divX(interpX(k)*gradX(phi)) + divY(..) + divZ(..)     (form 0)
+ * r <<= -divX(gradX(phi)*interpX(k)) - divY(..) - divZ(..)    (form 1,
+ *       test/StencilTest.cpp:193-195)
+ * over r's interior (the ghost inference of the tree gives interior only). */
+int nebo_divkgrad(const nebo_field_t* r, const nebo_field_t* k,
+                  const nebo_field_t* phi, const nebo_axis_coefs_t* c, int form,
+                  void* stream);
+
+/* out <<= phi + s*(divX(gradX(phi)) + divY(gradY(phi)) + divZ(gradZ(phi)))
+ * over out's interior: one explicit-Euler heat-equation step with s = dt*alpha. */
+int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
+                    const nebo_axis_coefs_t* c, double s, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* batched patches: one persistent launch over a device patch table          */
+/* ------------------------------------------------------------------------- */
+
+/* One Uintah-style patch: its own interior dims, 1 ghost layer, in/out blocks. */
+typedef struct nebo_patch {
+  const double* in;      /* (n+2)^3 block, x fastest                        */
+  double*       out;     /* same shape                                      */
+  int32_t n[3];          /* interior cells per axis                         */
+  int32_t pad_;
+  double  inv_dx2[3];    /* (1/dx)^2 factors are not used: see coefs below  */
+} nebo_patch_t;
+
+typedef struct nebo_batch nebo_batch_t;   /* opaque */
+
+/* Build a device patch table and the tile schedule for `n` patches.  tile[3]
+ * is the (x,y,z) tile of one work item; tile == NULL lets the autotuner pick
+ * (nebo_batch_autotune).  Coefficients and s are shared by all patches. */
+int nebo_batch_create(const nebo_patch_t* patches, int32_t n,
+                      const nebo_axis_coefs_t* c, double s,
+                      const int32_t* tile, nebo_batch_t** out);
+int nebo_batch_destroy(nebo_batch_t* b);
+/* One Laplacian+Euler step on every patch (in -> out) in one persistent launch. */
+int nebo_batch_step(nebo_batch_t* b, int swap_in_out, void* stream);
+/* The same step as one launch per patch (the comparison path). */
+int nebo_batch_step_per_patch(nebo_batch_t* b, int swap_in_out, void* stream);
+/* Capture `steps` alternating steps into a hipGraph (built once, replayed by
+ * nebo_batch_graph_launch). */
+int nebo_batch_graph_build(nebo_batch_t* b, int32_t steps, void* stream);
+int nebo_batch_graph_launch(nebo_batch_t* b, void* stream);
+/* Time candidate tiles on the device and keep the fastest; writes it to tile_out. */
+int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* adjoint of the var-coefficient diffusion step                              */
+/* ------------------------------------------------------------------------- */
+
+/* Forward explicit-Euler step on a periodic field:
+ *   fill_ghosts_periodic(phi); out = phi + dt*div(interp(k)*grad(phi))
+ * (interior), then fill_ghosts_periodic(out).  k must have periodic ghosts. */
+int nebo_diffusion_step(const nebo_field_t* out, const nebo_field_t* phi,
+                        const nebo_field_t* k, const nebo_axis_coefs_t* c,
+                        double dt, void* stream);
+
+/* Adjoint of nebo_diffusion_step (transposed Div∘Gradient with variable
+ * coefficient).  Given lam = dL/d(out) (periodic ghosts filled) and the step's
+ * input phi (periodic ghosts filled):
+ *   lam_out = lam + dt*A(k)^T lam        (dL/d phi)
+ *   gk     += dt * I^T( (D^T lam) .* (G phi) )   (dL/dk, interior, accumulated)
+ * lam_out gets periodic ghosts filled. */
+int nebo_diffusion_adjoint_step(const nebo_field_t* lam_out, double* gk,
+                                const nebo_field_t* lam, const nebo_field_t* phi,
+                                const nebo_field_t* k, const nebo_axis_coefs_t* c,
+                                double dt, void* stream);
+
+/* Whole trajectory: n_steps forward steps from phi0 with checkpoints every
+ * `every` steps into ckpt (n_steps/every + 1 fields laid out back to back,
+ * each with phi0's window), final state into phiT. */
+int nebo_diffusion_forward(const nebo_field_t* phiT, const nebo_field_t* phi0,
+                           const nebo_field_t* k, const nebo_axis_coefs_t* c,
+                           double dt, int32_t n_steps, int32_t every,
+                           double* ckpt, void* stream);
+
+/* Reverse sweep with checkpoint/recompute: lamT = dL/d phiT on entry.  On
+ * return lam0 = dL/d phi0 and gk = dL/dk (interior cells; ghosts zero).
+ * work must hold (every+2) fields of phi0's window. */
+int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk,
+                            const nebo_field_t* lamT, const nebo_field_t* k,
+                            const nebo_axis_coefs_t* c, double dt,
+                            int32_t n_steps, int32_t every, const double* ckpt,
+                            double* work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEBO_HIP_H */

@@ -1,0 +1,83 @@
+/*
+ * nebo_oracle.h -- CPU restatement of the reference's Nebo evaluation semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The engine never calls it.
+ *
+ * Descriptors are the POD types of include/nebo_hip.h, with every `values` /
+ * `bits` pointer a HOST pointer.  Each function names the reference lines it
+ * restates; see the parity notes at the top of nebo_oracle.c.
+ */
+#ifndef NEBO_ORACLE_H
+#define NEBO_ORACLE_H
+
+#include "../include/nebo_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_ghost { int32_t minus[3]; int32_t plus[3]; } oracle_ghost_t;
+
+/* ghosts_with_bc / ghosts_without_bc of the node tree (NeboRhs.h:52-54,183-188;
+ * NeboStencils.h:311-317; NeboCond.h:86-91,284-289; NeboMask.h ghosts). */
+int oracle_node_ghosts(const nebo_expr_t* e, int node, int with_bc, oracle_ghost_t* g);
+
+/* calculate_actual_ghost + assign limits (NeboBasic.h:61-100, NeboLhs.h:67-88). */
+int oracle_assign_limits(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
+                         nebo_limits_t* lim, int* empty);
+
+/* Value of node at (x,y,z) in the frame of `frame_field`'s valid-ghost-excluded
+ * origin (NeboConstField SeqWalk eval, NeboRhs.h:271-285). */
+double oracle_eval(const nebo_expr_t* e, int node, int x, int y, int z);
+
+/* lhs <<= rhs, serial z,y,x walk (NeboLhs.h:315-332, 611-623). */
+int oracle_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost);
+
+/* lhs <<= rhs with the reference's thread-parallel split (NeboLhs.h:335-393,
+ * NeboBasic.h:112-191): one task per partition, partitions along z first. */
+int oracle_assign_threads(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
+                          int nthreads);
+
+/* masked_assign: lhs(p) = rhs(p) for each point.  passes = 2 restates the
+ * CPU-only build, which runs the point loop twice (NeboLhs.h:172-190, 254-275). */
+int oracle_masked_assign(const nebo_field_t* lhs, const int32_t* pts, int n,
+                         const nebo_expr_t* rhs, int passes);
+
+/* nebo_fold (NeboReductions.h:45-97) with max/min/sum and norm = sqrt(sum(pow(x,2))). */
+int oracle_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out);
+
+/* Periodic copy-fill of all allocated ghost layers (x, then y, then z sweep). */
+void oracle_fill_ghosts_periodic(const nebo_field_t* f);
+
+/* Bit field of a point list over the mask's glob block (BitField.h:291-362). */
+void oracle_mask_bits(const nebo_mask_t* m, const int32_t* pts, int n, uint32_t* words);
+
+/* Closed forms (per-cell formulas, written independently of the tree walker). */
+void oracle_divkgrad(const nebo_field_t* r, const nebo_field_t* k, const nebo_field_t* phi,
+                     const nebo_axis_coefs_t* c, int form);
+void oracle_divkgrad_threads(const nebo_field_t* r, const nebo_field_t* k,
+                             const nebo_field_t* phi, const nebo_axis_coefs_t* c, int form,
+                             int nthreads);
+void oracle_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
+                       const nebo_axis_coefs_t* c, double s);
+
+/* Adjoint of the periodic var-coefficient diffusion step, built from separate
+ * operator applications (grad, interp, div and their transposes). */
+void oracle_diffusion_step(const nebo_field_t* out, const nebo_field_t* phi,
+                           const nebo_field_t* k, const nebo_axis_coefs_t* c, double dt);
+void oracle_diffusion_adjoint_step(const nebo_field_t* lam_out, double* gk,
+                                   const nebo_field_t* lam, const nebo_field_t* phi,
+                                   const nebo_field_t* k, const nebo_axis_coefs_t* c,
+                                   double dt);
+
+/* Deterministic inputs shared by CPU and GPU runs. */
+uint64_t oracle_splitmix64(uint64_t* state);
+void oracle_fill_uniform(double* dst, size_t n, uint64_t seed, double lo, double hi);
+void oracle_fill_normal(double* dst, size_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,641 @@
+// nebo_eval.hip -- the generic fused evaluator and the assignment / reduction
+// entry points.
+//
+// An expression tree is expanded on the host into a post-order stack program in
+// which every stencil point has been folded into the offsets of the loads
+// (nebo_lower.cpp).  The device walks that program once per cell.  The stack
+// depth before every instruction is known on the host and is part of the
+// opcode, so each (opcode, depth) pair is a separate case with fixed stack
+// registers: there is no dynamically indexed register file and no scratch.
+// The program lives in the kernel-argument segment, so instruction fetches are
+// uniform scalar loads and every branch is uniform across the wavefront.
+//
+// Each thread evaluates V consecutive z cells of one (x, y) column; the
+// (x, y) plane is flattened so that a wavefront covers 64 consecutive x (or
+// the rows of a narrow box) and loads stay coalesced.
+
+#include "nebo_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+namespace nebo {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kVmV = 2;           // z cells per thread in the box modes
+constexpr int kRedMaxBlocks = 2048;
+
+__device__ __noinline__ double vm_math(int op, double x)
+{
+  switch (op) {
+  case VM_EXP: return exp(x);
+  case VM_LOG: return log(x);
+  case VM_LOG10: return log10(x);
+  case VM_SIN: return sin(x);
+  case VM_COS: return cos(x);
+  case VM_TAN: return tan(x);
+  case VM_TANH: return tanh(x);
+  case VM_ERF: return erf(x);
+  case VM_ERFC: return erfc(x);
+  default: return x;
+  }
+}
+
+__device__ __noinline__ double vm_pow(double a, double b) { return pow(a, b); }
+
+template <int NCLS, int V>
+struct CellIdx {
+  int i[NCLS][V];   // flat index of each cell in each glob-shape class
+  int x, y;
+  int z[V];
+};
+
+template <int NCLS, int V>
+__device__ __forceinline__ int pick(const CellIdx<NCLS, V>& c, int cls, int v)
+{
+  if constexpr (NCLS == 1) {
+    return c.i[0][v];
+  } else {
+    int r = c.i[0][v];
+#pragma unroll
+    for (int k = 1; k < NCLS; ++k) r = (cls == k) ? c.i[k][v] : r;
+    return r;
+  }
+}
+
+template <int OP>
+constexpr int vm_pops()
+{
+  if constexpr (OP == VM_LOAD || OP == VM_CONST || OP == VM_MASK) return 0;
+  else if constexpr (OP >= VM_ADD && OP <= VM_OR) return 2;
+  else if constexpr (OP == VM_SELECT) return 3;
+  else return 1;
+}
+
+template <int OP, int D, int NCLS, int V, bool MATH>
+__device__ __forceinline__ void vm_exec(double (&S)[kVmMaxDepth][V], const VmProgram& P,
+                                        int imm, const CellIdx<NCLS, V>& c)
+{
+  constexpr int pops = vm_pops<OP>();
+  constexpr bool push = (OP == VM_LOAD || OP == VM_CONST || OP == VM_MASK);
+  constexpr bool math = (OP == VM_POW || OP == VM_POWC || (OP >= VM_EXP && OP <= VM_ERFC));
+  if constexpr (D < pops || (push && D >= kVmMaxDepth) || (math && !MATH)) {
+    return;  // never emitted by the host for this instantiation
+  } else if constexpr (OP == VM_LOAD) {
+    const double* b = P.loads[imm].base;
+    const int cls = P.loads[imm].cls;
+#pragma unroll
+    for (int v = 0; v < V; ++v) S[D][v] = b[pick(c, cls, v)];
+  } else if constexpr (OP == VM_CONST) {
+    const double k = P.consts[imm];
+#pragma unroll
+    for (int v = 0; v < V; ++v) S[D][v] = k;
+  } else if constexpr (OP == VM_MASK) {
+    const VmMask& m = P.masks[imm];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int pos = (m.o[0] + c.x) + m.X * ((m.o[1] + c.y) + m.Y * (m.o[2] + c.z[v]));
+      S[D][v] = ((m.bits[pos >> 5] >> (pos & 31)) & 1u) ? 1.0 : 0.0;
+    }
+  } else if constexpr (OP == VM_MULC) {
+    const double k = P.consts[imm];
+#pragma unroll
+    for (int v = 0; v < V; ++v) S[D - 1][v] = S[D - 1][v] * k;
+  } else if constexpr (pops == 2) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double a = S[D - 2][v], b = S[D - 1][v];
+      double r;
+      if constexpr (OP == VM_ADD) r = a + b;
+      else if constexpr (OP == VM_SUB) r = a - b;
+      else if constexpr (OP == VM_MUL) r = a * b;
+      else if constexpr (OP == VM_DIV) r = a / b;
+      else if constexpr (OP == VM_MAX) r = (a > b) ? a : b;
+      else if constexpr (OP == VM_MIN) r = (a < b) ? a : b;
+      else if constexpr (OP == VM_POW) r = vm_pow(a, b);
+      else if constexpr (OP == VM_EQ) r = (a == b) ? 1.0 : 0.0;
+      else if constexpr (OP == VM_NE) r = (a != b) ? 1.0 : 0.0;
+      else if constexpr (OP == VM_LT) r = (a < b) ? 1.0 : 0.0;
+      else if constexpr (OP == VM_LE) r = (a <= b) ? 1.0 : 0.0;
+      else if constexpr (OP == VM_GT) r = (a > b) ? 1.0 : 0.0;
+      else if constexpr (OP == VM_GE) r = (a >= b) ? 1.0 : 0.0;
+      else if constexpr (OP == VM_AND) r = (a != 0.0 && b != 0.0) ? 1.0 : 0.0;
+      else r = (a != 0.0 || b != 0.0) ? 1.0 : 0.0;  // VM_OR
+      S[D - 2][v] = r;
+    }
+  } else if constexpr (OP == VM_SELECT) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      S[D - 3][v] = (S[D - 3][v] != 0.0) ? S[D - 2][v] : S[D - 1][v];
+  } else {
+    // unary
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double a = S[D - 1][v];
+      double r;
+      if constexpr (OP == VM_NEG) r = -a;
+      else if constexpr (OP == VM_ABS) r = fabs(a);
+      else if constexpr (OP == VM_SQRT) r = __builtin_sqrt(a);
+      else if constexpr (OP == VM_SQUARE) r = a * a;
+      else if constexpr (OP == VM_CUBE) r = a * a * a;
+      else if constexpr (OP == VM_POWC) r = vm_pow(a, P.consts[imm]);
+      else if constexpr (OP == VM_NOT) r = (a != 0.0) ? 0.0 : 1.0;
+      else r = vm_math(OP, a);
+      S[D - 1][v] = r;
+    }
+  }
+}
+
+#define VM_C(OP, D) \
+  case ((OP) << 4) | (D): vm_exec<OP, D, NCLS, V, MATH>(S, P, imm, c); break;
+#define VM_CD(OP)                                                                      \
+  VM_C(OP, 0) VM_C(OP, 1) VM_C(OP, 2) VM_C(OP, 3) VM_C(OP, 4) VM_C(OP, 5) VM_C(OP, 6) \
+  VM_C(OP, 7) VM_C(OP, 8) VM_C(OP, 9) VM_C(OP, 10)
+#define VM_ALL_CASES                                                                    \
+  VM_CD(VM_LOAD) VM_CD(VM_CONST) VM_CD(VM_MASK) VM_CD(VM_MULC) VM_CD(VM_ADD)           \
+  VM_CD(VM_SUB) VM_CD(VM_MUL) VM_CD(VM_DIV) VM_CD(VM_MAX) VM_CD(VM_MIN) VM_CD(VM_POW)  \
+  VM_CD(VM_EQ) VM_CD(VM_NE) VM_CD(VM_LT) VM_CD(VM_LE) VM_CD(VM_GT) VM_CD(VM_GE)        \
+  VM_CD(VM_AND) VM_CD(VM_OR) VM_CD(VM_NEG) VM_CD(VM_ABS) VM_CD(VM_SQRT)                \
+  VM_CD(VM_SQUARE) VM_CD(VM_CUBE) VM_CD(VM_POWC) VM_CD(VM_EXP) VM_CD(VM_LOG)           \
+  VM_CD(VM_LOG10) VM_CD(VM_SIN) VM_CD(VM_COS) VM_CD(VM_TAN) VM_CD(VM_TANH)             \
+  VM_CD(VM_ERF) VM_CD(VM_ERFC) VM_CD(VM_NOT) VM_CD(VM_SELECT)
+
+static_assert(kVmMaxDepth == 10, "VM_CD covers depths 0..10");
+static_assert(VM_NUM_OPS == 36, "VM_ALL_CASES lists every opcode");
+
+template <int NCLS, int V, bool MATH>
+__device__ __forceinline__ void vm_run(const VmProgram& P, double (&S)[kVmMaxDepth][V],
+                                       const CellIdx<NCLS, V>& c)
+{
+  const int n = P.n_insns;
+  for (int pc = 0; pc < n; ++pc) {
+    const VmInsn in = P.insns[pc];
+    const int imm = in.imm;
+    switch (in.code) {
+      VM_ALL_CASES
+    default: break;
+    }
+  }
+}
+
+struct Box {
+  int lo[3];
+  int n[3];        // extent of the box per axis
+  int nplane;      // n[0] * n[1]
+  int zchunks;     // ceil(n[2] / V)
+};
+
+template <int NCLS, int V>
+__device__ __forceinline__ void make_cell(const VmProgram& P, const Box& B, int t, int zc,
+                                          CellIdx<NCLS, V>& c, int* nv)
+{
+  const int yy = t / B.n[0];
+  c.x = B.lo[0] + (t - yy * B.n[0]);
+  c.y = B.lo[1] + yy;
+  const int z0 = B.lo[2] + zc * V;
+  const int left = B.lo[2] + B.n[2] - z0;
+  *nv = left < V ? left : V;
+#pragma unroll
+  for (int v = 0; v < V; ++v) c.z[v] = z0 + (v < *nv ? v : *nv - 1);  // clamp: stays in bounds
+#pragma unroll
+  for (int k = 0; k < NCLS; ++k) {
+    const int X = P.cls_X[k], XY = P.cls_XY[k];
+#pragma unroll
+    for (int v = 0; v < V; ++v) c.i[k][v] = c.x + X * c.y + XY * c.z[v];
+  }
+}
+
+template <int NCLS, int V, bool MATH>
+__global__ __launch_bounds__(kBlock) void vm_assign_kernel(const VmProgram P, Box B,
+                                                           double* __restrict__ out, int oX,
+                                                           int oXY)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= B.nplane) return;
+  CellIdx<NCLS, V> c;
+  int nv;
+  make_cell<NCLS, V>(P, B, t, blockIdx.y, c, &nv);
+  double S[kVmMaxDepth][V];
+  vm_run<NCLS, V, MATH>(P, S, c);
+  const int base = c.x + oX * c.y;
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    if (v < nv) out[base + oXY * c.z[v]] = S[0][v];
+}
+
+template <int NCLS, bool MATH>
+__global__ __launch_bounds__(kBlock) void vm_masked_kernel(const VmProgram P,
+                                                           const int32_t* __restrict__ pts,
+                                                           int npts, double* __restrict__ out,
+                                                           int oX, int oXY)
+{
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= npts) return;
+  CellIdx<NCLS, 1> c;
+  c.x = pts[3 * t];
+  c.y = pts[3 * t + 1];
+  c.z[0] = pts[3 * t + 2];
+#pragma unroll
+  for (int k = 0; k < NCLS; ++k) c.i[k][0] = c.x + P.cls_X[k] * c.y + P.cls_XY[k] * c.z[0];
+  double S[kVmMaxDepth][1];
+  vm_run<NCLS, 1, MATH>(P, S, c);
+  out[c.x + oX * c.y + oXY * c.z[0]] = S[0][0];
+}
+
+// ---- reductions ----------------------------------------------------------
+// max/min use the reference's select forms ((a < b) ? b : a and (a > b) ? b : a),
+// which skip NaN operands the same way the serial fold does.  Sums carry a
+// double-double (hi, lo) pair built from error-free TwoSum steps.
+
+struct DD { double hi, lo; };
+
+__device__ __forceinline__ DD dd_add(DD a, double b)
+{
+  const double s = a.hi + b;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b - bb);
+  return DD{s, a.lo + e};
+}
+
+__device__ __forceinline__ DD dd_merge(DD a, DD b)
+{
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double e = ((a.hi - (s - bb)) + (b.hi - bb)) + (a.lo + b.lo);
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
+}
+
+__device__ __forceinline__ double red_max(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double red_min(double a, double b) { return (a > b) ? b : a; }
+
+template <int OP>
+__device__ __forceinline__ DD red_combine(DD a, DD b)
+{
+  if constexpr (OP == NEBO_RED_MAX) return DD{red_max(a.hi, b.hi), 0.0};
+  else if constexpr (OP == NEBO_RED_MIN) return DD{red_min(a.hi, b.hi), 0.0};
+  else return dd_merge(a, b);
+}
+
+template <int OP>
+__device__ __forceinline__ DD red_init()
+{
+  if constexpr (OP == NEBO_RED_MAX) return DD{-INFINITY, 0.0};
+  else if constexpr (OP == NEBO_RED_MIN) return DD{INFINITY, 0.0};
+  else return DD{0.0, 0.0};
+}
+
+// Block-wide fixed-order reduction; returns the total in thread 0.
+template <int OP>
+__device__ DD block_reduce(DD v)
+{
+  __shared__ double sh_hi[kBlock / 64], sh_lo[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    DD o{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
+    v = red_combine<OP>(v, o);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { sh_hi[wave] = v.hi; sh_lo[wave] = v.lo; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DD r{sh_hi[0], sh_lo[0]};
+    for (int w = 1; w < kBlock / 64; ++w) r = red_combine<OP>(r, DD{sh_hi[w], sh_lo[w]});
+    v = r;
+  }
+  return v;
+}
+
+template <int NCLS, int V, int OP, bool MATH>
+__global__ __launch_bounds__(kBlock) void vm_reduce_kernel(const VmProgram P, Box B,
+                                                           double* __restrict__ partial)
+{
+  DD acc = red_init<OP>();
+  const long items = (long)B.nplane * B.zchunks;
+  for (long w = (long)blockIdx.x * kBlock + threadIdx.x; w < items;
+       w += (long)gridDim.x * kBlock) {
+    const int zc = (int)(w / B.nplane);
+    const int t = (int)(w - (long)zc * B.nplane);
+    CellIdx<NCLS, V> c;
+    int nv;
+    make_cell<NCLS, V>(P, B, t, zc, c, &nv);
+    double S[kVmMaxDepth][V];
+    vm_run<NCLS, V, MATH>(P, S, c);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      if (v < nv) {
+        const double x = S[0][v];
+        if constexpr (OP == NEBO_RED_MAX) acc.hi = red_max(acc.hi, x);
+        else if constexpr (OP == NEBO_RED_MIN) acc.hi = red_min(acc.hi, x);
+        else if constexpr (OP == NEBO_RED_SUM) acc = dd_add(acc, x);
+        else {
+          // x*x exactly as p + err (Dekker split; no FMA so the result is portable)
+          const double p = x * x;
+          const double t2 = 134217729.0 * x;
+          const double xh = t2 - (t2 - x), xl = x - xh;
+          const double err = ((xh * xh - p) + 2.0 * xh * xl) + xl * xl;
+          acc = dd_add(acc, p);
+          acc.lo = acc.lo + err;
+        }
+      }
+    }
+  }
+  DD r = block_reduce<OP>(acc);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = r.hi;
+    partial[2 * blockIdx.x + 1] = r.lo;
+  }
+}
+
+template <int OP>
+__global__ __launch_bounds__(kBlock) void reduce_final_kernel(const double* __restrict__ partial,
+                                                              int n, double* __restrict__ out)
+{
+  DD acc = red_init<OP>();
+  // fixed assignment of partials to threads, then a fixed tree: deterministic
+  for (int i = threadIdx.x; i < n; i += kBlock)
+    acc = red_combine<OP>(acc, DD{partial[2 * i], partial[2 * i + 1]});
+  DD r = block_reduce<OP>(acc);
+  if (threadIdx.x == 0) {
+    double v = r.hi + r.lo;
+    if constexpr (OP == NEBO_RED_NORM) v = __builtin_sqrt(v);
+    *out = v;
+  }
+}
+
+Box make_box(const nebo_limits_t& L, int V)
+{
+  Box B;
+  for (int d = 0; d < 3; ++d) { B.lo[d] = L.lo[d]; B.n[d] = L.hi[d] - L.lo[d]; }
+  B.nplane = B.n[0] * B.n[1];
+  B.zchunks = (B.n[2] + V - 1) / V;
+  return B;
+}
+
+// Every field the program reads, and the LHS, must be addressable with 32-bit
+// flat indices.
+int check_index_range(const nebo_expr_t* e, const nebo_field_t* lhs, const char* ctx)
+{
+  auto big = [](const int32_t g[3]) { return (long)g[0] * g[1] * g[2] >= (1L << 31); };
+  if (lhs && big(lhs->glob)) return set_error(NEBO_ERR_TOO_LARGE, ctx, "field has 2^31 or more points");
+  for (int i = 0; i < e->n_fields; ++i)
+    if (big(e->fields[i].glob)) return set_error(NEBO_ERR_TOO_LARGE, ctx, "field has 2^31 or more points");
+  return NEBO_OK;
+}
+
+// Operands must live on the LHS device (NeboLhs.h:107-153 error texts).
+int check_devices(const nebo_expr_t* e, int device, const char* ctx)
+{
+  if (device < 0)
+    return set_error(NEBO_ERR_DEVICE, ctx,
+                     "Left-hand side of assignment allocated on CPU; this engine evaluates on GPU "
+                     "only (the CPU backend is the reference's SeqWalk loop)");
+  for (int i = 0; i < e->n_fields; ++i)
+    if (e->fields[i].device != device)
+      return set_error(NEBO_ERR_DEVICE, ctx,
+                       "Left-hand side of assignment allocated on GPU but right-hand side is not "
+                       "(completely) accessible on the same GPU");
+  return NEBO_OK;
+}
+
+bool subtree_reads(const nebo_expr_t* e, int node, const void* values)
+{
+  const nebo_node_t& n = e->nodes[node];
+  switch (n.op) {
+  case NEBO_SCALAR: case NEBO_MASK: return false;
+  case NEBO_FIELD: return e->fields[n.idx].values == values;
+  default:
+    if (subtree_reads(e, n.a, values)) return true;
+    if (n.op == NEBO_STENCIL || n.op == NEBO_NEG || n.op == NEBO_ABS || n.op == NEBO_SQRT ||
+        n.op == NEBO_SQUARE || n.op == NEBO_CUBE || n.op == NEBO_POWINT || n.op == NEBO_NOT ||
+        (n.op >= NEBO_EXP && n.op <= NEBO_ERFC))
+      return false;
+    if (subtree_reads(e, n.b, values)) return true;
+    return n.op == NEBO_COND && subtree_reads(e, n.c, values);
+  }
+}
+
+// An in-place assignment may read the LHS only at the point it writes.
+int check_aliasing(const nebo_field_t* lhs, const nebo_expr_t* e, const char* ctx)
+{
+  for (int i = 0; i < e->n_nodes; ++i) {
+    const nebo_node_t& s = e->nodes[i];
+    if (s.op != NEBO_STENCIL) continue;
+    if (subtree_reads(e, s.a, lhs->values))
+      return set_error(NEBO_ERR_INVALID, ctx,
+                       "left-hand side is read by a stencil on the right-hand side; neighbouring "
+                       "points would be read after they are written (assign into another field)");
+  }
+  return NEBO_OK;
+}
+
+}  // namespace
+
+bool uses_math(const VmProgram& P)
+{
+  for (int i = 0; i < P.n_insns; ++i) {
+    const int op = P.insns[i].code >> 4;
+    if (op == VM_POW || op == VM_POWC || (op >= VM_EXP && op <= VM_ERFC)) return true;
+  }
+  return false;
+}
+
+int launch_generic_assign(const nebo_field_t* lhs, const VmProgram& prog, int max_depth,
+                          const nebo_limits_t& lim, void* stream)
+{
+  (void)max_depth;
+  Box B = make_box(lim, kVmV);
+  dim3 grid((B.nplane + kBlock - 1) / kBlock, B.zchunks);
+  double* out = frame_origin(lhs);
+  const int oX = lhs->glob[0], oXY = lhs->glob[0] * lhs->glob[1];
+  hipStream_t s = (hipStream_t)stream;
+  const bool one = prog.n_classes == 1, math = uses_math(prog);
+  if (one && !math)
+    hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
+  else if (one)
+    hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, true>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
+  else if (!math)
+    hipLaunchKernelGGL((vm_assign_kernel<kVmMaxClasses, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
+  else
+    hipLaunchKernelGGL((vm_assign_kernel<kVmMaxClasses, kVmV, true>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
+  return check_hip(hipGetLastError(), "Nebo Assignment");
+}
+
+namespace {
+
+int do_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, void* stream,
+              bool allow_special)
+{
+  static const char* ctx = "Nebo Assignment";
+  clear_error();
+  if (!lhs || !rhs) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  nebo_limits_t lim;
+  int empty = 0;
+  int rc = assign_limits(lhs, rhs, use_ghost, &lim, &empty);
+  if (rc) return rc;
+  if ((rc = check_devices(rhs, lhs->device, ctx))) return rc;
+  if ((rc = check_index_range(rhs, lhs, ctx))) return rc;
+  if ((rc = check_aliasing(lhs, rhs, ctx))) return rc;
+  if (empty) return NEBO_OK;
+  if (allow_special) {
+    int kid = K_GENERIC;
+    if ((rc = nebo_assign_plan(lhs, rhs, use_ghost, &kid))) return rc;
+    if (kid == K_DIVKGRAD) {
+      DivKGradMatch m;
+      match_divkgrad(rhs, &m);
+      return launch_divkgrad(lhs, &rhs->fields[m.k_field], &rhs->fields[m.phi_field], m.c,
+                             m.form, lim, stream);
+    }
+    if (kid == K_LAPL_EULER) {
+      LaplEulerMatch m;
+      match_lapl_euler(rhs, &m);
+      return launch_lapl_euler(lhs, &rhs->fields[m.phi_field], m.c, m.s, lim, stream);
+    }
+  }
+  VmProgram prog;
+  int depth = 0;
+  if ((rc = lower_program(rhs, &prog, &depth))) return rc;
+  return launch_generic_assign(lhs, prog, depth, lim, stream);
+}
+
+template <int NCLS, int OP, bool MATH>
+void launch_reduce_op(const VmProgram& prog, const Box& B, int blocks, double* partial,
+                      double* out_dev, hipStream_t s)
+{
+  hipLaunchKernelGGL((vm_reduce_kernel<NCLS, kVmV, OP, MATH>), dim3(blocks), dim3(kBlock), 0, s, prog, B, partial);
+  hipLaunchKernelGGL((reduce_final_kernel<OP>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev);
+}
+
+template <int NCLS, bool MATH>
+void launch_reduce_any(const VmProgram& prog, const Box& B, int op, int blocks, double* partial,
+                       double* out_dev, hipStream_t s)
+{
+  switch (op) {
+  case NEBO_RED_MAX: launch_reduce_op<NCLS, NEBO_RED_MAX, MATH>(prog, B, blocks, partial, out_dev, s); break;
+  case NEBO_RED_MIN: launch_reduce_op<NCLS, NEBO_RED_MIN, MATH>(prog, B, blocks, partial, out_dev, s); break;
+  case NEBO_RED_SUM: launch_reduce_op<NCLS, NEBO_RED_SUM, MATH>(prog, B, blocks, partial, out_dev, s); break;
+  default: launch_reduce_op<NCLS, NEBO_RED_NORM, MATH>(prog, B, blocks, partial, out_dev, s); break;
+  }
+}
+
+int launch_reduce(const VmProgram& prog, const Box& B, int op, double* out_dev, hipStream_t s)
+{
+  const long items = (long)B.nplane * B.zchunks;
+  int blocks = (int)((items + kBlock - 1) / kBlock);
+  if (blocks > kRedMaxBlocks) blocks = kRedMaxBlocks;
+  if (blocks < 1) blocks = 1;
+  double* partial = nullptr;
+  int rc = check_hip(hipMallocAsync((void**)&partial, sizeof(double) * 2 * blocks, s), "Nebo Reduction");
+  if (rc) return rc;
+  const bool one = prog.n_classes == 1, math = uses_math(prog);
+  if (one && !math) launch_reduce_any<1, false>(prog, B, op, blocks, partial, out_dev, s);
+  else if (one) launch_reduce_any<1, true>(prog, B, op, blocks, partial, out_dev, s);
+  else if (!math) launch_reduce_any<kVmMaxClasses, false>(prog, B, op, blocks, partial, out_dev, s);
+  else launch_reduce_any<kVmMaxClasses, true>(prog, B, op, blocks, partial, out_dev, s);
+  rc = check_hip(hipGetLastError(), "Nebo Reduction");
+  (void)hipFreeAsync(partial, s);
+  return rc;
+}
+
+__global__ void fill_value_kernel(double* out, double v) { *out = v; }
+
+int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void* stream)
+{
+  static const char* ctx = "Nebo Reduction";
+  if (!e || !out_dev) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  if (op < NEBO_RED_MAX || op > NEBO_RED_NORM) return set_error(NEBO_ERR_INVALID, ctx, "unknown reduction");
+  nebo_limits_t lim;
+  int empty = 0;
+  int rc = reduce_limits(e, use_ghost, &lim, &empty);
+  if (rc) return rc;
+  int device = -1;
+  for (int i = 0; i < e->n_fields; ++i) { device = e->fields[i].device; break; }
+  if (e->n_fields > 0 && (rc = check_devices(e, device, ctx))) return rc;
+  if ((rc = check_index_range(e, nullptr, ctx))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (empty) {
+    const double v = op == NEBO_RED_MAX ? -INFINITY : op == NEBO_RED_MIN ? INFINITY : 0.0;
+    hipLaunchKernelGGL(fill_value_kernel, dim3(1), dim3(1), 0, s, out_dev, v);
+    return check_hip(hipGetLastError(), ctx);
+  }
+  VmProgram prog;
+  int depth = 0;
+  if ((rc = lower_program(e, &prog, &depth))) return rc;
+  Box B = make_box(lim, kVmV);
+  return launch_reduce(prog, B, op, out_dev, s);
+}
+
+}  // namespace
+}  // namespace nebo
+
+using namespace nebo;
+
+extern "C" int nebo_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
+                           void* stream)
+{
+  return do_assign(lhs, rhs, use_ghost, stream, true);
+}
+
+extern "C" int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
+                                   int use_ghost, void* stream)
+{
+  return do_assign(lhs, rhs, use_ghost, stream, false);
+}
+
+extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points_dev,
+                                  int32_t n_points, const nebo_expr_t* rhs, void* stream)
+{
+  static const char* ctx = "Nebo Masked Assignment";
+  clear_error();
+  if (!lhs || !rhs || (n_points > 0 && !points_dev))
+    return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  int rc = validate_expr(rhs, ctx);
+  if (rc) return rc;
+  if ((rc = check_devices(rhs, lhs->device, ctx))) return rc;
+  if ((rc = check_index_range(rhs, lhs, ctx))) return rc;
+  if ((rc = check_aliasing(lhs, rhs, ctx))) return rc;
+  if (n_points <= 0) return NEBO_OK;
+  VmProgram prog;
+  int depth = 0;
+  if ((rc = lower_program(rhs, &prog, &depth))) return rc;
+  double* out = frame_origin(lhs);
+  const int oX = lhs->glob[0], oXY = lhs->glob[0] * lhs->glob[1];
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid((n_points + kBlock - 1) / kBlock);
+  const bool one = prog.n_classes == 1, math = uses_math(prog);
+  if (one && !math)
+    hipLaunchKernelGGL((vm_masked_kernel<1, false>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+  else if (one)
+    hipLaunchKernelGGL((vm_masked_kernel<1, true>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+  else if (!math)
+    hipLaunchKernelGGL((vm_masked_kernel<kVmMaxClasses, false>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+  else
+    hipLaunchKernelGGL((vm_masked_kernel<kVmMaxClasses, true>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+  return check_hip(hipGetLastError(), ctx);
+}
+
+extern "C" int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
+                                 double* out_dev, void* stream)
+{
+  clear_error();
+  return do_reduce(expr, op, use_ghost, out_dev, stream);
+}
+
+extern "C" int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
+                           void* stream)
+{
+  clear_error();
+  if (!out) return set_error(NEBO_ERR_INVALID, "Nebo Reduction", "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  double* dev = nullptr;
+  int rc = check_hip(hipMallocAsync((void**)&dev, sizeof(double), s), "Nebo Reduction");
+  if (rc) return rc;
+  rc = do_reduce(expr, op, use_ghost, dev, stream);
+  if (!rc) rc = check_hip(hipMemcpyAsync(out, dev, sizeof(double), hipMemcpyDeviceToHost, s), "Nebo Reduction");
+  (void)hipFreeAsync(dev, s);
+  if (!rc) rc = check_hip(hipStreamSynchronize(s), "Nebo Reduction");
+  return rc;
+}

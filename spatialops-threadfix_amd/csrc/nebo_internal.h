@@ -1,0 +1,141 @@
+// nebo_internal.h -- shared declarations of the engine's host and device code.
+#pragma once
+
+#include "../../include/nebo_hip.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nebo {
+
+// ---------------------------------------------------------------------------
+// errors: thread-local message, "Nebo error in <ctx>:\n<msg>" like the reference
+// ---------------------------------------------------------------------------
+int set_error(int code, const char* ctx, const std::string& msg);
+void clear_error();
+
+// ---------------------------------------------------------------------------
+// ghost algebra (GhostData.h:54-257): six counts; all >= NEBO_GHOST_MAX = infinite
+// ---------------------------------------------------------------------------
+struct Ghost {
+  int m[3];
+  int p[3];
+  bool infinite() const {
+    for (int d = 0; d < 3; ++d)
+      if (m[d] < NEBO_GHOST_MAX || p[d] < NEBO_GHOST_MAX) return false;
+    return true;
+  }
+  static Ghost uniform(int n) { return Ghost{{n, n, n}, {n, n, n}}; }
+};
+
+// Ghost counts of a node (ghosts_with_bc when with_bc, else ghosts_without_bc).
+int node_ghosts(const nebo_expr_t* e, int node, bool with_bc, Ghost* out);
+// Validate node indices, opcodes and table indices of an expression.
+int validate_expr(const nebo_expr_t* e, const char* ctx);
+// Loop limits of lhs <<= rhs (NeboBasic.h:61-100, NeboLhs.h:67-88).
+int assign_limits(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
+                  nebo_limits_t* lim, int* empty);
+// Loop limits of a reduction (NeboReductions.h:54-70).
+int reduce_limits(const nebo_expr_t* e, int use_ghost, nebo_limits_t* lim, int* empty);
+
+// ---------------------------------------------------------------------------
+// generic fused evaluator: postfix stack program with host-known stack depth
+// ---------------------------------------------------------------------------
+// Opcodes of the device stack machine.  The device dispatches on
+// (op << 4) | depth, so every case reads and writes fixed stack slots.
+enum VmOp : uint8_t {
+  VM_LOAD = 0,   // push field value          imm = load slot
+  VM_CONST,      // push constant             imm = const index
+  VM_MASK,       // push mask bit (0/1)       imm = mask slot
+  VM_MULC,       // top = top * const[imm]    (stencil coefficient)
+  VM_ADD, VM_SUB, VM_MUL, VM_DIV, VM_MAX, VM_MIN, VM_POW,
+  VM_EQ, VM_NE, VM_LT, VM_LE, VM_GT, VM_GE, VM_AND, VM_OR,
+  VM_NEG, VM_ABS, VM_SQRT, VM_SQUARE, VM_CUBE, VM_POWC,  // POWC: pow(top, const[imm])
+  VM_EXP, VM_LOG, VM_LOG10, VM_SIN, VM_COS, VM_TAN, VM_TANH, VM_ERF, VM_ERFC, VM_NOT,
+  VM_SELECT,     // c = pop, b = pop, a = pop, push (a != 0 ? b : c)
+  VM_NUM_OPS
+};
+
+constexpr int kVmMaxDepth = 10;     // stack slots
+constexpr int kVmMaxInsns = 200;
+constexpr int kVmMaxConsts = 48;
+constexpr int kVmMaxLoads = 40;
+constexpr int kVmMaxMasks = 6;
+constexpr int kVmMaxClasses = 4;    // distinct (glob x, glob y) shapes
+
+struct VmInsn {
+  uint16_t code;   // (op << 4) | depth before the instruction
+  int16_t imm;
+};
+
+struct VmLoad {
+  const double* base;   // values + flat(offset + valid_minus + shift)
+  int32_t cls;          // index class (glob shape)
+  int32_t pad_;
+};
+
+struct VmMask {
+  const uint32_t* bits;
+  int32_t o[3];         // offset + ghost_minus + shift
+  int32_t X, Y;
+  int32_t pad_;
+};
+
+// Kernel-argument image of one lowered expression.
+struct VmProgram {
+  int32_t n_insns;
+  int32_t n_classes;
+  int32_t cls_X[kVmMaxClasses];
+  int32_t cls_XY[kVmMaxClasses];
+  VmLoad loads[kVmMaxLoads];
+  VmMask masks[kVmMaxMasks];
+  double consts[kVmMaxConsts];
+  VmInsn insns[kVmMaxInsns];
+};
+
+// Lower a tree to a stack program.  Also records the max stack depth.
+int lower_program(const nebo_expr_t* e, VmProgram* prog, int* max_depth);
+
+// ---------------------------------------------------------------------------
+// recognised trees
+// ---------------------------------------------------------------------------
+enum KernelId { K_GENERIC = 0, K_DIVKGRAD = 1, K_LAPL_EULER = 2 };
+
+struct DivKGradMatch {
+  int k_field, phi_field;
+  int form;                       // 0: (Dx+Dy)+Dz, 1: ((-Dx)-Dy)-Dz
+  nebo_axis_coefs_t c;
+};
+struct LaplEulerMatch {
+  int phi_field;                  // read field (centre term and stencils)
+  double s;                       // scalar factor
+  nebo_axis_coefs_t c;
+};
+bool match_divkgrad(const nebo_expr_t* e, DivKGradMatch* m);
+bool match_lapl_euler(const nebo_expr_t* e, LaplEulerMatch* m);
+
+// ---------------------------------------------------------------------------
+// launch helpers (device code)
+// ---------------------------------------------------------------------------
+int launch_generic_assign(const nebo_field_t* lhs, const VmProgram& prog, int max_depth,
+                          const nebo_limits_t& lim, void* stream);
+int launch_divkgrad(const nebo_field_t* r, const nebo_field_t* k, const nebo_field_t* phi,
+                    const nebo_axis_coefs_t& c, int form, const nebo_limits_t& lim,
+                    void* stream);
+int launch_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
+                      const nebo_axis_coefs_t& c, double s, const nebo_limits_t& lim,
+                      void* stream);
+
+// Pointer of the first valid-ghost-excluded point of a field (the frame origin).
+inline double* frame_origin(const nebo_field_t* f)
+{
+  long gx = f->offset[0] + f->valid_minus[0];
+  long gy = f->offset[1] + f->valid_minus[1];
+  long gz = f->offset[2] + f->valid_minus[2];
+  return static_cast<double*>(f->values) + gx + (long)f->glob[0] * (gy + (long)f->glob[1] * gz);
+}
+
+int check_hip(int hip_status, const char* ctx);
+
+}  // namespace nebo

@@ -1,0 +1,33 @@
+// nebo_pending.cpp -- entry points whose kernels are not built yet: they fail
+// loudly with NEBO_ERR_UNSUPPORTED instead of computing anything.
+#include "nebo_internal.h"
+
+using namespace nebo;
+
+static int pending(const char* what)
+{
+  clear_error();
+  return set_error(NEBO_ERR_UNSUPPORTED, what, "not provided by this build");
+}
+
+extern "C" {
+int nebo_batch_create(const nebo_patch_t*, int32_t, const nebo_axis_coefs_t*, double,
+                      const int32_t*, nebo_batch_t**) { return pending("Nebo Batch"); }
+int nebo_batch_destroy(nebo_batch_t*) { return pending("Nebo Batch"); }
+int nebo_batch_step(nebo_batch_t*, int, void*) { return pending("Nebo Batch"); }
+int nebo_batch_step_per_patch(nebo_batch_t*, int, void*) { return pending("Nebo Batch"); }
+int nebo_batch_graph_build(nebo_batch_t*, int32_t, void*) { return pending("Nebo Batch"); }
+int nebo_batch_graph_launch(nebo_batch_t*, void*) { return pending("Nebo Batch"); }
+int nebo_batch_autotune(nebo_batch_t*, int32_t*, void*) { return pending("Nebo Batch"); }
+int nebo_diffusion_step(const nebo_field_t*, const nebo_field_t*, const nebo_field_t*,
+                        const nebo_axis_coefs_t*, double, void*) { return pending("Nebo Adjoint"); }
+int nebo_diffusion_adjoint_step(const nebo_field_t*, double*, const nebo_field_t*,
+                                const nebo_field_t*, const nebo_field_t*,
+                                const nebo_axis_coefs_t*, double, void*) { return pending("Nebo Adjoint"); }
+int nebo_diffusion_forward(const nebo_field_t*, const nebo_field_t*, const nebo_field_t*,
+                           const nebo_axis_coefs_t*, double, int32_t, int32_t, double*,
+                           void*) { return pending("Nebo Adjoint"); }
+int nebo_diffusion_backward(const nebo_field_t*, double*, const nebo_field_t*,
+                            const nebo_field_t*, const nebo_axis_coefs_t*, double, int32_t,
+                            int32_t, const double*, double*, void*) { return pending("Nebo Adjoint"); }
+}

@@ -1,0 +1,212 @@
+// nebo_runtime.hip -- errors, device selection, the device memory pool and copies.
+//
+// The pool replaces Pool<T>::get/put (spatialops/structured/MemoryPool.cpp:102-186):
+// freed blocks are kept in per-size free lists and handed out again for requests
+// of the same size, so a time loop that creates temporaries allocates only once.
+
+#include "nebo_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <unordered_map>
+#include <vector>
+
+namespace nebo {
+
+namespace {
+thread_local std::string t_error;
+}
+
+int set_error(int code, const char* ctx, const std::string& msg)
+{
+  std::ostringstream s;
+  s << "Nebo error in " << ctx << ":\n" << msg;
+  t_error = s.str();
+  return code;
+}
+
+void clear_error() { t_error.clear(); }
+
+int check_hip(int st, const char* ctx)
+{
+  if (st == hipSuccess) return NEBO_OK;
+  return set_error(NEBO_ERR_HIP, ctx, std::string("\t - ") + hipGetErrorString((hipError_t)st));
+}
+
+namespace {
+
+struct DevicePool {
+  std::unordered_map<size_t, std::vector<void*>> free_lists;
+  std::unordered_map<void*, size_t> live;
+  size_t in_use = 0;
+  size_t high_water = 0;
+};
+
+std::mutex g_pool_mutex;
+std::map<int, DevicePool> g_pools;
+
+// Run f with `device` current, restoring the caller's device afterwards.
+template <class F>
+int with_device(int device, const char* ctx, F f)
+{
+  int cur = 0;
+  int rc = check_hip(hipGetDevice(&cur), ctx);
+  if (rc) return rc;
+  if (cur != device && (rc = check_hip(hipSetDevice(device), ctx))) return rc;
+  rc = f();
+  if (cur != device) (void)hipSetDevice(cur);
+  return rc;
+}
+
+}  // namespace
+}  // namespace nebo
+
+using namespace nebo;
+
+extern "C" const char* nebo_last_error(void) { return t_error.c_str(); }
+
+extern "C" int nebo_device_count(int* count)
+{
+  clear_error();
+  if (!count) return set_error(NEBO_ERR_INVALID, "Nebo Device", "null argument");
+  *count = 0;
+  int n = 0;
+  hipError_t st = hipGetDeviceCount(&n);
+  if (st == hipErrorNoDevice) { (void)hipGetLastError(); return NEBO_OK; }
+  int rc = check_hip(st, "Nebo Device");
+  if (rc) return rc;
+  *count = n;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_init(int device)
+{
+  clear_error();
+  int n = 0;
+  int rc = nebo_device_count(&n);
+  if (rc) return rc;
+  if (device < 0 || device >= n) {
+    std::ostringstream m;
+    m << "device index " << device << " is not a GPU of this host (" << n << " visible)";
+    return set_error(NEBO_ERR_DEVICE, "Nebo Device", m.str());
+  }
+  if ((rc = check_hip(hipSetDevice(device), "Nebo Device"))) return rc;
+  return check_hip(hipFree(nullptr), "Nebo Device");
+}
+
+extern "C" int nebo_alloc(int device, size_t bytes, void** out)
+{
+  clear_error();
+  if (!out) return set_error(NEBO_ERR_INVALID, "Nebo Memory", "null argument");
+  *out = nullptr;
+  if (bytes == 0) bytes = 8;
+  std::lock_guard<std::mutex> lock(g_pool_mutex);
+  DevicePool& pool = g_pools[device];
+  auto it = pool.free_lists.find(bytes);
+  void* p = nullptr;
+  if (it != pool.free_lists.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+  } else {
+    int rc = with_device(device, "Nebo Memory", [&] {
+      return check_hip(hipMalloc(&p, bytes), "Nebo Memory");
+    });
+    if (rc) return rc;
+  }
+  pool.live[p] = bytes;
+  pool.in_use += bytes;
+  if (pool.in_use > pool.high_water) pool.high_water = pool.in_use;
+  *out = p;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_free(int device, void* ptr)
+{
+  clear_error();
+  if (!ptr) return NEBO_OK;
+  std::lock_guard<std::mutex> lock(g_pool_mutex);
+  DevicePool& pool = g_pools[device];
+  auto it = pool.live.find(ptr);
+  if (it == pool.live.end())
+    return set_error(NEBO_ERR_INVALID, "Nebo Memory", "pointer was not allocated by nebo_alloc on this device");
+  pool.in_use -= it->second;
+  pool.free_lists[it->second].push_back(ptr);
+  pool.live.erase(it);
+  return NEBO_OK;
+}
+
+extern "C" int nebo_pool_trim(int device)
+{
+  clear_error();
+  std::lock_guard<std::mutex> lock(g_pool_mutex);
+  DevicePool& pool = g_pools[device];
+  return with_device(device, "Nebo Memory", [&] {
+    int rc = check_hip(hipDeviceSynchronize(), "Nebo Memory");
+    if (rc) return rc;
+    for (auto& kv : pool.free_lists)
+      for (void* p : kv.second)
+        if ((rc = check_hip(hipFree(p), "Nebo Memory"))) return rc;
+    pool.free_lists.clear();
+    return NEBO_OK;
+  });
+}
+
+extern "C" int nebo_pool_stats(int device, size_t* in_use, size_t* high_water)
+{
+  clear_error();
+  std::lock_guard<std::mutex> lock(g_pool_mutex);
+  DevicePool& pool = g_pools[device];
+  if (in_use) *in_use = pool.in_use;
+  if (high_water) *high_water = pool.high_water;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream)
+{
+  clear_error();
+  if (bytes == 0) return NEBO_OK;
+  if (!dst || !src) return set_error(NEBO_ERR_INVALID, "Nebo Memory", "null pointer");
+  hipMemcpyKind k;
+  switch (kind) {
+  case 0: k = hipMemcpyHostToDevice; break;
+  case 1: k = hipMemcpyDeviceToHost; break;
+  case 2: k = hipMemcpyDeviceToDevice; break;
+  default: return set_error(NEBO_ERR_INVALID, "Nebo Memory", "unknown copy kind");
+  }
+  if (!stream) return check_hip(hipMemcpy(dst, src, bytes, k), "Nebo Memory");
+  return check_hip(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream), "Nebo Memory");
+}
+
+extern "C" int nebo_memcpy_peer(void* dst, int dst_dev, const void* src, int src_dev,
+                                size_t bytes, void* stream)
+{
+  clear_error();
+  if (bytes == 0) return NEBO_OK;
+  if (!stream)
+    return check_hip(hipMemcpyPeer(dst, dst_dev, src, src_dev, bytes), "Nebo Memory");
+  return check_hip(hipMemcpyPeerAsync(dst, dst_dev, src, src_dev, bytes, (hipStream_t)stream),
+                   "Nebo Memory");
+}
+
+extern "C" int nebo_memset(void* dst, int value, size_t bytes, void* stream)
+{
+  clear_error();
+  if (bytes == 0) return NEBO_OK;
+  if (!stream) return check_hip(hipMemset(dst, value, bytes), "Nebo Memory");
+  return check_hip(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "Nebo Memory");
+}
+
+extern "C" int nebo_stream_sync(void* stream)
+{
+  clear_error();
+  return check_hip(hipStreamSynchronize((hipStream_t)stream), "Nebo Stream");
+}
+
+extern "C" int nebo_device_sync(void)
+{
+  clear_error();
+  return check_hip(hipDeviceSynchronize(), "Nebo Stream");
+}

@@ -1,0 +1,14 @@
+"""Python binding of the MI355X Nebo evaluation engine (lib/libnebo_hip.so).
+
+The engine itself is C++/HIP behind the C-ABI of include/nebo_hip.h; this package
+builds expression descriptors, owns test fields and calls that ABI.
+"""
+from ._abi import NeboError, lib, header_symbols, LIB_PATH  # noqa: F401
+from .fields import Layout, Field, Mask, LOCATIONS, FACE_TYPES  # noqa: F401
+from .expr import (E, Expr, Stencil, cond, serialize, sin, cos, tan, exp, tanh, log, log10,  # noqa: F401
+                   sqrt, square, cube, erf, erfc, abs_, max_, min_, pow_, powint)
+from .ops import build_stencils, basic_op_types, axis_coefs, OperatorDatabase  # noqa: F401
+from .api import (assign, interior_assign, masked_assign, assign_limits, assign_plan,  # noqa: F401
+                  field_max, field_min, field_sum, field_norm, field_max_interior,
+                  field_min_interior, field_sum_interior, field_norm_interior,
+                  fill_ghosts_periodic, divkgrad, lapl_euler, device_count)

@@ -1,0 +1,108 @@
+"""Assignment, reductions and the direct kernel entry points of the binding.
+
+``assign(lhs, rhs)`` is ``lhs <<= rhs`` (NeboAssignment.h:29-133),
+``interior_assign`` is its useGhost=false variant (NeboAssignment.h:135-244),
+``masked_assign`` writes only the mask's points (NeboAssignment.h:246-313) and the
+``field_*`` functions are the reductions of NeboReductions.h:28-240.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _abi
+from .expr import E, serialize
+from .fields import Field, Mask
+
+
+def _stream(stream=None):
+    if stream is None:
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, torch.cuda.Stream):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(stream)
+
+
+def assign(lhs: Field, rhs, use_ghost=True, stream=None, generic=False):
+    s = serialize(E(rhs), "gpu")
+    d = lhs.desc("gpu")
+    fn = "nebo_assign_generic" if generic else "nebo_assign"
+    _abi.call(fn, C.byref(d), s.ref, int(use_ghost), _stream(stream))
+    return lhs
+
+
+def interior_assign(lhs: Field, rhs, stream=None, generic=False):
+    return assign(lhs, rhs, use_ghost=False, stream=stream, generic=generic)
+
+
+def assign_limits(lhs: Field, rhs, use_ghost=True, where="gpu"):
+    """Loop limits [lo, hi) per axis that lhs <<= rhs covers, or None if empty."""
+    s = serialize(E(rhs), where)
+    d = lhs.desc(where)
+    lim = _abi.LimitsT()
+    empty = C.c_int(0)
+    _abi.call("nebo_assign_limits", C.byref(d), s.ref, int(use_ghost), C.byref(lim), C.byref(empty))
+    if empty.value:
+        return None
+    return tuple(lim.lo), tuple(lim.hi)
+
+
+def assign_plan(lhs: Field, rhs, use_ghost=True, where="gpu") -> int:
+    s = serialize(E(rhs), where)
+    d = lhs.desc(where)
+    k = C.c_int(-1)
+    _abi.call("nebo_assign_plan", C.byref(d), s.ref, int(use_ghost), C.byref(k))
+    return k.value
+
+
+def masked_assign(mask: Mask, lhs: Field, rhs, stream=None):
+    if mask.gpu_points is None:
+        mask.add_consumer(lhs.device)
+    s = serialize(E(rhs), "gpu")
+    d = lhs.desc("gpu")
+    _abi.call("nebo_masked_assign", C.byref(d), C.c_void_p(mask.gpu_points.data_ptr()),
+              int(len(mask.points)), s.ref, _stream(stream))
+    return lhs
+
+
+def _reduce(expr, op, use_ghost, stream=None):
+    s = serialize(E(expr), "gpu")
+    out = C.c_double(0.0)
+    _abi.call("nebo_reduce", s.ref, op, int(use_ghost), C.byref(out), _stream(stream))
+    return out.value
+
+
+def field_max(e, stream=None): return _reduce(e, _abi.RED_MAX, True, stream)
+def field_min(e, stream=None): return _reduce(e, _abi.RED_MIN, True, stream)
+def field_sum(e, stream=None): return _reduce(e, _abi.RED_SUM, True, stream)
+def field_norm(e, stream=None): return _reduce(e, _abi.RED_NORM, True, stream)
+def field_max_interior(e, stream=None): return _reduce(e, _abi.RED_MAX, False, stream)
+def field_min_interior(e, stream=None): return _reduce(e, _abi.RED_MIN, False, stream)
+def field_sum_interior(e, stream=None): return _reduce(e, _abi.RED_SUM, False, stream)
+def field_norm_interior(e, stream=None): return _reduce(e, _abi.RED_NORM, False, stream)
+
+
+def fill_ghosts_periodic(f: Field, stream=None):
+    d = f.desc("gpu")
+    _abi.call("nebo_fill_ghosts_periodic", C.byref(d), _stream(stream))
+    return f
+
+
+def divkgrad(r: Field, k: Field, phi: Field, coefs, form=0, stream=None):
+    dr, dk, dp = r.desc("gpu"), k.desc("gpu"), phi.desc("gpu")
+    _abi.call("nebo_divkgrad", C.byref(dr), C.byref(dk), C.byref(dp), C.byref(coefs), int(form),
+              _stream(stream))
+    return r
+
+
+def lapl_euler(out: Field, phi: Field, coefs, s, stream=None):
+    do, dp = out.desc("gpu"), phi.desc("gpu")
+    _abi.call("nebo_lapl_euler", C.byref(do), C.byref(dp), C.byref(coefs), float(s), _stream(stream))
+    return out
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _abi.call("nebo_device_count", C.byref(n))
+    return n.value

@@ -1,0 +1,141 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU checker (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "spatialops-threadfix_amd", "python"))
+
+from nebo_hip import _abi  # noqa: E402
+
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class GhostT(C.Structure):
+    _fields_ = [("minus", _abi.I3), ("plus", _abi.I3)]
+
+
+_P = C.POINTER
+_vp = C.c_void_p
+_SIGS = {
+    "oracle_node_ghosts": (C.c_int, [_P(_abi.ExprT), C.c_int, C.c_int, _P(GhostT)]),
+    "oracle_assign_limits": (C.c_int, [_P(_abi.FieldT), _P(_abi.ExprT), C.c_int, _P(_abi.LimitsT), _P(C.c_int)]),
+    "oracle_eval": (C.c_double, [_P(_abi.ExprT), C.c_int, C.c_int, C.c_int, C.c_int]),
+    "oracle_assign": (C.c_int, [_P(_abi.FieldT), _P(_abi.ExprT), C.c_int]),
+    "oracle_assign_threads": (C.c_int, [_P(_abi.FieldT), _P(_abi.ExprT), C.c_int, C.c_int]),
+    "oracle_masked_assign": (C.c_int, [_P(_abi.FieldT), _vp, C.c_int, _P(_abi.ExprT), C.c_int]),
+    "oracle_reduce": (C.c_int, [_P(_abi.ExprT), C.c_int, C.c_int, _P(C.c_double)]),
+    "oracle_fill_ghosts_periodic": (None, [_P(_abi.FieldT)]),
+    "oracle_mask_bits": (None, [_P(_abi.MaskT), _vp, C.c_int, _vp]),
+    "oracle_divkgrad": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_int]),
+    "oracle_divkgrad_threads": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT),
+                                       C.c_int, C.c_int]),
+    "oracle_lapl_euler": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double]),
+    "oracle_diffusion_step": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT),
+                                     C.c_double]),
+    "oracle_diffusion_adjoint_step": (None, [_P(_abi.FieldT), _vp, _P(_abi.FieldT), _P(_abi.FieldT),
+                                             _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double]),
+    "oracle_splitmix64": (C.c_uint64, [_P(C.c_uint64)]),
+    "oracle_fill_uniform": (None, [_vp, C.c_size_t, C.c_uint64, C.c_double, C.c_double]),
+    "oracle_fill_normal": (None, [_vp, C.c_size_t, C.c_uint64]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"oracle not built: {ORACLE_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(ORACLE_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"oracle status {rc}")
+
+
+def assign(lhs, rhs_expr, use_ghost=True, threads=0):
+    """lhs <<= rhs on host copies (lhs is a nebo_hip.Field with a host array)."""
+    from nebo_hip.expr import serialize, E
+    s = serialize(E(rhs_expr), "host")
+    d = lhs.desc("host")
+    if threads:
+        check(lib().oracle_assign_threads(C.byref(d), s.ref, int(use_ghost), int(threads)))
+    else:
+        check(lib().oracle_assign(C.byref(d), s.ref, int(use_ghost)))
+    return lhs
+
+
+def limits(lhs, rhs_expr, use_ghost=True):
+    from nebo_hip.expr import serialize, E
+    s = serialize(E(rhs_expr), "host")
+    d = lhs.desc("host")
+    lim = _abi.LimitsT()
+    empty = C.c_int(0)
+    check(lib().oracle_assign_limits(C.byref(d), s.ref, int(use_ghost), C.byref(lim), C.byref(empty)))
+    return None if empty.value else (tuple(lim.lo), tuple(lim.hi))
+
+
+def reduce(expr, op, use_ghost=True):
+    from nebo_hip.expr import serialize, E
+    s = serialize(E(expr), "host")
+    out = C.c_double(0)
+    check(lib().oracle_reduce(s.ref, int(op), int(use_ghost), C.byref(out)))
+    return out.value
+
+
+def masked_assign(mask, lhs, rhs_expr, passes=2):
+    from nebo_hip.expr import serialize, E
+    import numpy as np
+    s = serialize(E(rhs_expr), "host")
+    d = lhs.desc("host")
+    pts = np.ascontiguousarray(mask.points.reshape(-1), dtype=np.int32)
+    check(lib().oracle_masked_assign(C.byref(d), pts.ctypes.data, len(mask.points), s.ref, passes))
+    return lhs
+
+
+def fill_periodic(f):
+    d = f.desc("host")
+    lib().oracle_fill_ghosts_periodic(C.byref(d))
+    return f
+
+
+def divkgrad(r, k, phi, coefs, form=0, threads=0):
+    dr, dk, dp = r.desc("host"), k.desc("host"), phi.desc("host")
+    if threads:
+        lib().oracle_divkgrad_threads(C.byref(dr), C.byref(dk), C.byref(dp), C.byref(coefs), form, threads)
+    else:
+        lib().oracle_divkgrad(C.byref(dr), C.byref(dk), C.byref(dp), C.byref(coefs), form)
+    return r
+
+
+def lapl_euler(out, phi, coefs, s):
+    do, dp = out.desc("host"), phi.desc("host")
+    lib().oracle_lapl_euler(C.byref(do), C.byref(dp), C.byref(coefs), float(s))
+    return out
+
+
+def diffusion_step(out, phi, k, coefs, dt):
+    do, dp, dk = out.desc("host"), phi.desc("host"), k.desc("host")
+    lib().oracle_diffusion_step(C.byref(do), C.byref(dp), C.byref(dk), C.byref(coefs), float(dt))
+    return out
+
+
+def diffusion_adjoint_step(lam_out, gk, lam, phi, k, coefs, dt):
+    dlo, dl, dp, dk = lam_out.desc("host"), lam.desc("host"), phi.desc("host"), k.desc("host")
+    lib().oracle_diffusion_adjoint_step(C.byref(dlo), gk.ctypes.data, C.byref(dl), C.byref(dp), C.byref(dk),
+                                        C.byref(coefs), float(dt))
+    return lam_out
