@@ -204,10 +204,12 @@ int nebo_assign_plan(const nebo_field_t* lhs, const nebo_expr_t* rhs,
                      int use_ghost, int* kernel_id);
 
 /* masked_assign(mask, lhs, rhs): lhs(p) = rhs(p) for each listed point p
- * (interior-relative ijk triples, device array of 3*n_points int32).  The
- * reference evaluates this on CPU only (NeboLhs.h:164-283, NeboAssignment.h:246-313);
- * this is its GPU counterpart, with one evaluation per point. */
-int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points_dev,
+ * (interior-relative ijk triples, HOST array of 3*n_points int32; the library
+ * checks every point against the LHS window and the RHS's valid ghost range,
+ * then copies the list to the device).  The reference evaluates this on CPU only
+ * (NeboLhs.h:164-283, NeboAssignment.h:246-313); this is its GPU counterpart,
+ * with one evaluation per point. */
+int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points,
                        int32_t n_points, const nebo_expr_t* rhs, void* stream);
 
 /* ------------------------------------------------------------------------- */

@@ -20,6 +20,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <sstream>
 
 namespace nebo {
 namespace {
@@ -585,12 +586,12 @@ extern "C" int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* r
   return do_assign(lhs, rhs, use_ghost, stream, false);
 }
 
-extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points_dev,
+extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points,
                                   int32_t n_points, const nebo_expr_t* rhs, void* stream)
 {
   static const char* ctx = "Nebo Masked Assignment";
   clear_error();
-  if (!lhs || !rhs || (n_points > 0 && !points_dev))
+  if (!lhs || !rhs || (n_points > 0 && !points))
     return set_error(NEBO_ERR_INVALID, ctx, "null argument");
   int rc = validate_expr(rhs, ctx);
   if (rc) return rc;
@@ -598,23 +599,45 @@ extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points
   if ((rc = check_index_range(rhs, lhs, ctx))) return rc;
   if ((rc = check_aliasing(lhs, rhs, ctx))) return rc;
   if (n_points <= 0) return NEBO_OK;
+  // every point must lie in the LHS window and in the RHS's valid range
+  Ghost g;
+  if ((rc = node_ghosts(rhs, rhs->n_nodes - 1, true, &g))) return rc;
+  for (int i = 0; i < n_points; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const int p = points[3 * i + d];
+      const int ext = lhs->extent[d] - lhs->valid_minus[d] - lhs->valid_plus[d];
+      const int wlo = -lhs->valid_minus[d], whi = ext + lhs->valid_plus[d];
+      const int rlo = g.infinite() ? wlo : -g.m[d], rhi = g.infinite() ? whi : ext + g.p[d];
+      if (p < wlo || p >= whi || p < rlo || p >= rhi) {
+        std::ostringstream m;
+        m << "mask point " << i << " (" << points[3 * i] << ", " << points[3 * i + 1] << ", "
+          << points[3 * i + 2] << ") lies outside the valid range of the assignment";
+        return set_error(NEBO_ERR_GHOST, ctx, m.str());
+      }
+    }
   VmProgram prog;
   int depth = 0;
   if ((rc = lower_program(rhs, &prog, &depth))) return rc;
   double* out = frame_origin(lhs);
   const int oX = lhs->glob[0], oXY = lhs->glob[0] * lhs->glob[1];
   hipStream_t s = (hipStream_t)stream;
+  int32_t* pts = nullptr;
+  const size_t bytes = sizeof(int32_t) * 3 * (size_t)n_points;
+  if ((rc = check_hip(hipMallocAsync((void**)&pts, bytes, s), ctx))) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(pts, points, bytes, hipMemcpyHostToDevice, s), ctx))) return rc;
   dim3 grid((n_points + kBlock - 1) / kBlock);
   const bool one = prog.n_classes == 1, math = uses_math(prog);
   if (one && !math)
-    hipLaunchKernelGGL((vm_masked_kernel<1, false>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+    hipLaunchKernelGGL((vm_masked_kernel<1, false>), grid, dim3(kBlock), 0, s, prog, pts, n_points, out, oX, oXY);
   else if (one)
-    hipLaunchKernelGGL((vm_masked_kernel<1, true>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+    hipLaunchKernelGGL((vm_masked_kernel<1, true>), grid, dim3(kBlock), 0, s, prog, pts, n_points, out, oX, oXY);
   else if (!math)
-    hipLaunchKernelGGL((vm_masked_kernel<kVmMaxClasses, false>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
+    hipLaunchKernelGGL((vm_masked_kernel<kVmMaxClasses, false>), grid, dim3(kBlock), 0, s, prog, pts, n_points, out, oX, oXY);
   else
-    hipLaunchKernelGGL((vm_masked_kernel<kVmMaxClasses, true>), grid, dim3(kBlock), 0, s, prog, points_dev, n_points, out, oX, oXY);
-  return check_hip(hipGetLastError(), ctx);
+    hipLaunchKernelGGL((vm_masked_kernel<kVmMaxClasses, true>), grid, dim3(kBlock), 0, s, prog, pts, n_points, out, oX, oXY);
+  rc = check_hip(hipGetLastError(), ctx);
+  (void)hipFreeAsync(pts, s);
+  return rc;
 }
 
 extern "C" int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,

@@ -57,11 +57,11 @@ def assign_plan(lhs: Field, rhs, use_ghost=True, where="gpu") -> int:
 
 
 def masked_assign(mask: Mask, lhs: Field, rhs, stream=None):
-    if mask.gpu_points is None:
-        mask.add_consumer(lhs.device)
+    import numpy as np
     s = serialize(E(rhs), "gpu")
     d = lhs.desc("gpu")
-    _abi.call("nebo_masked_assign", C.byref(d), C.c_void_p(mask.gpu_points.data_ptr()),
+    pts = np.ascontiguousarray(mask.points.reshape(-1), dtype=np.int32)
+    _abi.call("nebo_masked_assign", C.byref(d), C.c_void_p(pts.ctypes.data),
               int(len(mask.points)), s.ref, _stream(stream))
     return lhs
 

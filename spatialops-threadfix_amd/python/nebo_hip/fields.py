@@ -177,14 +177,11 @@ class Mask:
         self.words = words
         self.device = device
         self.gpu = None
-        self.gpu_points = None
 
     def add_consumer(self, device=0):
-        """Copy the bit field (and the point list, for masked_assign) to a GPU
-        (BitField::add_consumer, BitField.h:466-481)."""
+        """Copy the bit field to a GPU (BitField::add_consumer, BitField.h:466-481)."""
         self.device = device
         self.gpu = torch.from_numpy(self.words.view(np.int32)).to(f"cuda:{device}")
-        self.gpu_points = torch.from_numpy(self.points.reshape(-1).copy()).to(f"cuda:{device}")
         return self
 
     def desc(self, where="gpu") -> _abi.MaskT:
