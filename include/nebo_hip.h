@@ -267,34 +267,42 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
 /* batched patches: one persistent launch over a device patch table          */
 /* ------------------------------------------------------------------------- */
 
-/* One Uintah-style patch: its own interior dims, 1 ghost layer, in/out blocks. */
+/* One Uintah-style patch: its own interior cell counts and one ghost layer.
+ * a and b are two (n[0]+2)*(n[1]+2)*(n[2]+2) x-fastest blocks of the patch's
+ * state; a step reads one and writes the interior of the other (ghost cells are
+ * never written: they hold whatever boundary values the caller put there). */
 typedef struct nebo_patch {
-  const double* in;      /* (n+2)^3 block, x fastest                        */
-  double*       out;     /* same shape                                      */
-  int32_t n[3];          /* interior cells per axis                         */
+  double* a;
+  double* b;
+  int32_t n[3];          /* interior cells per axis (each >= 1)             */
   int32_t pad_;
-  double  inv_dx2[3];    /* (1/dx)^2 factors are not used: see coefs below  */
 } nebo_patch_t;
 
 typedef struct nebo_batch nebo_batch_t;   /* opaque */
 
-/* Build a device patch table and the tile schedule for `n` patches.  tile[3]
- * is the (x,y,z) tile of one work item; tile == NULL lets the autotuner pick
- * (nebo_batch_autotune).  Coefficients and s are shared by all patches. */
+/* Build a device patch table and the tile schedule for `n` patches, all on the
+ * current device.  tile[3] is the (x, y, z) extent of one work item (x*y <= 1024
+ * threads); tile == NULL starts from (64, 4, 16).  Coefficients and s are shared
+ * by all patches: out = in + s*((Lx + Ly) + Lz) as in nebo_lapl_euler. */
 int nebo_batch_create(const nebo_patch_t* patches, int32_t n,
                       const nebo_axis_coefs_t* c, double s,
                       const int32_t* tile, nebo_batch_t** out);
 int nebo_batch_destroy(nebo_batch_t* b);
-/* One Laplacian+Euler step on every patch (in -> out) in one persistent launch. */
-int nebo_batch_step(nebo_batch_t* b, int swap_in_out, void* stream);
+/* One step on every patch in one persistent launch: a -> b (swap = 0) or
+ * b -> a (swap = 1). */
+int nebo_batch_step(nebo_batch_t* b, int swap, void* stream);
 /* The same step as one launch per patch (the comparison path). */
-int nebo_batch_step_per_patch(nebo_batch_t* b, int swap_in_out, void* stream);
-/* Capture `steps` alternating steps into a hipGraph (built once, replayed by
- * nebo_batch_graph_launch). */
+int nebo_batch_step_per_patch(nebo_batch_t* b, int swap, void* stream);
+/* Capture `steps` alternating steps (a->b, b->a, ...) into a hipGraph, built
+ * once and replayed by nebo_batch_graph_launch.  stream must not be NULL. */
 int nebo_batch_graph_build(nebo_batch_t* b, int32_t steps, void* stream);
 int nebo_batch_graph_launch(nebo_batch_t* b, void* stream);
-/* Time candidate tiles on the device and keep the fastest; writes it to tile_out. */
+/* Time candidate tiles with one step each (a -> b), keep the fastest, write it
+ * to tile_out (3 ints) and drop any captured graph.  Synchronises stream. */
 int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* stream);
+/* Tile currently used, number of work items and of persistent workgroups. */
+int nebo_batch_info(nebo_batch_t* b, int32_t* tile_out, int32_t* n_items,
+                    int32_t* n_blocks);
 
 /* ------------------------------------------------------------------------- */
 /* adjoint of the var-coefficient diffusion step                              */
@@ -318,17 +326,21 @@ int nebo_diffusion_adjoint_step(const nebo_field_t* lam_out, double* gk,
                                 const nebo_field_t* k, const nebo_axis_coefs_t* c,
                                 double dt, void* stream);
 
-/* Whole trajectory: n_steps forward steps from phi0 with checkpoints every
- * `every` steps into ckpt (n_steps/every + 1 fields laid out back to back,
- * each with phi0's window), final state into phiT. */
+/* Whole trajectory: n_steps forward steps from phi0 (n_steps a multiple of
+ * `every`), keeping the state after every `every` steps: ckpt holds
+ * n_steps/every + 1 glob blocks of phi0's shape back to back (block j = state
+ * after j*every steps, block 0 = phi0 with periodic ghosts); the final state
+ * also goes to phiT. */
 int nebo_diffusion_forward(const nebo_field_t* phiT, const nebo_field_t* phi0,
                            const nebo_field_t* k, const nebo_axis_coefs_t* c,
                            double dt, int32_t n_steps, int32_t every,
                            double* ckpt, void* stream);
 
-/* Reverse sweep with checkpoint/recompute: lamT = dL/d phiT on entry.  On
- * return lam0 = dL/d phi0 and gk = dL/dk (interior cells; ghosts zero).
- * work must hold (every+2) fields of phi0's window. */
+/* Reverse sweep with checkpoint/recompute: lamT = dL/d phiT on entry (its
+ * ghosts are filled here).  Each segment of `every` steps is recomputed from its
+ * checkpoint, then swept backwards with the adjoint step.  On return
+ * lam0 = dL/d phi0 and gk (k's shape) = dL/dk on interior cells, ghosts zero.
+ * work must hold every+1 glob blocks of phi0's shape. */
 int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk,
                             const nebo_field_t* lamT, const nebo_field_t* k,
                             const nebo_axis_coefs_t* c, double dt,

@@ -1,0 +1,317 @@
+// nebo_batch.hip -- many uneven patches per launch.
+//
+// Uintah hands SpatialOps one patch at a time; on a GPU that means one small
+// launch per patch, each far too small to fill 256 CUs.  Here the patches live
+// in a device table and the step is cut into work items (patch, x0, y0, z0) of a
+// fixed tile shape.  One persistent launch of ~4 workgroups per CU walks the item
+// list with a fixed stride, so every patch of every size is covered by one
+// launch, and a hipGraph holds several such steps.  The tile shape is chosen by
+// timing candidates on the device (nebo_batch_autotune).
+//
+// The arithmetic per cell is the Laplacian + explicit Euler update of
+// nebo_lapl_euler (out = in + s*((Lx + Ly) + Lz)), bit for bit.
+
+#include "nebo_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace nebo {
+namespace {
+
+struct DevPatch {
+  double* a;
+  double* b;
+  int nx, ny, nz;
+  int X, XY;       // strides of the ghosted block
+  int origin;      // flat index of interior cell (0,0,0)
+};
+
+struct Coef6 {
+  double gx0, gx1, gy0, gy1, gz0, gz1, dx0, dx1, dy0, dy1, dz0, dz1;
+};
+
+__device__ __forceinline__ void patch_column(const double* __restrict__ in, double* __restrict__ out,
+                                             int i, int XY, int X, int zn, const Coef6& c, double s)
+{
+  double pc = in[i];
+  double gz_lo = in[i - XY] * c.gz0 + pc * c.gz1;
+  for (int z = 0; z < zn; ++z, i += XY) {
+    const double pz = in[i + XY];
+    const double pxm = in[i - 1], pxp = in[i + 1], pym = in[i - X], pyp = in[i + X];
+    const double gz_hi = pc * c.gz0 + pz * c.gz1;
+    const double Lx = (pxm * c.gx0 + pc * c.gx1) * c.dx0 + (pc * c.gx0 + pxp * c.gx1) * c.dx1;
+    const double Ly = (pym * c.gy0 + pc * c.gy1) * c.dy0 + (pc * c.gy0 + pyp * c.gy1) * c.dy1;
+    const double Lz = gz_lo * c.dz0 + gz_hi * c.dz1;
+    out[i] = pc + s * ((Lx + Ly) + Lz);
+    pc = pz;
+    gz_lo = gz_hi;
+  }
+}
+
+// Work item w = (patch, x0, y0, z0).  Workgroup b takes items b, b + G, b + 2G, ...
+__global__ __launch_bounds__(1024) void batch_kernel(const DevPatch* __restrict__ P,
+                                                     const int4* __restrict__ items, int n_items,
+                                                     int tx, int ty, int tz, Coef6 c, double s,
+                                                     int swap)
+{
+  const int lx = threadIdx.x % tx;
+  const int ly = threadIdx.x / tx;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int4 w = items[it];
+    const DevPatch p = P[w.x];
+    const int x = w.y + lx, y = w.z + ly;
+    if (x >= p.nx || y >= p.ny) continue;
+    const double* in = swap ? p.b : p.a;
+    double* out = swap ? p.a : p.b;
+    const int zn = min(tz, p.nz - w.w);
+    patch_column(in, out, p.origin + x + p.X * y + p.XY * w.w, p.XY, p.X, zn, c, s);
+  }
+}
+
+Coef6 coef6(const nebo_axis_coefs_t& c)
+{
+  return Coef6{c.grad[0][0], c.grad[0][1], c.grad[1][0], c.grad[1][1], c.grad[2][0], c.grad[2][1],
+               c.div[0][0],  c.div[0][1],  c.div[1][0],  c.div[1][1],  c.div[2][0],  c.div[2][1]};
+}
+
+}  // namespace
+}  // namespace nebo
+
+using namespace nebo;
+
+struct nebo_batch {
+  std::vector<nebo_patch_t> patches;
+  nebo_axis_coefs_t coefs;
+  double s = 0.0;
+  int tile[3] = {64, 4, 16};
+  int device = 0;
+  int cus = 256;
+  DevPatch* d_patches = nullptr;
+  int4* d_items = nullptr;
+  int n_items = 0;
+  std::vector<int> patch_first;   // first item of each patch (items are grouped by patch)
+  int blocks = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int build_items(nebo_batch* b)
+{
+  static const char* ctx = "Nebo Batch";
+  std::vector<int4> items;
+  b->patch_first.assign(b->patches.size() + 1, 0);
+  const int tx = b->tile[0], ty = b->tile[1], tz = b->tile[2];
+  for (size_t k = 0; k < b->patches.size(); ++k) {
+    b->patch_first[k] = (int)items.size();
+    const nebo_patch_t& p = b->patches[k];
+    for (int z = 0; z < p.n[2]; z += tz)
+      for (int y = 0; y < p.n[1]; y += ty)
+        for (int x = 0; x < p.n[0]; x += tx) items.push_back(make_int4((int)k, x, y, z));
+  }
+  b->patch_first[b->patches.size()] = (int)items.size();
+  if (b->d_items) (void)hipFree(b->d_items);
+  b->d_items = nullptr;
+  b->n_items = (int)items.size();
+  int rc = check_hip(hipMalloc((void**)&b->d_items, sizeof(int4) * std::max<size_t>(1, items.size())), ctx);
+  if (rc) return rc;
+  rc = check_hip(hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice), ctx);
+  if (rc) return rc;
+  // persistent grid: a few workgroups per CU, never more than there are items
+  const int threads = tx * ty;
+  const int per_cu = threads <= 256 ? 8 : threads <= 512 ? 4 : 2;
+  b->blocks = std::max(1, std::min(b->n_items, b->cus * per_cu));
+  return NEBO_OK;
+}
+
+void drop_graph(nebo_batch* b)
+{
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int check_tile(const int32_t* t)
+{
+  if (t[0] < 1 || t[1] < 1 || t[2] < 1 || t[0] * t[1] > 1024)
+    return set_error(NEBO_ERR_INVALID, "Nebo Batch", "tile must be positive with x*y <= 1024 threads");
+  return NEBO_OK;
+}
+
+int launch_items(nebo_batch* b, int first, int count, int grid, int swap, hipStream_t s)
+{
+  if (count <= 0) return NEBO_OK;
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(b->tile[0] * b->tile[1]), 0, s, b->d_patches,
+                     b->d_items + first, count, b->tile[0], b->tile[1], b->tile[2], coef6(b->coefs), b->s,
+                     swap);
+  return check_hip(hipGetLastError(), "Nebo Batch");
+}
+
+}  // namespace
+
+extern "C" int nebo_batch_create(const nebo_patch_t* patches, int32_t n, const nebo_axis_coefs_t* c,
+                                 double s, const int32_t* tile, nebo_batch_t** out)
+{
+  static const char* ctx = "Nebo Batch";
+  clear_error();
+  if (!patches || n <= 0 || !c || !out) return set_error(NEBO_ERR_INVALID, ctx, "null argument or no patches");
+  *out = nullptr;
+  if (tile) {
+    int rc = check_tile(tile);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < n; ++i) {
+    const nebo_patch_t& p = patches[i];
+    if (!p.a || !p.b || p.a == p.b || p.n[0] < 1 || p.n[1] < 1 || p.n[2] < 1)
+      return set_error(NEBO_ERR_INVALID, ctx, "patch needs two distinct blocks and positive sizes");
+    if ((long)(p.n[0] + 2) * (p.n[1] + 2) * (p.n[2] + 2) >= (1L << 31))
+      return set_error(NEBO_ERR_TOO_LARGE, ctx, "patch has 2^31 or more points");
+  }
+  nebo_batch* b = new nebo_batch;
+  b->patches.assign(patches, patches + n);
+  b->coefs = *c;
+  b->s = s;
+  if (tile) std::memcpy(b->tile, tile, sizeof b->tile);
+  int rc = check_hip(hipGetDevice(&b->device), ctx);
+  hipDeviceProp_t prop;
+  if (!rc) rc = check_hip(hipGetDeviceProperties(&prop, b->device), ctx);
+  if (!rc) b->cus = prop.multiProcessorCount;
+  std::vector<DevPatch> dp(n);
+  for (int i = 0; i < n; ++i) {
+    const nebo_patch_t& p = patches[i];
+    DevPatch& d = dp[i];
+    d.a = p.a;
+    d.b = p.b;
+    d.nx = p.n[0];
+    d.ny = p.n[1];
+    d.nz = p.n[2];
+    d.X = p.n[0] + 2;
+    d.XY = d.X * (p.n[1] + 2);
+    d.origin = 1 + d.X + d.XY;
+  }
+  if (!rc) rc = check_hip(hipMalloc((void**)&b->d_patches, sizeof(DevPatch) * n), ctx);
+  if (!rc) rc = check_hip(hipMemcpy(b->d_patches, dp.data(), sizeof(DevPatch) * n, hipMemcpyHostToDevice), ctx);
+  if (!rc) rc = build_items(b);
+  if (rc) {
+    nebo_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_batch_destroy(nebo_batch_t* b)
+{
+  clear_error();
+  if (!b) return NEBO_OK;
+  drop_graph(b);
+  if (b->d_patches) (void)hipFree(b->d_patches);
+  if (b->d_items) (void)hipFree(b->d_items);
+  delete b;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_batch_step(nebo_batch_t* b, int swap, void* stream)
+{
+  clear_error();
+  if (!b) return set_error(NEBO_ERR_INVALID, "Nebo Batch", "null batch");
+  return launch_items(b, 0, b->n_items, b->blocks, swap ? 1 : 0, (hipStream_t)stream);
+}
+
+extern "C" int nebo_batch_step_per_patch(nebo_batch_t* b, int swap, void* stream)
+{
+  clear_error();
+  if (!b) return set_error(NEBO_ERR_INVALID, "Nebo Batch", "null batch");
+  for (size_t k = 0; k < b->patches.size(); ++k) {
+    const int first = b->patch_first[k], count = b->patch_first[k + 1] - first;
+    int rc = launch_items(b, first, count, count, swap ? 1 : 0, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return NEBO_OK;
+}
+
+extern "C" int nebo_batch_graph_build(nebo_batch_t* b, int32_t steps, void* stream)
+{
+  static const char* ctx = "Nebo Batch";
+  clear_error();
+  if (!b || steps < 1 || !stream) return set_error(NEBO_ERR_INVALID, ctx, "null batch/stream or no steps");
+  drop_graph(b);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), ctx);
+  if (rc) return rc;
+  for (int i = 0; i < steps && !rc; ++i) rc = launch_items(b, 0, b->n_items, b->blocks, i & 1, s);
+  hipGraph_t g = nullptr;
+  const int rc2 = check_hip(hipStreamEndCapture(s, &g), ctx);
+  if (rc || rc2) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : rc2;
+  }
+  b->graph = g;
+  return check_hip(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), ctx);
+}
+
+extern "C" int nebo_batch_graph_launch(nebo_batch_t* b, void* stream)
+{
+  clear_error();
+  if (!b || !b->exec) return set_error(NEBO_ERR_INVALID, "Nebo Batch", "no graph built");
+  return check_hip(hipGraphLaunch(b->exec, (hipStream_t)stream), "Nebo Batch");
+}
+
+extern "C" int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* stream)
+{
+  static const char* ctx = "Nebo Batch";
+  clear_error();
+  if (!b) return set_error(NEBO_ERR_INVALID, ctx, "null batch");
+  drop_graph(b);
+  hipStream_t s = (hipStream_t)stream;
+  static const int cand[][3] = {
+      {64, 4, 16}, {64, 4, 32}, {64, 4, 8}, {64, 2, 16}, {64, 8, 16}, {64, 8, 8}, {64, 16, 8},
+      {32, 8, 16}, {32, 8, 8}, {32, 4, 16}, {32, 16, 8}, {16, 16, 16}, {16, 16, 8}, {16, 8, 16},
+      {64, 1, 64}, {128, 2, 16}, {128, 4, 8}, {256, 1, 16}};
+  hipEvent_t e0, e1;
+  int rc = check_hip(hipEventCreate(&e0), ctx);
+  if (!rc) rc = check_hip(hipEventCreate(&e1), ctx);
+  if (rc) return rc;
+  float best = 1e30f;
+  int best_tile[3] = {b->tile[0], b->tile[1], b->tile[2]};
+  for (const auto& t : cand) {
+    std::memcpy(b->tile, t, sizeof b->tile);
+    if ((rc = build_items(b))) break;
+    // warm once, then time three steps
+    if ((rc = launch_items(b, 0, b->n_items, b->blocks, 0, s))) break;
+    if ((rc = check_hip(hipEventRecord(e0, s), ctx))) break;
+    for (int r = 0; r < 3 && !rc; ++r) rc = launch_items(b, 0, b->n_items, b->blocks, 0, s);
+    if (rc || (rc = check_hip(hipEventRecord(e1, s), ctx))) break;
+    if ((rc = check_hip(hipEventSynchronize(e1), ctx))) break;
+    float ms = 0;
+    if ((rc = check_hip(hipEventElapsedTime(&ms, e0, e1), ctx))) break;
+    if (ms < best) {
+      best = ms;
+      std::memcpy(best_tile, t, sizeof best_tile);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::memcpy(b->tile, best_tile, sizeof b->tile);
+  const int rc2 = build_items(b);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (tile_out) std::memcpy(tile_out, b->tile, sizeof b->tile);
+  return NEBO_OK;
+}
+
+extern "C" int nebo_batch_info(nebo_batch_t* b, int32_t* tile_out, int32_t* n_items, int32_t* n_blocks)
+{
+  clear_error();
+  if (!b) return set_error(NEBO_ERR_INVALID, "Nebo Batch", "null batch");
+  if (tile_out) std::memcpy(tile_out, b->tile, sizeof b->tile);
+  if (n_items) *n_items = b->n_items;
+  if (n_blocks) *n_blocks = b->blocks;
+  return NEBO_OK;
+}

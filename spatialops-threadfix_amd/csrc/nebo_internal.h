@@ -127,6 +127,13 @@ int launch_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
                       const nebo_axis_coefs_t& c, double s, const nebo_limits_t& lim,
                       void* stream);
 
+// out = phi + dt*div(interp(k)*grad(phi)) over lim (the diffusion kernel's Euler form).
+int launch_diffusion_euler(const nebo_field_t* out, const nebo_field_t* k, const nebo_field_t* phi,
+                           const nebo_axis_coefs_t& c, double dt, const nebo_limits_t& lim,
+                           void* stream);
+// Periodic copy-fill of every allocated ghost layer of f.
+int fill_ghosts_periodic(const nebo_field_t* f, void* stream);
+
 // Pointer of the first valid-ghost-excluded point of a field (the frame origin).
 inline double* frame_origin(const nebo_field_t* f)
 {

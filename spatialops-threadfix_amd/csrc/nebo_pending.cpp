@@ -11,14 +11,6 @@ static int pending(const char* what)
 }
 
 extern "C" {
-int nebo_batch_create(const nebo_patch_t*, int32_t, const nebo_axis_coefs_t*, double,
-                      const int32_t*, nebo_batch_t**) { return pending("Nebo Batch"); }
-int nebo_batch_destroy(nebo_batch_t*) { return pending("Nebo Batch"); }
-int nebo_batch_step(nebo_batch_t*, int, void*) { return pending("Nebo Batch"); }
-int nebo_batch_step_per_patch(nebo_batch_t*, int, void*) { return pending("Nebo Batch"); }
-int nebo_batch_graph_build(nebo_batch_t*, int32_t, void*) { return pending("Nebo Batch"); }
-int nebo_batch_graph_launch(nebo_batch_t*, void*) { return pending("Nebo Batch"); }
-int nebo_batch_autotune(nebo_batch_t*, int32_t*, void*) { return pending("Nebo Batch"); }
 int nebo_diffusion_step(const nebo_field_t*, const nebo_field_t*, const nebo_field_t*,
                         const nebo_axis_coefs_t*, double, void*) { return pending("Nebo Adjoint"); }
 int nebo_diffusion_adjoint_step(const nebo_field_t*, double*, const nebo_field_t*,

@@ -62,7 +62,7 @@ __device__ __forceinline__ double face(double klo, double khi, double plo, doubl
 template <int FORM>
 __global__ __launch_bounds__(kTx * kTy) void divkgrad_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
-    Tiles T, int X, int XY, nebo_axis_coefs_t c)
+    Tiles T, int X, int XY, nebo_axis_coefs_t c, double dt)
 {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % T.nx;
@@ -99,7 +99,8 @@ __global__ __launch_bounds__(kTx * kTy) void divkgrad_kernel(
     const double Dz = fz_lo * dz0 + fz_hi * dz1;
     double v;
     if constexpr (FORM == 0) v = (Dx + Dy) + Dz;
-    else v = ((-Dx) - Dy) - Dz;
+    else if constexpr (FORM == 1) v = ((-Dx) - Dy) - Dz;
+    else v = pc + dt * ((Dx + Dy) + Dz);   // explicit Euler step of the diffusion equation
     __builtin_nontemporal_store(v, &r[i]);
     pc = pz;
     kc = kz;
@@ -194,11 +195,23 @@ int launch_divkgrad(const nebo_field_t* r, const nebo_field_t* k, const nebo_fie
   hipStream_t s = (hipStream_t)stream;
   if (form == 0)
     hipLaunchKernelGGL(divkgrad_kernel<0>, dim3(nb), dim3(kTx * kTy), 0, s, frame_origin(r),
-                       frame_origin(k), frame_origin(phi), T, X, XY, c);
+                       frame_origin(k), frame_origin(phi), T, X, XY, c, 0.0);
   else
     hipLaunchKernelGGL(divkgrad_kernel<1>, dim3(nb), dim3(kTx * kTy), 0, s, frame_origin(r),
-                       frame_origin(k), frame_origin(phi), T, X, XY, c);
+                       frame_origin(k), frame_origin(phi), T, X, XY, c, 0.0);
   return check_hip(hipGetLastError(), "Nebo Assignment");
+}
+
+int launch_diffusion_euler(const nebo_field_t* out, const nebo_field_t* k, const nebo_field_t* phi,
+                           const nebo_axis_coefs_t& c, double dt, const nebo_limits_t& lim,
+                           void* stream)
+{
+  Tiles T = make_tiles(lim, kZChunk);
+  const int nb = T.nx * T.ny * T.nz;
+  const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
+  hipLaunchKernelGGL(divkgrad_kernel<2>, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream,
+                     frame_origin(out), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
+  return check_hip(hipGetLastError(), "Nebo Diffusion");
 }
 
 int launch_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
