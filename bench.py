@@ -33,6 +33,10 @@ def parse():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-tune", action="store_true", help="keep the default tile (no autotune)")
+    p.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="only launch 3 known-size copies and 3 kernels (for rocprofv3 --pmc runs)")
     return p.parse_args()
 
 
@@ -75,12 +79,125 @@ def cpu_baseline(L, kh, ph, coefs, budget_s):
             "serial_value": round(gs, 4)}
 
 
+def batched_leg(steps=10, count=512):
+    """512 patches with interior sizes drawn from {16,24,32,48,64} per axis (seeded),
+    Laplacian + Euler per step: one persistent launch per step captured 10 at a
+    time in a hipGraph, against one launch per patch per step."""
+    import numpy as np
+    import torch
+    import nebo_hip as nh
+    from nebo_hip import inputs
+    sizes = np.array([16, 24, 32, 48, 64])
+    idx = (inputs.splitmix64(4, 3 * count) % np.uint64(5)).astype(np.int64).reshape(count, 3)
+    dims = [tuple(int(v) for v in sizes[i]) for i in idx]
+    cells = sum(d[0] * d[1] * d[2] for d in dims)
+    db = nh.build_stencils(64, 64, 64, 1.0, 1.0, 1.0)
+    coefs = nh.axis_coefs(db)
+    s = 0.1 / 64.0 ** 2
+    a, b = nh.alloc_patches(dims)
+    for x in a:
+        x.normal_()
+    B = nh.Batch(a, b, dims, coefs, s)
+    tile = B.autotune()
+    stream = torch.cuda.Stream()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        B.graph_build(steps, stream)
+        B.graph_launch(stream)
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for _ in range(3):
+            B.graph_launch(stream)
+        e1.record(stream)
+        stream.synchronize()
+        graph_ms = e0.elapsed_time(e1) / (3 * steps)
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for i in range(steps):
+            B.step(i & 1, stream)
+        e1.record(stream)
+        stream.synchronize()
+        eager_ms = e0.elapsed_time(e1) / steps
+        B.step_per_patch(0, stream)
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for i in range(steps):
+            B.step_per_patch(i & 1, stream)
+        e1.record(stream)
+        stream.synchronize()
+        per_patch_ms = e0.elapsed_time(e1) / steps
+    B.close()
+    return {"patches": count, "cells_per_step": cells, "tile": list(tile),
+            "graph_ms_per_step": round(graph_ms, 5), "persistent_ms_per_step": round(eager_ms, 5),
+            "per_patch_launch_ms_per_step": round(per_patch_ms, 5),
+            "speedup_vs_per_patch": round(per_patch_ms / graph_ms, 2),
+            "gcells_per_s": round(cells / (graph_ms * 1e-3) / 1e9, 2),
+            "hbm_frac": round(16 * cells / (graph_ms * 1e-3) / HBM_PEAK, 4)}
+
+
+def adjoint_leg(n=128, steps=100, every=10):
+    """100 explicit-Euler steps of div(k grad phi) at 128^3 (periodic) with
+    checkpoints every 10 steps, then the checkpoint/recompute reverse sweep."""
+    import numpy as np
+    import torch
+    import nebo_hip as nh
+    from nebo_hip import inputs
+    from nebo_hip import adjoint as adj
+    L = nh.Layout.build((n, n, n), 1)
+    db = nh.build_stencils(n, n, n, 1.0, 1.0, 1.0)
+    c = nh.axis_coefs(db)
+    k = torch.from_numpy(inputs.coefficient_field(n, 5).reshape(-1)).cuda()
+    phi0 = torch.from_numpy(inputs.fourier_field(n, 3).reshape(-1)).cuda()
+    dt = 0.1 * (1.0 / n) ** 2 / 2.0
+    phiT, ckpt = adj.forward(L, phi0, k, c, dt, steps, every)
+    lamT = phiT.clone()
+    adj.backward(L, lamT, k, c, dt, steps, every, ckpt)
+    torch.cuda.synchronize()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    e0, e1, e2 = ev(), ev(), ev()
+    e0.record()
+    phiT, ckpt = adj.forward(L, phi0, k, c, dt, steps, every)
+    e1.record()
+    adj.backward(L, lamT, k, c, dt, steps, every, ckpt)
+    e2.record()
+    torch.cuda.synchronize()
+    fwd, bwd = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    return {"n": n, "steps": steps, "checkpoint_every": every, "forward_ms": round(fwd, 3),
+            "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3)}
+
+
+def pmc_probe(n):
+    """Three copies of one field (known bytes) and three fused kernel launches,
+    nothing else: the counter runs divide the kernel's FETCH/WRITE by the copy's."""
+    import torch
+    import nebo_hip as nh
+    from nebo_hip import inputs
+    L = nh.Layout.build((n, n, n), 1)
+    k = nh.Field.from_numpy(L, inputs.coefficient_field(n, 2).reshape(-1), where="gpu")
+    phi = nh.Field.from_numpy(L, inputs.fourier_field(n, 3).reshape(-1), where="gpu")
+    r = nh.Field.empty(L, "gpu")
+    dst = torch.empty_like(phi.gpu)
+    db = nh.build_stencils(n, n, n, 1.0, 1.0, 1.0)
+    c = nh.axis_coefs(db)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(phi.gpu)
+    for _ in range(3):
+        nh.divkgrad(r, k, phi, c, 0)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": True, "copy_bytes_read": L.npts * 8, "copy_bytes_written": L.npts * 8,
+                      "kernel_bytes_algorithmic": BYTES_PER_CELL * n ** 3}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.pmc_probe:
+        pmc_probe(a.n)
+        return
     import numpy as np
     import torch
     import nebo_hip as nh
@@ -114,6 +231,12 @@ def main():
         if rc:
             nh._abi.check(rc)
 
+    tune = None
+    if not a.no_tune:
+        best, times = nh.stencil_autotune(r, k, phi, coefs)
+        tune = {"tile": f"64x{best[0]}x{best[1]}",
+                "ms_per_tile": {f"64x{t[0]}x{t[1]}": round(v, 5) for t, v in times.items()}}
+    tile = nh.stencil_get_tile(0)
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
@@ -150,6 +273,11 @@ def main():
         except (OSError, ValueError):
             traffic = None
 
+    extras = {}
+    if not a.no_extras:
+        extras["batched"] = batched_leg()
+        extras["adjoint"] = adjoint_leg()
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(L, kh, ph, coefs, a.cpu_seconds)
@@ -160,13 +288,15 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded k ~ U[0.5,2), 8-mode Fourier phi, periodic ghosts)",
         "config": {"workload": f"div(k grad phi) {n}^3 fp64, 1 ghost layer, fused <<=",
-                   "n": n, "cells": cells, "kernel": "divkgrad (hand-written HIP, gfx950)"},
+                   "n": n, "cells": cells, "kernel": "divkgrad (hand-written HIP, gfx950)",
+                   "tile": f"64x{tile[0]}x{tile[1]}", "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "bytes_per_launch_algorithmic": BYTES_PER_CELL * cells,
                      "avg_kernel_ms": round(avg_kernel_ms, 5),
                      "median_kernel_ms": round(kernel_ms[len(kernel_ms) // 2], 5)},
         "cpu_baseline": cpu,
+        **extras,
         "wall_s_timed_region": round(wall, 4),
     }
     print(json.dumps(line))

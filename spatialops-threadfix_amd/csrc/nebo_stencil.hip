@@ -9,22 +9,26 @@
 // (window offset + valid ghost minus) and the shared strides X = glob[0],
 // XY = glob[0]*glob[1].
 //
-// Work split: a workgroup of 256 threads covers a 64 (x) by 4 (y) column tile
-// and marches through a chunk of z planes, keeping the column's z neighbours and
-// the shared z face in registers.  x and y neighbours come from the L1/L2
-// caches.  Workgroups are numbered so that consecutive tiles (which share halo
-// rows) land on the same XCD and therefore the same L2.
+// Work split: a workgroup of 64*TY threads covers a 64 (x) by TY (y) column tile
+// and marches through a chunk of ZC z planes, keeping the column's z neighbours
+// and the shared z face in registers; x and y neighbours come from the L1/L2
+// caches.  Halo rows and planes are re-read by the neighbouring tile, so larger
+// TY and ZC cut the traffic beyond L2 ((TY+2)/TY * (ZC+2)/ZC for the operands)
+// at the cost of fewer workgroups; the best shape is measured on the device
+// (nebo_stencil_autotune).  Workgroups are numbered so that consecutive tiles,
+// which share halo rows, land on the same XCD and therefore the same L2.
 
 #include "nebo_internal.h"
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <sstream>
+
 namespace nebo {
 namespace {
 
 constexpr int kTx = 64;
-constexpr int kTy = 4;
-constexpr int kZChunk = 16;
 
 // Hardware dispatches workgroup b to XCD b % 8.  Map b to a logical tile so that
 // each XCD receives one contiguous run of logical tiles.
@@ -40,12 +44,12 @@ struct Tiles {
   int nx, ny, nz;  // tile counts
 };
 
-Tiles make_tiles(const nebo_limits_t& L, int zc)
+Tiles make_tiles(const nebo_limits_t& L, int ty, int zc)
 {
   Tiles t;
   for (int d = 0; d < 3; ++d) { t.lo[d] = L.lo[d]; t.hi[d] = L.hi[d]; }
   t.nx = (L.hi[0] - L.lo[0] + kTx - 1) / kTx;
-  t.ny = (L.hi[1] - L.lo[1] + kTy - 1) / kTy;
+  t.ny = (L.hi[1] - L.lo[1] + ty - 1) / ty;
   t.nz = (L.hi[2] - L.lo[2] + zc - 1) / zc;
   return t;
 }
@@ -59,8 +63,9 @@ __device__ __forceinline__ double face(double klo, double khi, double plo, doubl
   return ki * gr;
 }
 
-template <int FORM>
-__global__ __launch_bounds__(kTx * kTy) void divkgrad_kernel(
+// FORM 0: (Dx + Dy) + Dz;  FORM 1: ((-Dx) - Dy) - Dz;  FORM 2: phi + dt*((Dx + Dy) + Dz)
+template <int FORM, int TY, int ZC>
+__global__ __launch_bounds__(kTx * TY) void divkgrad_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
     Tiles T, int X, int XY, nebo_axis_coefs_t c, double dt)
 {
@@ -70,10 +75,10 @@ __global__ __launch_bounds__(kTx * kTy) void divkgrad_kernel(
   const int ty = rest % T.ny;
   const int tz = rest / T.ny;
   const int x = T.lo[0] + tx * kTx + (threadIdx.x & (kTx - 1));
-  const int y = T.lo[1] + ty * kTy + (threadIdx.x / kTx);
+  const int y = T.lo[1] + ty * TY + (threadIdx.x / kTx);
   if (x >= T.hi[0] || y >= T.hi[1]) return;
-  const int z0 = T.lo[2] + tz * kZChunk;
-  const int z1 = min(z0 + kZChunk, T.hi[2]);
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
 
   const double ix0 = c.interp[0][0], ix1 = c.interp[0][1], gx0 = c.grad[0][0], gx1 = c.grad[0][1];
   const double iy0 = c.interp[1][0], iy1 = c.interp[1][1], gy0 = c.grad[1][0], gy1 = c.grad[1][1];
@@ -109,7 +114,8 @@ __global__ __launch_bounds__(kTx * kTy) void divkgrad_kernel(
 }
 
 // out = phi + s*((Lx + Ly) + Lz),  L_d = (phi[-1]*g0 + phi[0]*g1)*d0 + (phi[0]*g0 + phi[+1]*g1)*d1
-__global__ __launch_bounds__(kTx * kTy) void lapl_euler_kernel(
+template <int TY, int ZC>
+__global__ __launch_bounds__(kTx * TY) void lapl_euler_kernel(
     double* __restrict__ out, const double* __restrict__ p, Tiles T, int X, int XY,
     nebo_axis_coefs_t c, double s)
 {
@@ -119,10 +125,10 @@ __global__ __launch_bounds__(kTx * kTy) void lapl_euler_kernel(
   const int ty = rest % T.ny;
   const int tz = rest / T.ny;
   const int x = T.lo[0] + tx * kTx + (threadIdx.x & (kTx - 1));
-  const int y = T.lo[1] + ty * kTy + (threadIdx.x / kTx);
+  const int y = T.lo[1] + ty * TY + (threadIdx.x / kTx);
   if (x >= T.hi[0] || y >= T.hi[1]) return;
-  const int z0 = T.lo[2] + tz * kZChunk;
-  const int z1 = min(z0 + kZChunk, T.hi[2]);
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
   const double gx0 = c.grad[0][0], gx1 = c.grad[0][1], gy0 = c.grad[1][0], gy1 = c.grad[1][1];
   const double gz0 = c.grad[2][0], gz1 = c.grad[2][1];
   const double dx0 = c.div[0][0], dx1 = c.div[0][1], dy0 = c.div[1][0], dy1 = c.div[1][1];
@@ -143,6 +149,66 @@ __global__ __launch_bounds__(kTx * kTy) void lapl_euler_kernel(
     pc = pz;
     gz_lo = gz_hi;
   }
+}
+
+// ---- tile table -------------------------------------------------------------
+
+struct TileShape { int ty, zc; };
+constexpr TileShape kShapes[] = {{4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {8, 64},
+                                 {16, 16}, {16, 32}, {16, 64}, {2, 64}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+constexpr int kDefaultShape = 0;
+
+using DivkFn = void (*)(double*, const double*, const double*, Tiles, int, int, nebo_axis_coefs_t, double);
+using LaplFn = void (*)(double*, const double*, Tiles, int, int, nebo_axis_coefs_t, double);
+
+template <int FORM, int S>
+constexpr DivkFn divk_fn() { return divkgrad_kernel<FORM, kShapes[S].ty, kShapes[S].zc>; }
+template <int S>
+constexpr LaplFn lapl_fn() { return lapl_euler_kernel<kShapes[S].ty, kShapes[S].zc>; }
+
+template <int FORM>
+DivkFn divk_table(int s)
+{
+  static const DivkFn t[kNumShapes] = {divk_fn<FORM, 0>(), divk_fn<FORM, 1>(), divk_fn<FORM, 2>(),
+                                       divk_fn<FORM, 3>(), divk_fn<FORM, 4>(), divk_fn<FORM, 5>(),
+                                       divk_fn<FORM, 6>(), divk_fn<FORM, 7>(), divk_fn<FORM, 8>(),
+                                       divk_fn<FORM, 9>()};
+  return t[s];
+}
+
+LaplFn lapl_table(int s)
+{
+  static const LaplFn t[kNumShapes] = {lapl_fn<0>(), lapl_fn<1>(), lapl_fn<2>(), lapl_fn<3>(),
+                                       lapl_fn<4>(), lapl_fn<5>(), lapl_fn<6>(), lapl_fn<7>(),
+                                       lapl_fn<8>(), lapl_fn<9>()};
+  return t[s];
+}
+
+static_assert(kNumShapes == 10, "tables list every shape");
+
+std::atomic<int> g_divk_shape{kDefaultShape};
+std::atomic<int> g_lapl_shape{kDefaultShape};
+
+int find_shape(int ty, int zc)
+{
+  for (int s = 0; s < kNumShapes; ++s)
+    if (kShapes[s].ty == ty && kShapes[s].zc == zc) return s;
+  return -1;
+}
+
+int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_field_t* k,
+                      const nebo_field_t* phi, const nebo_axis_coefs_t& c, double dt,
+                      const nebo_limits_t& lim, void* stream)
+{
+  const TileShape sh = kShapes[shape];
+  Tiles T = make_tiles(lim, sh.ty, sh.zc);
+  const int nb = T.nx * T.ny * T.nz;
+  const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
+  DivkFn fn = form == 0 ? divk_table<0>(shape) : form == 1 ? divk_table<1>(shape) : divk_table<2>(shape);
+  hipLaunchKernelGGL(fn, dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream, frame_origin(r),
+                     frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
+  return check_hip(hipGetLastError(), "Nebo Assignment");
 }
 
 // Periodic fill of the ghost slab of axis d: every ghost cell takes the value of
@@ -189,39 +255,25 @@ __global__ void fill_periodic_kernel(double* __restrict__ v, int d, int X, int Y
 int launch_divkgrad(const nebo_field_t* r, const nebo_field_t* k, const nebo_field_t* phi,
                     const nebo_axis_coefs_t& c, int form, const nebo_limits_t& lim, void* stream)
 {
-  Tiles T = make_tiles(lim, kZChunk);
-  const int nb = T.nx * T.ny * T.nz;
-  const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
-  hipStream_t s = (hipStream_t)stream;
-  if (form == 0)
-    hipLaunchKernelGGL(divkgrad_kernel<0>, dim3(nb), dim3(kTx * kTy), 0, s, frame_origin(r),
-                       frame_origin(k), frame_origin(phi), T, X, XY, c, 0.0);
-  else
-    hipLaunchKernelGGL(divkgrad_kernel<1>, dim3(nb), dim3(kTx * kTy), 0, s, frame_origin(r),
-                       frame_origin(k), frame_origin(phi), T, X, XY, c, 0.0);
-  return check_hip(hipGetLastError(), "Nebo Assignment");
+  return launch_divk_shape(g_divk_shape.load(), form, r, k, phi, c, 0.0, lim, stream);
 }
 
 int launch_diffusion_euler(const nebo_field_t* out, const nebo_field_t* k, const nebo_field_t* phi,
                            const nebo_axis_coefs_t& c, double dt, const nebo_limits_t& lim,
                            void* stream)
 {
-  Tiles T = make_tiles(lim, kZChunk);
-  const int nb = T.nx * T.ny * T.nz;
-  const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
-  hipLaunchKernelGGL(divkgrad_kernel<2>, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream,
-                     frame_origin(out), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
-  return check_hip(hipGetLastError(), "Nebo Diffusion");
+  return launch_divk_shape(g_divk_shape.load(), 2, out, k, phi, c, dt, lim, stream);
 }
 
 int launch_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
                       const nebo_axis_coefs_t& c, double sc, const nebo_limits_t& lim,
                       void* stream)
 {
-  Tiles T = make_tiles(lim, kZChunk);
+  const TileShape sh = kShapes[g_lapl_shape.load()];
+  Tiles T = make_tiles(lim, sh.ty, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
-  hipLaunchKernelGGL(lapl_euler_kernel, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(lapl_table(g_lapl_shape.load()), dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream,
                      frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
   return check_hip(hipGetLastError(), "Nebo Assignment");
 }
@@ -267,15 +319,18 @@ int check_same_shape(const nebo_field_t* a, const nebo_field_t* b, const char* c
   return NEBO_OK;
 }
 
-int interior(const nebo_field_t* f, nebo_limits_t* L, const char* ctx)
+// Cells of f's interior whose one-cell stencil reach stays inside the valid
+// ghosts: [max(0, 1 - valid_minus), ext - max(0, 1 - valid_plus)).  *empty when
+// no cell qualifies (e.g. an axis of extent 1 without ghosts).
+void stencil_range(const nebo_field_t* f, nebo_limits_t* L, int* empty)
 {
+  *empty = 0;
   for (int d = 0; d < 3; ++d) {
-    if (f->valid_minus[d] < 1 || f->valid_plus[d] < 1)
-      return set_error(NEBO_ERR_GHOST, ctx, "the stencil needs one valid ghost layer on each face");
-    L->lo[d] = 0;
-    L->hi[d] = f->extent[d] - f->valid_minus[d] - f->valid_plus[d];
+    const int ext = f->extent[d] - f->valid_minus[d] - f->valid_plus[d];
+    L->lo[d] = f->valid_minus[d] >= 1 ? 0 : 1 - f->valid_minus[d];
+    L->hi[d] = ext - (f->valid_plus[d] >= 1 ? 0 : 1 - f->valid_plus[d]);
+    if (L->hi[d] <= L->lo[d]) *empty = 1;
   }
-  return NEBO_OK;
 }
 
 }  // namespace
@@ -304,8 +359,9 @@ extern "C" int nebo_divkgrad(const nebo_field_t* r, const nebo_field_t* k,
   if (r->values == k->values || r->values == phi->values)
     return set_error(NEBO_ERR_INVALID, ctx, "result must not alias an operand");
   nebo_limits_t L;
-  if ((rc = interior(k, &L, ctx))) return rc;
-  for (int d = 0; d < 3; ++d) L.hi[d] = r->extent[d] - r->valid_minus[d] - r->valid_plus[d];
+  int empty = 0;
+  stencil_range(k, &L, &empty);
+  if (empty) return NEBO_OK;
   return launch_divkgrad(r, k, phi, *c, form, L, stream);
 }
 
@@ -319,6 +375,87 @@ extern "C" int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
   if ((rc = check_same_shape(out, phi, ctx))) return rc;
   if (out->values == phi->values) return set_error(NEBO_ERR_INVALID, ctx, "result must not alias the operand");
   nebo_limits_t L;
-  if ((rc = interior(phi, &L, ctx))) return rc;
+  int empty = 0;
+  stencil_range(phi, &L, &empty);
+  if (empty) return NEBO_OK;
   return launch_lapl_euler(out, phi, *c, s, L, stream);
+}
+
+extern "C" int nebo_stencil_set_tile(int kernel, int32_t ty, int32_t zchunk)
+{
+  clear_error();
+  const int s = find_shape(ty, zchunk);
+  if (s < 0 || (kernel != 0 && kernel != 1)) {
+    std::ostringstream m;
+    m << "no stencil kernel instantiated for kernel " << kernel << ", tile 64x" << ty << "x" << zchunk;
+    return set_error(NEBO_ERR_INVALID, "Nebo Stencil Tile", m.str());
+  }
+  (kernel == 0 ? g_divk_shape : g_lapl_shape).store(s);
+  return NEBO_OK;
+}
+
+extern "C" int nebo_stencil_get_tile(int kernel, int32_t* ty, int32_t* zchunk)
+{
+  clear_error();
+  if (kernel != 0 && kernel != 1) return set_error(NEBO_ERR_INVALID, "Nebo Stencil Tile", "unknown kernel");
+  const TileShape sh = kShapes[(kernel == 0 ? g_divk_shape : g_lapl_shape).load()];
+  if (ty) *ty = sh.ty;
+  if (zchunk) *zchunk = sh.zc;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_stencil_autotune(const nebo_field_t* r, const nebo_field_t* k,
+                                     const nebo_field_t* phi, const nebo_axis_coefs_t* c,
+                                     void* stream, int32_t* ty, int32_t* zchunk, float* times_ms)
+{
+  static const char* ctx = "Nebo Stencil Tile";
+  clear_error();
+  if (!r || !k || !phi || !c) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  int rc;
+  if ((rc = check_same_shape(r, k, ctx)) || (rc = check_same_shape(r, phi, ctx))) return rc;
+  if (r->values == k->values || r->values == phi->values)
+    return set_error(NEBO_ERR_INVALID, ctx, "result must not alias an operand");
+  nebo_limits_t L;
+  int empty = 0;
+  stencil_range(k, &L, &empty);
+  if (empty) return set_error(NEBO_ERR_INVALID, ctx, "no interior cells to time");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  if ((rc = check_hip(hipEventCreate(&e0), ctx))) return rc;
+  if ((rc = check_hip(hipEventCreate(&e1), ctx))) return rc;
+  int best = g_divk_shape.load();
+  float best_ms = 1e30f;
+  for (int sh = 0; sh < kNumShapes && !rc; ++sh) {
+    rc = launch_divk_shape(sh, 0, r, k, phi, *c, 0.0, L, stream);   // warm
+    if (!rc) rc = check_hip(hipEventRecord(e0, s), ctx);
+    for (int rep = 0; rep < 5 && !rc; ++rep) rc = launch_divk_shape(sh, 0, r, k, phi, *c, 0.0, L, stream);
+    if (!rc) rc = check_hip(hipEventRecord(e1, s), ctx);
+    if (!rc) rc = check_hip(hipEventSynchronize(e1), ctx);
+    float ms = 0.f;
+    if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), ctx);
+    ms /= 5.0f;
+    if (times_ms) times_ms[sh] = ms;
+    if (!rc && ms < best_ms) {
+      best_ms = ms;
+      best = sh;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  g_divk_shape.store(best);
+  if (ty) *ty = kShapes[best].ty;
+  if (zchunk) *zchunk = kShapes[best].zc;
+  return NEBO_OK;
+}
+
+extern "C" int nebo_stencil_tiles(int32_t* n, int32_t* ty, int32_t* zchunk)
+{
+  clear_error();
+  if (n) *n = kNumShapes;
+  for (int s = 0; s < kNumShapes; ++s) {
+    if (ty) ty[s] = kShapes[s].ty;
+    if (zchunk) zchunk[s] = kShapes[s].zc;
+  }
+  return NEBO_OK;
 }

@@ -11,4 +11,7 @@ from .ops import build_stencils, basic_op_types, axis_coefs, OperatorDatabase  #
 from .api import (assign, interior_assign, masked_assign, assign_limits, assign_plan,  # noqa: F401
                   field_max, field_min, field_sum, field_norm, field_max_interior,
                   field_min_interior, field_sum_interior, field_norm_interior,
-                  fill_ghosts_periodic, divkgrad, lapl_euler, device_count)
+                  fill_ghosts_periodic, divkgrad, lapl_euler, device_count, stencil_tiles,
+                  stencil_set_tile, stencil_get_tile, stencil_autotune)
+from .batch import Batch, alloc_patches  # noqa: F401,E402
+from . import adjoint  # noqa: F401,E402

@@ -111,11 +111,7 @@ class AxisCoefsT(C.Structure):
 
 
 class PatchT(C.Structure):
-    _fields_ = [
-        ("inp", C.c_void_p), ("out", C.c_void_p),
-        ("n", I3), ("pad_", C.c_int32),
-        ("inv_dx2", C.c_double * 3),
-    ]
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("n", I3), ("pad_", C.c_int32)]
 
 
 P = C.POINTER
@@ -143,6 +139,11 @@ _SIGS = {
     "nebo_fill_ghosts_periodic": (C.c_int, [P(FieldT), _vp]),
     "nebo_divkgrad": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), C.c_int, _vp]),
     "nebo_lapl_euler": (C.c_int, [P(FieldT), P(FieldT), P(AxisCoefsT), C.c_double, _vp]),
+    "nebo_stencil_tiles": (C.c_int, [P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+    "nebo_stencil_set_tile": (C.c_int, [C.c_int, C.c_int32, C.c_int32]),
+    "nebo_stencil_get_tile": (C.c_int, [C.c_int, P(C.c_int32), P(C.c_int32)]),
+    "nebo_stencil_autotune": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), _vp,
+                                        P(C.c_int32), P(C.c_int32), P(C.c_float)]),
     "nebo_batch_create": (C.c_int, [P(PatchT), C.c_int32, P(AxisCoefsT), C.c_double, P(C.c_int32), P(_vp)]),
     "nebo_batch_destroy": (C.c_int, [_vp]),
     "nebo_batch_step": (C.c_int, [_vp, C.c_int, _vp]),
@@ -150,6 +151,7 @@ _SIGS = {
     "nebo_batch_graph_build": (C.c_int, [_vp, C.c_int32, _vp]),
     "nebo_batch_graph_launch": (C.c_int, [_vp, _vp]),
     "nebo_batch_autotune": (C.c_int, [_vp, P(C.c_int32), _vp]),
+    "nebo_batch_info": (C.c_int, [_vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
     "nebo_diffusion_step": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), C.c_double, _vp]),
     "nebo_diffusion_adjoint_step": (C.c_int, [P(FieldT), _vp, P(FieldT), P(FieldT), P(FieldT),
                                               P(AxisCoefsT), C.c_double, _vp]),

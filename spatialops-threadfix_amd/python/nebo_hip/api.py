@@ -106,3 +106,34 @@ def device_count() -> int:
     n = C.c_int(0)
     _abi.call("nebo_device_count", C.byref(n))
     return n.value
+
+
+def stencil_tiles():
+    """(ty, zchunk) shapes instantiated for the hand-written stencil kernels."""
+    n = C.c_int32()
+    _abi.call("nebo_stencil_tiles", C.byref(n), None, None)
+    ty, zc = (C.c_int32 * n.value)(), (C.c_int32 * n.value)()
+    _abi.call("nebo_stencil_tiles", C.byref(n), ty, zc)
+    return list(zip(ty, zc))
+
+
+def stencil_set_tile(kernel, ty, zchunk):
+    _abi.call("nebo_stencil_set_tile", int(kernel), int(ty), int(zchunk))
+
+
+def stencil_get_tile(kernel=0):
+    ty, zc = C.c_int32(), C.c_int32()
+    _abi.call("nebo_stencil_get_tile", int(kernel), C.byref(ty), C.byref(zc))
+    return ty.value, zc.value
+
+
+def stencil_autotune(r: Field, k: Field, phi: Field, coefs, stream=None):
+    """Time every tile of the diffusion kernel; keep the fastest.  Returns
+    ((ty, zchunk), {tile: ms})."""
+    tiles = stencil_tiles()
+    times = (C.c_float * len(tiles))()
+    ty, zc = C.c_int32(), C.c_int32()
+    dr, dk, dp = r.desc("gpu"), k.desc("gpu"), phi.desc("gpu")
+    _abi.call("nebo_stencil_autotune", C.byref(dr), C.byref(dk), C.byref(dp), C.byref(coefs),
+              _stream(stream), C.byref(ty), C.byref(zc), times)
+    return (ty.value, zc.value), dict(zip(tiles, list(times)))

@@ -1,0 +1,211 @@
+"""GPU tests of the batched persistent kernel (many uneven patches per launch,
+hipGraph replay, autotuned tiles) and of the adjoint diffusion kernels with
+checkpoint / recompute, against the oracle and finite differences."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import nebo_hip as nh
+from nebo_hip import inputs
+from nebo_hip import adjoint as adj
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _patch_dims(count, seed):
+    sizes = np.array([16, 24, 32, 48, 64])
+    idx = (inputs.splitmix64(seed, 3 * count) % np.uint64(5)).astype(np.int64).reshape(count, 3)
+    return [tuple(int(v) for v in sizes[i]) for i in idx]
+
+
+def _oracle_patch_step(d, a_host, coefs, s):
+    L = nh.Layout.build(d, 1)
+    src = nh.Field.from_numpy(L, a_host, where="cpu")
+    dst = nh.Field.from_numpy(L, a_host.copy(), where="cpu")   # ghosts carried over
+    orc.lapl_euler(dst, src, coefs, s)
+    return dst.host
+
+
+@pytest.mark.parametrize("count", [1, 7, 40])
+def test_batch_matches_oracle_and_per_patch(count):
+    dims = _patch_dims(count, 4 + count)
+    db = nh.build_stencils(32, 32, 32, 1.0, 1.0, 1.0)
+    coefs = nh.axis_coefs(db)
+    s = 0.1 / (32.0 ** 2)
+    rng = np.random.default_rng(count)
+    init = [rng.standard_normal((d[0] + 2) * (d[1] + 2) * (d[2] + 2)) for d in dims]
+    a, b = nh.alloc_patches(dims)
+    a2, b2 = nh.alloc_patches(dims)
+    for x, x2, h in zip(a, a2, init):
+        x.copy_(torch.from_numpy(h))
+        x2.copy_(torch.from_numpy(h))
+    for x, x2, h in zip(b, b2, init):  # ghost values of the b blocks = those of a
+        x.copy_(torch.from_numpy(h))
+        x2.copy_(torch.from_numpy(h))
+    B = nh.Batch(a, b, dims, coefs, s)
+    B2 = nh.Batch(a2, b2, dims, coefs, s, tile=(16, 8, 4))
+    for step in range(4):
+        B.step(swap=step & 1)
+        B2.step_per_patch(swap=step & 1)
+    torch.cuda.synchronize()
+    host = [h.copy() for h in init]
+    for step in range(4):
+        host = [_oracle_patch_step(d, h, coefs, s) for d, h in zip(dims, host)]
+    for i in range(count):
+        got = a[i].cpu().numpy()   # 4 steps end in block a
+        assert np.array_equal(got, host[i]), f"patch {i} dims {dims[i]}"
+        assert np.array_equal(a2[i].cpu().numpy(), host[i])
+    B.close()
+    B2.close()
+
+
+def test_batch_graph_replays_and_autotune():
+    dims = _patch_dims(64, 4)
+    db = nh.build_stencils(32, 32, 32, 1.0, 1.0, 1.0)
+    coefs = nh.axis_coefs(db)
+    s = 0.05 / (32.0 ** 2)
+    rng = np.random.default_rng(1)
+    init = [rng.standard_normal((d[0] + 2) * (d[1] + 2) * (d[2] + 2)) for d in dims]
+    a, b = nh.alloc_patches(dims)
+    ea, eb = nh.alloc_patches(dims)
+    for blocks in (a, b, ea, eb):
+        for x, h in zip(blocks, init):
+            x.copy_(torch.from_numpy(h))
+    B = nh.Batch(a, b, dims, coefs, s)
+    tile = B.autotune()
+    assert len(tile) == 3 and tile[0] * tile[1] <= 1024
+    for x, h in zip(a, init):       # autotune stepped a -> b; restore
+        x.copy_(torch.from_numpy(h))
+    for x, h in zip(b, init):
+        x.copy_(torch.from_numpy(h))
+    E = nh.Batch(ea, eb, dims, coefs, s, tile=(64, 4, 16))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        B.graph_build(10, stream)
+        for rep in range(3):
+            B.graph_launch(stream)
+            stream.synchronize()
+            for st in range(10):
+                E.step(swap=st & 1, stream=stream)
+            stream.synchronize()
+            # eager work between replays, then compare every patch
+            for i in range(len(dims)):
+                assert torch.equal(a[i], ea[i]), f"replay {rep} patch {i}"
+    B.close()
+    E.close()
+
+
+# --------------------------------------------------------------------------
+# adjoint
+# --------------------------------------------------------------------------
+
+def _periodic(L, arr):
+    a = np.zeros(L.glob[::-1])
+    n = L.interior
+    a[1:-1, 1:-1, 1:-1] = arr.reshape(n[2], n[1], n[0])
+    return inputs.periodic_fill(a, 1).reshape(-1)
+
+
+@pytest.mark.parametrize("n", [(16, 16, 16), (13, 10, 9)])
+def test_diffusion_step_and_adjoint_bit_exact(n):
+    L = nh.Layout.build(n, 1)
+    db = nh.build_stencils(*n, 1.0, 1.0, 1.0)
+    c = nh.axis_coefs(db)
+    rng = np.random.default_rng(sum(n))
+    N = n[0] * n[1] * n[2]
+    k = _periodic(L, rng.uniform(0.5, 2.0, N))
+    phi = _periodic(L, rng.standard_normal(N))
+    lam = _periodic(L, rng.standard_normal(N))
+    dt = 0.1 / (max(n) ** 2 * 2.0)
+    g = {name: torch.from_numpy(v).cuda() for name, v in (("k", k), ("phi", phi), ("lam", lam))}
+    out = torch.zeros_like(g["phi"])
+    adj.diffusion_step(L, out, g["phi"], g["k"], c, dt)
+    lam_out = torch.zeros_like(g["phi"])
+    gk = torch.zeros_like(g["phi"])
+    adj.diffusion_adjoint_step(L, lam_out, gk, g["lam"], g["phi"], g["k"], c, dt)
+    torch.cuda.synchronize()
+    ro = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+    orc.diffusion_step(ro, nh.Field(L, host=phi), nh.Field(L, host=k), c, dt)
+    rl = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+    rg = np.zeros(L.npts)
+    orc.diffusion_adjoint_step(rl, rg, nh.Field(L, host=lam), nh.Field(L, host=phi), nh.Field(L, host=k), c, dt)
+    assert np.array_equal(out.cpu().numpy(), ro.host)
+    assert np.array_equal(lam_out.cpu().numpy(), rl.host)
+    assert np.array_equal(gk.cpu().numpy(), rg)
+
+
+def _trajectory_case(n, steps, seed=5):
+    L = nh.Layout.build((n, n, n), 1)
+    db = nh.build_stencils(n, n, n, 1.0, 1.0, 1.0)
+    c = nh.axis_coefs(db)
+    N = n ** 3
+    rng = np.random.default_rng(seed)
+    k = _periodic(L, inputs.uniform(2, N, 0.5, 2.0))
+    phi0 = inputs.fourier_field(n, 3).reshape(-1)
+    dt = 0.1 * (1.0 / n) ** 2 / 2.0
+    return L, c, k, phi0, dt, rng
+
+
+def test_checkpointed_trajectory_bit_exact_vs_oracle():
+    n, steps, every = 12, 8, 4
+    L, c, k, phi0, dt, rng = _trajectory_case(n, steps)
+    target = _periodic(L, rng.standard_normal(n ** 3))
+    gk_, gp_ = torch.from_numpy(k).cuda(), torch.from_numpy(phi0).cuda()
+    phiT, ckpt = adj.forward(L, gp_, gk_, c, dt, steps, every)
+    lamT = phiT - torch.from_numpy(target).cuda()
+    lam0, gk = adj.backward(L, lamT, gk_, c, dt, steps, every, ckpt)
+    torch.cuda.synchronize()
+    # oracle: plain loop keeping every state
+    states = [phi0.copy()]
+    for _ in range(steps):
+        o = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+        orc.diffusion_step(o, nh.Field(L, host=states[-1]), nh.Field(L, host=k), c, dt)
+        states.append(o.host)
+    assert np.array_equal(phiT.cpu().numpy(), states[-1])
+    lam = states[-1] - target
+    lam = inputs.periodic_fill(lam.reshape(L.glob[::-1]).copy(), 1).reshape(-1)
+    g = np.zeros(L.npts)
+    for s in range(steps - 1, -1, -1):
+        o = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+        orc.diffusion_adjoint_step(o, g, nh.Field(L, host=lam), nh.Field(L, host=states[s]),
+                                   nh.Field(L, host=k), c, dt)
+        lam = o.host
+    assert np.array_equal(lam0.cpu().numpy(), lam)
+    assert np.array_equal(gk.cpu().numpy(), g)
+
+
+def test_autograd_gradient_matches_finite_differences():
+    """Adjoint: dL/dk and dL/dphi0 within 1e-6 (relative) of central differences
+    (eps = 1e-6) on random directions; 32^3, 20 steps, checkpoint every 5."""
+    n, steps, every = 32, 20, 5
+    L, c, k, phi0, dt, rng = _trajectory_case(n, steps)
+    target = torch.from_numpy(_periodic(L, rng.standard_normal(n ** 3))).cuda()
+    inner = torch.zeros(L.glob[::-1], dtype=torch.bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner = inner.reshape(-1).cuda()
+
+    def loss_of(kk, pp):
+        out = adj.DiffusionTrajectory.apply(pp, kk, L, c, dt, steps, every)
+        d = (out - target)[inner]
+        return 0.5 * (d * d).sum()
+
+    kt = torch.from_numpy(k).cuda().requires_grad_(True)
+    pt = torch.from_numpy(phi0).cuda().requires_grad_(True)
+    loss = loss_of(kt, pt)
+    loss.backward()
+    gk, gp = kt.grad.detach(), pt.grad.detach()
+    eps = 1e-6
+    with torch.no_grad():
+        for j in range(6):
+            v = torch.from_numpy(_periodic(L, rng.standard_normal(n ** 3))).cuda()
+            for which, grad in (("k", gk), ("phi", gp)):
+                if which == "k":
+                    lp, lm = loss_of(kt + eps * v, pt), loss_of(kt - eps * v, pt)
+                else:
+                    lp, lm = loss_of(kt, pt + eps * v), loss_of(kt, pt - eps * v)
+                fd = (lp - lm).item() / (2 * eps)
+                an = (grad[inner] * v[inner]).sum().item()
+                assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (which, j, fd, an)
