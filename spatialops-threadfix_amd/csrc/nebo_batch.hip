@@ -115,7 +115,12 @@ int build_items(nebo_batch* b)
         for (int x = 0; x < p.n[0]; x += tx) items.push_back(make_int4((int)k, x, y, z));
   }
   b->patch_first[b->patches.size()] = (int)items.size();
-  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_items) {
+    // launches still in flight may read the old table
+    int rc0 = check_hip(hipDeviceSynchronize(), ctx);
+    if (rc0) return rc0;
+    (void)hipFree(b->d_items);
+  }
   b->d_items = nullptr;
   b->n_items = (int)items.size();
   int rc = check_hip(hipMalloc((void**)&b->d_items, sizeof(int4) * std::max<size_t>(1, items.size())), ctx);
