@@ -1,0 +1,5 @@
+// FVStaggered.h -- field types, operator types and the expression language.
+#pragma once
+#include <spatialops/structured/SpatialField.h>
+#include <spatialops/Nebo.h>
+#include <spatialops/NeboStencilBuilder.h>

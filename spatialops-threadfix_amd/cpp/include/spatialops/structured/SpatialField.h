@@ -1,0 +1,414 @@
+// SpatialField.h -- fields, memory windows, ghost data and boundary-cell info of
+// the C++ front end.  Same names and meaning as the reference's
+// spatialops/structured/{SpatialField,MemoryWindow,GhostData,BoundaryCellInfo,
+// FVStaggeredLocationTypes,FVStaggeredFieldTypes}.h; storage lives on the host
+// and/or on GPUs through the engine's C-ABI (include/nebo_hip.h).
+#pragma once
+
+#include <nebo_hip.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <ostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#define CPU_INDEX (-1)
+#define GPU_INDEX 0
+#define IS_CPU_INDEX(i) ((i) == CPU_INDEX)
+#define IS_GPU_INDEX(i) ((i) >= 0)
+#define GHOST_MAX NEBO_GHOST_MAX
+
+namespace SpatialOps {
+
+// Rethrow a C-ABI failure as the reference's std::runtime_error("Nebo error in ...").
+inline void nebo_check(int rc)
+{
+  if (rc != NEBO_OK) throw std::runtime_error(nebo_last_error());
+}
+
+// ---------------------------------------------------------------------------
+// IntVec (spatialops/structured/IntVec.h)
+// ---------------------------------------------------------------------------
+struct IntVec {
+  int v[3];
+  IntVec(int x = 0, int y = 0, int z = 0) : v{x, y, z} {}
+  int& operator[](int i) { return v[i]; }
+  int operator[](int i) const { return v[i]; }
+  IntVec operator+(const IntVec& o) const { return IntVec(v[0] + o.v[0], v[1] + o.v[1], v[2] + o.v[2]); }
+  IntVec operator-(const IntVec& o) const { return IntVec(v[0] - o.v[0], v[1] - o.v[1], v[2] - o.v[2]); }
+  IntVec operator*(const IntVec& o) const { return IntVec(v[0] * o.v[0], v[1] * o.v[1], v[2] * o.v[2]); }
+  IntVec operator+(int s) const { return IntVec(v[0] + s, v[1] + s, v[2] + s); }
+  bool operator==(const IntVec& o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+  bool operator!=(const IntVec& o) const { return !(*this == o); }
+};
+inline std::ostream& operator<<(std::ostream& os, const IntVec& v)
+{
+  return os << "[" << v[0] << "," << v[1] << "," << v[2] << "]";
+}
+
+struct XDIR { enum { value = 0 }; };
+struct YDIR { enum { value = 1 }; };
+struct ZDIR { enum { value = 2 }; };
+struct NODIR { enum { value = -1 }; };
+
+// ---------------------------------------------------------------------------
+// GhostData (GhostData.h:54-257): counts >= GHOST_MAX on all six faces = infinite
+// ---------------------------------------------------------------------------
+class GhostData {
+  IntVec minus_, plus_;
+
+ public:
+  GhostData(int n = 0) : minus_(n, n, n), plus_(n, n, n) {}
+  GhostData(int nx, int px, int ny, int py, int nz, int pz) : minus_(nx, ny, nz), plus_(px, py, pz) {}
+  GhostData(const IntVec& m, const IntVec& p) : minus_(m), plus_(p) {}
+  IntVec get_minus() const { return minus_; }
+  IntVec get_plus() const { return plus_; }
+  int get_minus(int i) const { return minus_[i]; }
+  int get_plus(int i) const { return plus_[i]; }
+  void set_minus(const IntVec& m) { minus_ = m; }
+  void set_plus(const IntVec& p) { plus_ = p; }
+  bool operator==(const GhostData& o) const { return minus_ == o.minus_ && plus_ == o.plus_; }
+  GhostData limit_by_extent(const IntVec& e) const
+  {
+    return GhostData(e[0] <= 1 ? 0 : minus_[0], e[0] <= 1 ? 0 : plus_[0], e[1] <= 1 ? 0 : minus_[1],
+                     e[1] <= 1 ? 0 : plus_[1], e[2] <= 1 ? 0 : minus_[2], e[2] <= 1 ? 0 : plus_[2]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Locations (FVStaggeredLocationTypes.h:63-126): staggering offset and the
+// extra cell a field gets on a (+) face that carries a boundary condition.
+// ---------------------------------------------------------------------------
+#define NEBO_LOCATION(NAME, DIR, OX, OY, OZ, EX, EY, EZ)                          \
+  struct NAME {                                                                  \
+    typedef DIR FaceDir;                                                         \
+    static IntVec offset() { return IntVec(OX, OY, OZ); }                        \
+    static IntVec bc_extra() { return IntVec(EX, EY, EZ); }                      \
+    static constexpr int ox = OX, oy = OY, oz = OZ;                              \
+  };
+NEBO_LOCATION(SVol, NODIR, 0, 0, 0, 0, 0, 0)
+NEBO_LOCATION(SSurfX, XDIR, -1, 0, 0, 1, 0, 0)
+NEBO_LOCATION(SSurfY, YDIR, 0, -1, 0, 0, 1, 0)
+NEBO_LOCATION(SSurfZ, ZDIR, 0, 0, -1, 0, 0, 1)
+NEBO_LOCATION(XVol, NODIR, -1, 0, 0, 1, 0, 0)
+NEBO_LOCATION(XSurfX, XDIR, 0, 0, 0, 0, 0, 0)
+NEBO_LOCATION(XSurfY, YDIR, -1, -1, 0, 0, 1, 0)
+NEBO_LOCATION(XSurfZ, ZDIR, -1, 0, -1, 0, 0, 1)
+NEBO_LOCATION(YVol, NODIR, 0, -1, 0, 0, 1, 0)
+NEBO_LOCATION(YSurfX, XDIR, -1, -1, 0, 1, 0, 0)
+NEBO_LOCATION(YSurfY, YDIR, 0, 0, 0, 0, 0, 0)
+NEBO_LOCATION(YSurfZ, ZDIR, 0, -1, -1, 0, 0, 1)
+NEBO_LOCATION(ZVol, NODIR, 0, 0, -1, 0, 0, 1)
+NEBO_LOCATION(ZSurfX, XDIR, -1, 0, -1, 1, 0, 0)
+NEBO_LOCATION(ZSurfY, YDIR, 0, -1, -1, 0, 1, 0)
+NEBO_LOCATION(ZSurfZ, ZDIR, 0, 0, 0, 0, 0, 0)
+NEBO_LOCATION(SingleValue, NODIR, 0, 0, 0, 0, 0, 0)
+#undef NEBO_LOCATION
+
+// ---------------------------------------------------------------------------
+// BoundaryCellInfo (BoundaryCellInfo.h:45-158)
+// ---------------------------------------------------------------------------
+class BoundaryCellInfo {
+  IntVec isbc_, nExtra_;
+  BoundaryCellInfo(bool bx, bool by, bool bz, const IntVec& extra)
+      : isbc_(bx, by, bz), nExtra_(IntVec(bx, by, bz) * extra) {}
+
+ public:
+  template <typename FieldT>
+  static BoundaryCellInfo build(bool bx, bool by, bool bz)
+  {
+    return BoundaryCellInfo(bx, by, bz, FieldT::Location::bc_extra());
+  }
+  template <typename FieldT>
+  static BoundaryCellInfo build(const IntVec& bc)
+  {
+    return build<FieldT>(bc[0] != 0, bc[1] != 0, bc[2] != 0);
+  }
+  template <typename FieldT>
+  static BoundaryCellInfo build() { return build<FieldT>(false, false, false); }
+  bool has_bc(int d) const { return isbc_[d] != 0; }
+  IntVec has_bc() const { return isbc_; }
+  int num_extra(int d) const { return nExtra_[d]; }
+  IntVec num_extra() const { return nExtra_; }
+  int has_extra(int d) const { return has_bc(d) ? nExtra_[d] : 0; }
+  IntVec has_extra() const { return IntVec(has_extra(0), has_extra(1), has_extra(2)); }
+  BoundaryCellInfo limit_by_extent(const IntVec& e) const
+  {
+    BoundaryCellInfo b(*this);
+    for (int d = 0; d < 3; ++d)
+      if (e[d] == 1) b.nExtra_[d] = 0;
+    return b;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// MemoryWindow (MemoryWindow.h:63-321)
+// ---------------------------------------------------------------------------
+class MemoryWindow {
+  IntVec glob_, offset_, extent_;
+
+ public:
+  explicit MemoryWindow(const IntVec& glob) : glob_(glob), offset_(0, 0, 0), extent_(glob) {}
+  MemoryWindow(const IntVec& glob, const IntVec& offset, const IntVec& extent)
+      : glob_(glob), offset_(offset), extent_(extent) {}
+  const IntVec& glob_dim() const { return glob_; }
+  const IntVec& offset() const { return offset_; }
+  const IntVec& extent() const { return extent_; }
+  size_t glob_dim(int i) const { return (size_t)glob_[i]; }
+  size_t offset(int i) const { return (size_t)offset_[i]; }
+  size_t extent(int i) const { return (size_t)extent_[i]; }
+  size_t glob_npts() const { return (size_t)glob_[0] * glob_[1] * glob_[2]; }
+  size_t local_npts() const { return (size_t)extent_[0] * extent_[1] * extent_[2]; }
+  int flat_index(IntVec loc) const
+  {
+    for (int d = 0; d < 3; ++d) loc[d] = glob_[d] > 1 ? loc[d] + offset_[d] : 0;
+    return loc[0] + glob_[0] * (loc[1] + glob_[1] * loc[2]);
+  }
+  bool operator==(const MemoryWindow& o) const
+  {
+    return glob_ == o.glob_ && offset_ == o.offset_ && extent_ == o.extent_;
+  }
+  bool operator!=(const MemoryWindow& o) const { return !(*this == o); }
+};
+
+inline int get_dim_with_ghost(int n, int gm, int gp, int bc) { return n > 1 ? n + gm + gp + bc : 1; }
+
+inline MemoryWindow get_window_with_ghost(const IntVec& n, const GhostData& g, const BoundaryCellInfo& bc)
+{
+  return MemoryWindow(IntVec(get_dim_with_ghost(n[0], g.get_minus(0), g.get_plus(0), bc.has_extra(0)),
+                             get_dim_with_ghost(n[1], g.get_minus(1), g.get_plus(1), bc.has_extra(1)),
+                             get_dim_with_ghost(n[2], g.get_minus(2), g.get_plus(2), bc.has_extra(2))));
+}
+
+enum StorageMode { InternalStorage, ExternalStorage };
+
+// ---------------------------------------------------------------------------
+// Field storage with one copy per device and validity flags (FieldInfo.h:91-795)
+// ---------------------------------------------------------------------------
+template <typename T>
+class FieldStorage {
+ public:
+  struct Copy {
+    T* ptr = nullptr;
+    bool valid = false;
+    bool owned = false;
+  };
+
+  FieldStorage(size_t n, T* external, StorageMode mode, short dev) : n_(n), active_(dev)
+  {
+    Copy& c = copies_[dev];
+    if (mode == ExternalStorage && external) {
+      c.ptr = external;
+    } else {
+      c.ptr = allocate(dev);
+      c.owned = true;
+    }
+    c.valid = true;
+  }
+  ~FieldStorage()
+  {
+    for (auto& kv : copies_)
+      if (kv.second.owned) release(kv.first, kv.second.ptr);
+  }
+  FieldStorage(const FieldStorage&) = delete;
+  FieldStorage& operator=(const FieldStorage&) = delete;
+
+  size_t npts() const { return n_; }
+  short active() const { return active_; }
+  bool is_valid(short d) const
+  {
+    auto it = copies_.find(d);
+    return it != copies_.end() && it->second.valid;
+  }
+  bool is_available(short d) const { return copies_.count(d) != 0; }
+
+  void add_device(short d)
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    Copy& c = copies_[d];
+    if (!c.ptr) {
+      c.ptr = allocate(d);
+      c.owned = true;
+    }
+    if (!c.valid) {
+      copy(d, c.ptr, active_, copies_[active_].ptr);
+      c.valid = true;
+    }
+  }
+  void validate_device(short d) { add_device(d); }
+  void set_device_as_active(short d)
+  {
+    add_device(d);
+    active_ = d;
+  }
+  // write access: must be the active device; other copies become stale
+  T* values(short d)
+  {
+    if (d != active_)
+      throw std::runtime_error("Nebo error in FieldStorage:\nrequested write access to a device that is not active");
+    for (auto& kv : copies_)
+      if (kv.first != d) kv.second.valid = false;
+    return copies_[d].ptr;
+  }
+  const T* const_values(short d) const
+  {
+    auto it = copies_.find(d);
+    if (it == copies_.end() || !it->second.valid)
+      throw std::runtime_error("Nebo error in FieldStorage:\nrequested read access to a device without a valid copy");
+    return it->second.ptr;
+  }
+  T* raw(short d) { return copies_[d].ptr; }
+  void mark_written(short d)
+  {
+    for (auto& kv : copies_) kv.second.valid = (kv.first == d);
+  }
+
+ private:
+  T* allocate(short d)
+  {
+    if (IS_CPU_INDEX(d)) return new T[n_]();
+    void* p = nullptr;
+    nebo_check(nebo_alloc(d, n_ * sizeof(T), &p));
+    nebo_check(nebo_memset(p, 0, n_ * sizeof(T), nullptr));
+    return static_cast<T*>(p);
+  }
+  void release(short d, T* p)
+  {
+    if (IS_CPU_INDEX(d)) delete[] p;
+    else nebo_free(d, p);
+  }
+  void copy(short dd, T* dst, short sd, const T* src)
+  {
+    const size_t bytes = n_ * sizeof(T);
+    if (IS_CPU_INDEX(dd) && IS_CPU_INDEX(sd)) std::memcpy(dst, src, bytes);
+    else if (IS_CPU_INDEX(dd)) nebo_check(nebo_memcpy(dst, src, bytes, 1, nullptr));
+    else if (IS_CPU_INDEX(sd)) nebo_check(nebo_memcpy(dst, src, bytes, 0, nullptr));
+    else if (dd == sd) nebo_check(nebo_memcpy(dst, src, bytes, 2, nullptr));
+    else nebo_check(nebo_memcpy_peer(dst, dd, src, sd, bytes, nullptr));
+  }
+
+  size_t n_;
+  short active_;
+  std::map<short, Copy> copies_;
+  std::mutex mu_;
+};
+
+// ---------------------------------------------------------------------------
+// SpatialField (SpatialField.h:128-603): a shallow handle; copies share storage.
+// ---------------------------------------------------------------------------
+template <typename Loc, typename T = double>
+class SpatialField {
+ public:
+  typedef Loc Location;
+  typedef T value_type;
+  typedef SpatialField<Loc, T> field_type;
+
+  SpatialField(const MemoryWindow& window, const BoundaryCellInfo& bc, const GhostData& ghosts,
+               T* values = nullptr, StorageMode mode = InternalStorage, short devIdx = CPU_INDEX)
+      : window_(window),
+        bc_(bc.limit_by_extent(window.extent())),
+        ghosts_(ghosts.limit_by_extent(window.extent())),
+        valid_(ghosts_),
+        info_(std::make_shared<FieldStorage<T>>(window.glob_npts(), values, mode, devIdx))
+  {}
+
+  const MemoryWindow& window_with_ghost() const { return window_; }
+  MemoryWindow window_without_ghost() const
+  {
+    return MemoryWindow(window_.glob_dim(), window_.offset() + ghosts_.get_minus(),
+                        window_.extent() - ghosts_.get_minus() - ghosts_.get_plus() - bc_.has_extra());
+  }
+  const BoundaryCellInfo& boundary_info() const { return bc_; }
+  const GhostData& get_ghost_data() const { return ghosts_; }
+  const GhostData& get_valid_ghost_data() const { return valid_; }
+  void reset_valid_ghosts(const GhostData& g) { valid_ = g; }
+  size_t allocated_bytes() const { return window_.glob_npts() * sizeof(T); }
+
+  short active_device_index() const { return info_->active(); }
+  void add_device(short d) { info_->add_device(d); }
+  void add_device_async(short d) { info_->add_device(d); }
+  void validate_device(short d) { info_->validate_device(d); }
+  void validate_device_async(short d) { info_->validate_device(d); }
+  void set_device_as_active(short d) { info_->set_device_as_active(d); }
+  void set_device_as_active_async(short d) { info_->set_device_as_active(d); }
+  bool is_valid(short d) const { return info_->is_valid(d); }
+  bool is_available(short d) const { return info_->is_available(d); }
+  T* field_values(short d = CPU_INDEX) { return info_->values(d); }
+  const T* field_values(short d = CPU_INDEX) const { return info_->const_values(d); }
+
+  void set_stream(void* s) { stream_ = s; }
+  void* get_stream() const { return stream_; }
+  void wait_for_synchronization() { nebo_check(nebo_stream_sync(stream_)); }
+
+  // host element access (the active copy must be the CPU one)
+  T& operator()(int i, int j, int k) { return field_values(CPU_INDEX)[window_.flat_index(IntVec(i, j, k))]; }
+  const T& operator()(int i, int j, int k) const
+  {
+    return field_values(CPU_INDEX)[window_.flat_index(IntVec(i, j, k))];
+  }
+  T& operator[](size_t i) { return field_values(CPU_INDEX)[i]; }
+
+  // the engine-side description of this field on its active device
+  nebo_field_t descriptor() const
+  {
+    nebo_field_t f;
+    std::memset(&f, 0, sizeof f);
+    const short d = info_->active();
+    f.values = info_->raw(d);
+    for (int a = 0; a < 3; ++a) {
+      f.glob[a] = window_.glob_dim()[a];
+      f.offset[a] = window_.offset()[a];
+      f.extent[a] = window_.extent()[a];
+      f.ghost_minus[a] = ghosts_.get_minus(a);
+      f.ghost_plus[a] = ghosts_.get_plus(a);
+      f.valid_minus[a] = valid_.get_minus(a);
+      f.valid_plus[a] = valid_.get_plus(a);
+      f.has_bc[a] = bc_.has_bc(a);
+      f.bc_extra[a] = bc_.has_extra(a);
+    }
+    f.device = d;
+    return f;
+  }
+  const void* storage_id() const { return info_.get(); }
+  void mark_written() { info_->mark_written(info_->active()); }
+
+ private:
+  MemoryWindow window_;
+  BoundaryCellInfo bc_;
+  GhostData ghosts_;
+  GhostData valid_;
+  std::shared_ptr<FieldStorage<T>> info_;
+  void* stream_ = nullptr;
+};
+
+// field typedefs (FVStaggeredFieldTypes.h)
+typedef SpatialField<SVol> SVolField;
+typedef SpatialField<SSurfX> SSurfXField;
+typedef SpatialField<SSurfY> SSurfYField;
+typedef SpatialField<SSurfZ> SSurfZField;
+typedef SpatialField<XVol> XVolField;
+typedef SpatialField<XSurfX> XSurfXField;
+typedef SpatialField<XSurfY> XSurfYField;
+typedef SpatialField<XSurfZ> XSurfZField;
+typedef SpatialField<YVol> YVolField;
+typedef SpatialField<YSurfX> YSurfXField;
+typedef SpatialField<YSurfY> YSurfYField;
+typedef SpatialField<YSurfZ> YSurfZField;
+typedef SpatialField<ZVol> ZVolField;
+typedef SpatialField<ZSurfX> ZSurfXField;
+typedef SpatialField<ZSurfY> ZSurfYField;
+typedef SpatialField<ZSurfZ> ZSurfZField;
+
+template <typename VolT> struct FaceTypes;
+template <> struct FaceTypes<SVolField> { typedef SSurfXField XFace; typedef SSurfYField YFace; typedef SSurfZField ZFace; };
+template <> struct FaceTypes<XVolField> { typedef XSurfXField XFace; typedef XSurfYField YFace; typedef XSurfZField ZFace; };
+template <> struct FaceTypes<YVolField> { typedef YSurfXField XFace; typedef YSurfYField YFace; typedef YSurfZField ZFace; };
+template <> struct FaceTypes<ZVolField> { typedef ZSurfXField XFace; typedef ZSurfYField YFace; typedef ZSurfZField ZFace; };
+
+}  // namespace SpatialOps
