@@ -203,6 +203,13 @@ int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
 int nebo_assign_plan(const nebo_field_t* lhs, const nebo_expr_t* rhs,
                      int use_ghost, int* kernel_id);
 
+/* Host-only: size of the fused program the generic evaluator runs for rhs (its
+ * instruction count, evaluation-stack depth and distinct (field, offset) reads).
+ * Fails with NEBO_ERR_TOO_LARGE when the tree exceeds the evaluator's limits
+ * (200 instructions, depth 10, 40 reads, 48 constants, 6 mask offsets). */
+int nebo_lower_stats(const nebo_expr_t* rhs, int32_t* n_insns, int32_t* max_depth,
+                     int32_t* n_loads);
+
 /* masked_assign(mask, lhs, rhs): lhs(p) = rhs(p) for each listed point p
  * (interior-relative ijk triples, HOST array of 3*n_points int32; the library
  * checks every point against the LHS window and the RHS's valid ghost range,

@@ -599,3 +599,23 @@ extern "C" int nebo_assign_plan(const nebo_field_t* lhs, const nebo_expr_t* rhs,
   }
   return NEBO_OK;
 }
+
+extern "C" int nebo_lower_stats(const nebo_expr_t* rhs, int32_t* n_insns, int32_t* max_depth,
+                                int32_t* n_loads)
+{
+  nebo::clear_error();
+  int rc = nebo::validate_expr(rhs, "Nebo Assignment");
+  if (rc) return rc;
+  nebo::VmProgram prog;
+  int depth = 0;
+  if ((rc = nebo::lower_program(rhs, &prog, &depth))) return rc;
+  if (n_insns) *n_insns = prog.n_insns;
+  if (max_depth) *max_depth = depth;
+  if (n_loads) {
+    int n = 0;
+    for (int i = 0; i < nebo::kVmMaxLoads; ++i)
+      if (prog.loads[i].base) ++n;
+    *n_loads = n;
+  }
+  return NEBO_OK;
+}

@@ -133,6 +133,7 @@ _SIGS = {
     "nebo_assign": (C.c_int, [P(FieldT), P(ExprT), C.c_int, _vp]),
     "nebo_assign_generic": (C.c_int, [P(FieldT), P(ExprT), C.c_int, _vp]),
     "nebo_assign_plan": (C.c_int, [P(FieldT), P(ExprT), C.c_int, P(C.c_int)]),
+    "nebo_lower_stats": (C.c_int, [P(ExprT), P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
     "nebo_masked_assign": (C.c_int, [P(FieldT), _vp, C.c_int32, P(ExprT), _vp]),
     "nebo_reduce": (C.c_int, [P(ExprT), C.c_int, C.c_int, P(C.c_double), _vp]),
     "nebo_reduce_async": (C.c_int, [P(ExprT), C.c_int, C.c_int, _vp, _vp]),

@@ -137,3 +137,11 @@ def stencil_autotune(r: Field, k: Field, phi: Field, coefs, stream=None):
     _abi.call("nebo_stencil_autotune", C.byref(dr), C.byref(dk), C.byref(dp), C.byref(coefs),
               _stream(stream), C.byref(ty), C.byref(zc), times)
     return (ty.value, zc.value), dict(zip(tiles, list(times)))
+
+
+def lower_stats(rhs, where="host"):
+    """(instructions, stack depth, distinct reads) of the fused program for rhs."""
+    s = serialize(E(rhs), where)
+    n, d, l = C.c_int32(), C.c_int32(), C.c_int32()
+    _abi.call("nebo_lower_stats", s.ref, C.byref(n), C.byref(d), C.byref(l))
+    return n.value, d.value, l.value
