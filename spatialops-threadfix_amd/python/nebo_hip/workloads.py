@@ -150,3 +150,59 @@ def transport_problem(n, seed_k=2, seed_phi=3, seed_u=7):
 
 __all__ = ["heat_initial", "ghost_points", "heat_problem", "heat_step_exprs", "transport_velocity",
            "upwind_ops", "transport_rhs", "rk3_stage_exprs", "transport_problem", "Field"]
+
+
+def transport_problem_gpu(n, device=0, seed_k=2, seed_phi=3, seed_u=7, modes_phi=8, modes_u=4):
+    """transport_problem's fields generated on the GPU with torch (same formulas and
+    seeds; the device sin differs from numpy's in the last bits, so use this only
+    where no CPU result is compared, e.g. the 512^3 benchmark leg)."""
+    import torch
+    dev = torch.device(f"cuda:{device}")
+    L = Layout.build((n, n, n), 1)
+    Ls = (L.like("SSurfX"), L.like("SSurfY"), L.like("SSurfZ"))
+    db = build_stencils(n, n, n, 1.0, 1.0, 1.0)
+    ops = basic_op_types(db)
+    h = 1.0 / n
+    idx = torch.arange(-1, n + 1, dtype=torch.float64, device=dev)
+
+    def periodic(a):
+        a[0], a[-1] = a[-2].clone(), a[1].clone()
+        a[:, 0], a[:, -1] = a[:, -2].clone(), a[:, 1].clone()
+        a[:, :, 0], a[:, :, -1] = a[:, :, -2].clone(), a[:, :, 1].clone()
+        return a
+
+    # k ~ U[0.5, 2) * 1e-3 on the interior (seeded on the host, n^3 draws), periodic ghosts
+    k = torch.zeros((n + 2,) * 3, dtype=torch.float64, device=dev)
+    k[1:-1, 1:-1, 1:-1] = torch.from_numpy(inputs.uniform(seed_k, n ** 3, 0.5, 2.0).reshape(n, n, n)).to(dev)
+    k = periodic(k * 1e-3)
+    # phi: 8 Fourier modes at cell centres
+    kk = (inputs.splitmix64(seed_phi, 3 * modes_phi) % np.uint64(8)).astype(np.int64).reshape(modes_phi, 3) + 1
+    amp = inputs.normal(seed_phi + 1, modes_phi)
+    th = inputs.uniform(seed_phi + 2, modes_phi, 0.0, 2 * np.pi)
+    xc = (idx + 0.5) * h
+    phi = torch.zeros((n + 2,) * 3, dtype=torch.float64, device=dev)
+    for m in range(modes_phi):
+        phi += float(amp[m]) * torch.sin(2 * np.pi * (int(kk[m, 2]) * xc[:, None, None] + int(kk[m, 1]) * xc[None, :, None]
+                                                     + int(kk[m, 0]) * xc[None, None, :]) + float(th[m]))
+    # velocity: the discrete curl of edge-centred potentials (as transport_velocity)
+    rk = (inputs.splitmix64(seed_u, 3 * modes_u) % np.uint64(4)).astype(np.int64).reshape(modes_u, 3) + 1
+    ua = inputs.normal(seed_u + 1, 3 * modes_u).reshape(modes_u, 3)
+    up = inputs.uniform(seed_u + 2, 3 * modes_u, 0.0, 2 * np.pi).reshape(modes_u, 3)
+
+    def A(comp, sx, sy, sz):
+        z = (idx + sz)[:, None, None] * h
+        y = (idx + sy)[None, :, None] * h
+        x = (idx + sx)[None, None, :] * h
+        out = torch.zeros((n + 2,) * 3, dtype=torch.float64, device=dev)
+        for m in range(modes_u):
+            out += float(ua[m, comp]) * torch.sin(2 * np.pi * (int(rk[m, 0]) * x + int(rk[m, 1]) * y
+                                                              + int(rk[m, 2]) * z) + float(up[m, comp]))
+        return out
+
+    ux = ((A(2, 0, 1, .5) - A(2, 0, 0, .5)) - (A(1, 0, .5, 1) - A(1, 0, .5, 0))) / h
+    uy = ((A(0, .5, 0, 1) - A(0, .5, 0, 0)) - (A(2, 1, 0, .5) - A(2, 0, 0, .5))) / h
+    uz = ((A(1, 1, .5, 0) - A(1, 0, .5, 0)) - (A(0, .5, 1, 0) - A(0, .5, 0, 0))) / h
+    umax = max(float(ux.abs().max()), float(uy.abs().max()), float(uz.abs().max()))
+    u = [periodic(v / umax).reshape(-1) for v in (ux, uy, uz)]
+    dt = 0.2 * min(h, h * h / (6 * 2e-3))
+    return L, Ls, ops, k.reshape(-1), periodic(phi).reshape(-1), u, dt
