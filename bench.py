@@ -34,10 +34,17 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-tune", action="store_true", help="keep the default tile (no autotune)")
+    p.add_argument("--tile", default=None,
+                   help="ty,zchunk,variant of the diffusion kernel (implies --no-tune)")
     p.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
     p.add_argument("--pmc-probe", action="store_true",
                    help="only launch 3 known-size copies and 3 kernels (for rocprofv3 --pmc runs)")
     return p.parse_args()
+
+
+def _tile_name(t):
+    """64 x ty x zchunk, with the schedule: direct loads or LDS-staged + prefetch depth."""
+    return f"64x{t[0]}x{t[1]}" + ("" if t[2] == 0 else f"/staged-pf{t[2]}")
 
 
 def cpu_baseline(L, kh, ph, coefs, budget_s):
@@ -209,7 +216,7 @@ def transport_leg(n=512, steps=20):
             "program_insns": nh.lower_stats(e3, where="gpu")[0]}
 
 
-def pmc_probe(n):
+def pmc_probe(n, tile=None):
     """Three copies of one field (known bytes) and three fused kernel launches,
     nothing else: the counter runs divide the kernel's FETCH/WRITE by the copy's."""
     import torch
@@ -222,6 +229,8 @@ def pmc_probe(n):
     dst = torch.empty_like(phi.gpu)
     db = nh.build_stencils(n, n, n, 1.0, 1.0, 1.0)
     c = nh.axis_coefs(db)
+    if tile:
+        nh.stencil_set_tile(0, *tile)
     torch.cuda.synchronize()
     for _ in range(3):
         dst.copy_(phi.gpu)
@@ -229,7 +238,8 @@ def pmc_probe(n):
         nh.divkgrad(r, k, phi, c, 0)
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": True, "copy_bytes_read": L.npts * 8, "copy_bytes_written": L.npts * 8,
-                      "kernel_bytes_algorithmic": BYTES_PER_CELL * n ** 3}))
+                      "kernel_bytes_algorithmic": BYTES_PER_CELL * n ** 3,
+                      "tile": _tile_name(nh.stencil_get_tile(0))}))
 
 
 def main():
@@ -238,8 +248,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    tile_arg = tuple(int(v) for v in a.tile.split(",")) if a.tile else None
     if a.pmc_probe:
-        pmc_probe(a.n)
+        pmc_probe(a.n, tile_arg)
         return
     import numpy as np
     import torch
@@ -275,10 +286,12 @@ def main():
             nh._abi.check(rc)
 
     tune = None
-    if not a.no_tune:
+    if tile_arg:
+        nh.stencil_set_tile(0, *tile_arg)
+    elif not a.no_tune:
         best, times = nh.stencil_autotune(r, k, phi, coefs)
-        tune = {"tile": f"64x{best[0]}x{best[1]}",
-                "ms_per_tile": {f"64x{t[0]}x{t[1]}": round(v, 5) for t, v in times.items()}}
+        tune = {"tile": _tile_name(best),
+                "ms_per_tile": {_tile_name(t): round(v, 5) for t, v in times.items()}}
     tile = nh.stencil_get_tile(0)
     for _ in range(a.warmup):
         step()
@@ -308,12 +321,14 @@ def main():
     gcells = cells / (ms_per_step * 1e-3) / 1e9
     achieved = BYTES_PER_CELL * cells / (avg_kernel_ms * 1e-3)
 
+    # HBM bytes per launch measured with rocprofv3 counters for this tile (null if
+    # this tile has not been measured)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"divkgrad_{n}_traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch", {}).get(_tile_name(tile))
+        except (OSError, ValueError, AttributeError):
             traffic = None
 
     extras = {}
@@ -333,7 +348,7 @@ def main():
         "data": "synthetic (seeded k ~ U[0.5,2), 8-mode Fourier phi, periodic ghosts)",
         "config": {"workload": f"div(k grad phi) {n}^3 fp64, 1 ghost layer, fused <<=",
                    "n": n, "cells": cells, "kernel": "divkgrad (hand-written HIP, gfx950)",
-                   "tile": f"64x{tile[0]}x{tile[1]}", "autotune": tune},
+                   "tile": _tile_name(tile), "tile_args": ",".join(map(str, tile)), "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "bytes_per_launch_algorithmic": BYTES_PER_CELL * cells,

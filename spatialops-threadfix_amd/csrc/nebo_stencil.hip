@@ -9,14 +9,27 @@
 // (window offset + valid ghost minus) and the shared strides X = glob[0],
 // XY = glob[0]*glob[1].
 //
-// Work split: a workgroup of 64*TY threads covers a 64 (x) by TY (y) column tile
-// and marches through a chunk of ZC z planes, keeping the column's z neighbours
-// and the shared z face in registers; x and y neighbours come from the L1/L2
-// caches.  Halo rows and planes are re-read by the neighbouring tile, so larger
-// TY and ZC cut the traffic beyond L2 ((TY+2)/TY * (ZC+2)/ZC for the operands)
-// at the cost of fewer workgroups; the best shape is measured on the device
-// (nebo_stencil_autotune).  Workgroups are numbered so that consecutive tiles,
-// which share halo rows, land on the same XCD and therefore the same L2.
+// Work split: a workgroup of 64*TY threads (one wave per y row) covers a 64 (x)
+// by TY (y) column tile and marches through a chunk of ZC z planes, keeping the
+// column's z neighbours and the shared z face in registers.  Two schedules:
+//
+//  * direct (variant 0): x and y neighbours are plain loads served by L1/L2;
+//    every plane costs 10 loads per cell and the wave waits for them before it
+//    computes the plane.
+//  * staged (variants 1, 2): each thread loads its own cell of the next planes
+//    ahead of use (1 or 2 planes of prefetch held in registers), x neighbours
+//    come from the adjacent lanes through DPP wave shifts, y neighbours from the
+//    plane's rows staged in LDS (double-buffered, one barrier per plane).  Three
+//    loads per field and plane: the cell, an x-edge load that the two end lanes
+//    use for the tile's x halo, and a row load that waves 0 and TY-1 use for the
+//    y halo rows (for the other lanes and waves these two re-read their own cell,
+//    so every load is unconditional and the prefetch stays in flight).
+//
+// Halo rows and planes are re-read by the neighbouring tile, so larger TY and
+// ZC cut the traffic beyond L2 ((TY+2)/TY * (ZC+2)/ZC for the operands) at the
+// cost of fewer workgroups; the best shape and schedule are measured on the
+// device (nebo_stencil_autotune).  Workgroups are numbered so that consecutive
+// tiles, which share halo rows, land on the same XCD and therefore the same L2.
 
 #include "nebo_internal.h"
 
@@ -24,6 +37,7 @@
 
 #include <atomic>
 #include <sstream>
+#include <utility>
 
 namespace nebo {
 namespace {
@@ -63,11 +77,77 @@ __device__ __forceinline__ double face(double klo, double khi, double plo, doubl
   return ki * gr;
 }
 
+// Lane l receives lane l-1's value (DPP wave_shr:1); lane 0 keeps its own.
+__device__ __forceinline__ double from_lower_lane(double v)
+{
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Lane l receives lane l+1's value (DPP wave_shl:1); lane 63 keeps its own.
+__device__ __forceinline__ double from_upper_lane(double v)
+{
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Where one thread of a staged tile loads within a plane.  Every coordinate is
+// clamped into [lo-1, hi] on its axis, the box the one-cell stencil reads, so a
+// thread outside the limits still loads in bounds (its values are never used).
+struct StageGeom {
+  int oc;        // own cell (x, y), clamped
+  int oe;        // lane 0: (x0-1, y); lane 63: (x0+64, y); other lanes: own cell
+  int oh;        // wave 0: (x, y0-1); wave TY-1: (x, y0+TY); other waves: own cell
+  int halo_row;  // LDS row the halo load goes to (0 or TY+1), -1 for other waves
+};
+
+template <int TY>
+__device__ __forceinline__ StageGeom stage_geom(const Tiles& T, int X, int x0, int y0, int lane, int wy)
+{
+  const int xc = min(x0 + lane, T.hi[0]), yc = min(y0 + wy, T.hi[1]);
+  const int xe = lane == 0 ? x0 - 1 : lane == kTx - 1 ? min(x0 + kTx, T.hi[0]) : xc;
+  const int yh = wy == 0 ? y0 - 1 : wy == TY - 1 ? min(y0 + TY, T.hi[1]) : yc;
+  StageGeom g;
+  g.oc = xc + X * yc;
+  g.oe = xe + X * yc;
+  g.oh = xc + X * yh;
+  g.halo_row = wy == 0 ? 0 : wy == TY - 1 ? TY + 1 : -1;
+  return g;
+}
+
+// ---- div(interp(k) * grad(phi)) ------------------------------------------------
+
 // FORM 0: (Dx + Dy) + Dz;  FORM 1: ((-Dx) - Dy) - Dz;  FORM 2: phi + dt*((Dx + Dy) + Dz)
+template <int FORM>
+__device__ __forceinline__ double divk_combine(double Dx, double Dy, double Dz, double pc, double dt)
+{
+  if constexpr (FORM == 0) return (Dx + Dy) + Dz;
+  else if constexpr (FORM == 1) return ((-Dx) - Dy) - Dz;
+  else return pc + dt * ((Dx + Dy) + Dz);   // explicit Euler step of the diffusion equation
+}
+
+struct DivkCoefs {
+  double ix0, ix1, gx0, gx1, iy0, iy1, gy0, gy1, iz0, iz1, gz0, gz1;
+  double dx0, dx1, dy0, dy1, dz0, dz1;
+};
+
+__device__ __forceinline__ DivkCoefs divk_coefs(const nebo_axis_coefs_t& c)
+{
+  return DivkCoefs{c.interp[0][0], c.interp[0][1], c.grad[0][0], c.grad[0][1],
+                   c.interp[1][0], c.interp[1][1], c.grad[1][0], c.grad[1][1],
+                   c.interp[2][0], c.interp[2][1], c.grad[2][0], c.grad[2][1],
+                   c.div[0][0],    c.div[0][1],    c.div[1][0],  c.div[1][1],
+                   c.div[2][0],    c.div[2][1]};
+}
+
 template <int FORM, int TY, int ZC>
 __global__ __launch_bounds__(kTx * TY) void divkgrad_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
-    Tiles T, int X, int XY, nebo_axis_coefs_t c, double dt)
+    Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
 {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % T.nx;
@@ -79,45 +159,145 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_kernel(
   if (x >= T.hi[0] || y >= T.hi[1]) return;
   const int z0 = T.lo[2] + tz * ZC;
   const int z1 = min(z0 + ZC, T.hi[2]);
-
-  const double ix0 = c.interp[0][0], ix1 = c.interp[0][1], gx0 = c.grad[0][0], gx1 = c.grad[0][1];
-  const double iy0 = c.interp[1][0], iy1 = c.interp[1][1], gy0 = c.grad[1][0], gy1 = c.grad[1][1];
-  const double iz0 = c.interp[2][0], iz1 = c.interp[2][1], gz0 = c.grad[2][0], gz1 = c.grad[2][1];
-  const double dx0 = c.div[0][0], dx1 = c.div[0][1];
-  const double dy0 = c.div[1][0], dy1 = c.div[1][1];
-  const double dz0 = c.div[2][0], dz1 = c.div[2][1];
+  const DivkCoefs c = divk_coefs(cc);
 
   int i = x + X * y + XY * z0;
   double pc = p[i], kc = k[i];
-  double fz_lo = face(k[i - XY], kc, p[i - XY], pc, iz0, iz1, gz0, gz1);
+  double fz_lo = face(k[i - XY], kc, p[i - XY], pc, c.iz0, c.iz1, c.gz0, c.gz1);
   for (int z = z0; z < z1; ++z, i += XY) {
     const double pz = p[i + XY], kz = k[i + XY];
     const double pxm = p[i - 1], pxp = p[i + 1], kxm = k[i - 1], kxp = k[i + 1];
     const double pym = p[i - X], pyp = p[i + X], kym = k[i - X], kyp = k[i + X];
-    const double fx_lo = face(kxm, kc, pxm, pc, ix0, ix1, gx0, gx1);
-    const double fx_hi = face(kc, kxp, pc, pxp, ix0, ix1, gx0, gx1);
-    const double fy_lo = face(kym, kc, pym, pc, iy0, iy1, gy0, gy1);
-    const double fy_hi = face(kc, kyp, pc, pyp, iy0, iy1, gy0, gy1);
-    const double fz_hi = face(kc, kz, pc, pz, iz0, iz1, gz0, gz1);
-    const double Dx = fx_lo * dx0 + fx_hi * dx1;
-    const double Dy = fy_lo * dy0 + fy_hi * dy1;
-    const double Dz = fz_lo * dz0 + fz_hi * dz1;
-    double v;
-    if constexpr (FORM == 0) v = (Dx + Dy) + Dz;
-    else if constexpr (FORM == 1) v = ((-Dx) - Dy) - Dz;
-    else v = pc + dt * ((Dx + Dy) + Dz);   // explicit Euler step of the diffusion equation
-    __builtin_nontemporal_store(v, &r[i]);
+    const double fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+    const double fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+    const double fy_lo = face(kym, kc, pym, pc, c.iy0, c.iy1, c.gy0, c.gy1);
+    const double fy_hi = face(kc, kyp, pc, pyp, c.iy0, c.iy1, c.gy0, c.gy1);
+    const double fz_hi = face(kc, kz, pc, pz, c.iz0, c.iz1, c.gz0, c.gz1);
+    const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
+    const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
+    const double Dz = fz_lo * c.dz0 + fz_hi * c.dz1;
+    __builtin_nontemporal_store(divk_combine<FORM>(Dx, Dy, Dz, pc, dt), &r[i]);
     pc = pz;
     kc = kz;
     fz_lo = fz_hi;
   }
 }
 
+// Staged schedule, PD = planes of prefetch (1 or 2).
+template <int FORM, int TY, int ZC, int PD>
+__global__ __launch_bounds__(kTx * TY) void divkgrad_staged_kernel(
+    double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
+    Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
+{
+  static_assert(TY >= 2 && (PD == 1 || PD == 2), "staged tiles need two rows and a prefetch of 1 or 2");
+  __shared__ double sp[2][TY + 2][kTx];
+  __shared__ double sk[2][TY + 2][kTx];
+
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kTx - 1);
+  const int wy = threadIdx.x / kTx;
+  const int x0 = T.lo[0] + tx * kTx, y0 = T.lo[1] + ty * TY;
+  const int x = x0 + lane, y = y0 + wy;
+  const bool active = x < T.hi[0] && y < T.hi[1];
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
+  const StageGeom g = stage_geom<TY>(T, X, x0, y0, lane, wy);
+  const DivkCoefs c = divk_coefs(cc);
+
+  struct Plane { double p, k, pe, ke, ph, kh; };
+  // plane z, any z in [z0-1, z1] (inside the readable box)
+  auto load = [&](int z) {
+    const int b = XY * z;
+    return Plane{p[b + g.oc], k[b + g.oc], p[b + g.oe], k[b + g.oe], p[b + g.oh], k[b + g.oh]};
+  };
+  auto stage = [&](int buf, const Plane& v) {
+    sp[buf][wy + 1][lane] = v.p;
+    sk[buf][wy + 1][lane] = v.k;
+    if (g.halo_row >= 0) {
+      sp[buf][g.halo_row][lane] = v.ph;
+      sk[buf][g.halo_row][lane] = v.kh;
+    }
+  };
+
+  // Register slots: at step z (r = z - z0) slot r % P holds plane z and slot
+  // (r + j) % P plane z + j.  One period of P steps is unrolled so the slots are
+  // renamed rather than copied: a copy would wait for its load and undo the
+  // prefetch.  The chunk is padded to whole periods; padded steps load clamped
+  // planes and store nothing.
+  constexpr int P = PD + 1;
+  Plane S[P];
+  S[0] = load(z0);
+  const double pzm = p[XY * (z0 - 1) + g.oc], kzm = k[XY * (z0 - 1) + g.oc];
+#pragma unroll
+  for (int j = 1; j < P; ++j) S[j] = load(min(z0 + j, z1));
+  stage(0, S[0]);
+  __syncthreads();
+
+  double fz_lo = face(kzm, S[0].k, pzm, S[0].p, c.iz0, c.iz1, c.gz0, c.gz1);
+  int buf = 0;
+  const int periods = (z1 - z0 + P - 1) / P;
+  for (int q = 0; q < periods; ++q) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int z = z0 + q * P + j;
+      const double pc = S[j].p, kc = S[j].k, pe = S[j].pe, ke = S[j].ke;
+      const Plane& B1 = S[(j + 1) % P];   // plane z+1
+      const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
+      const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
+      const double pxm = lane == 0 ? pe : pl, pxp = lane == kTx - 1 ? pe : pu;
+      const double kxm = lane == 0 ? ke : kl, kxp = lane == kTx - 1 ? ke : ku;
+      const double pym = sp[buf][wy][lane], pyp = sp[buf][wy + 2][lane];
+      const double kym = sk[buf][wy][lane], kyp = sk[buf][wy + 2][lane];
+      const double fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+      const double fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+      const double fy_lo = face(kym, kc, pym, pc, c.iy0, c.iy1, c.gy0, c.gy1);
+      const double fy_hi = face(kc, kyp, pc, pyp, c.iy0, c.iy1, c.gy0, c.gy1);
+      const double fz_hi = face(kc, B1.k, pc, B1.p, c.iz0, c.iz1, c.gz0, c.gz1);
+      const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
+      const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
+      const double Dz = fz_lo * c.dz0 + fz_hi * c.dz1;
+      const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
+      if (active && z < z1) __builtin_nontemporal_store(v, &r[x + X * y + XY * z]);
+      // plane z is consumed: its slot receives plane z+P (same registers, no copy)
+      S[j] = load(min(z + P, z1));
+      stage(buf ^ 1, B1);   // plane z+1 for the next step
+      __syncthreads();
+      buf ^= 1;
+      fz_lo = fz_hi;
+    }
+  }
+}
+
+// ---- phi + s*lapl(phi) -----------------------------------------------------------
+
 // out = phi + s*((Lx + Ly) + Lz),  L_d = (phi[-1]*g0 + phi[0]*g1)*d0 + (phi[0]*g0 + phi[+1]*g1)*d1
+struct LaplCoefs {
+  double gx0, gx1, gy0, gy1, gz0, gz1, dx0, dx1, dy0, dy1, dz0, dz1;
+};
+
+__device__ __forceinline__ LaplCoefs lapl_coefs(const nebo_axis_coefs_t& c)
+{
+  return LaplCoefs{c.grad[0][0], c.grad[0][1], c.grad[1][0], c.grad[1][1], c.grad[2][0], c.grad[2][1],
+                   c.div[0][0],  c.div[0][1],  c.div[1][0],  c.div[1][1],  c.div[2][0],  c.div[2][1]};
+}
+
+__device__ __forceinline__ double lapl_cell(double pc, double pxm, double pxp, double pym, double pyp,
+                                            double gz_lo, double gz_hi, const LaplCoefs& c, double s)
+{
+  const double Lx = (pxm * c.gx0 + pc * c.gx1) * c.dx0 + (pc * c.gx0 + pxp * c.gx1) * c.dx1;
+  const double Ly = (pym * c.gy0 + pc * c.gy1) * c.dy0 + (pc * c.gy0 + pyp * c.gy1) * c.dy1;
+  const double Lz = gz_lo * c.dz0 + gz_hi * c.dz1;
+  return pc + s * ((Lx + Ly) + Lz);
+}
+
 template <int TY, int ZC>
 __global__ __launch_bounds__(kTx * TY) void lapl_euler_kernel(
     double* __restrict__ out, const double* __restrict__ p, Tiles T, int X, int XY,
-    nebo_axis_coefs_t c, double s)
+    nebo_axis_coefs_t cc, double s)
 {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % T.nx;
@@ -129,33 +309,96 @@ __global__ __launch_bounds__(kTx * TY) void lapl_euler_kernel(
   if (x >= T.hi[0] || y >= T.hi[1]) return;
   const int z0 = T.lo[2] + tz * ZC;
   const int z1 = min(z0 + ZC, T.hi[2]);
-  const double gx0 = c.grad[0][0], gx1 = c.grad[0][1], gy0 = c.grad[1][0], gy1 = c.grad[1][1];
-  const double gz0 = c.grad[2][0], gz1 = c.grad[2][1];
-  const double dx0 = c.div[0][0], dx1 = c.div[0][1], dy0 = c.div[1][0], dy1 = c.div[1][1];
-  const double dz0 = c.div[2][0], dz1 = c.div[2][1];
+  const LaplCoefs c = lapl_coefs(cc);
 
   int i = x + X * y + XY * z0;
   double pc = p[i];
-  double gz_lo = p[i - XY] * gz0 + pc * gz1;
+  double gz_lo = p[i - XY] * c.gz0 + pc * c.gz1;
   for (int z = z0; z < z1; ++z, i += XY) {
     const double pz = p[i + XY];
     const double pxm = p[i - 1], pxp = p[i + 1], pym = p[i - X], pyp = p[i + X];
-    const double gz_hi = pc * gz0 + pz * gz1;
-    const double Lx = (pxm * gx0 + pc * gx1) * dx0 + (pc * gx0 + pxp * gx1) * dx1;
-    const double Ly = (pym * gy0 + pc * gy1) * dy0 + (pc * gy0 + pyp * gy1) * dy1;
-    const double Lz = gz_lo * dz0 + gz_hi * dz1;
-    const double v = pc + s * ((Lx + Ly) + Lz);
-    __builtin_nontemporal_store(v, &out[i]);
+    const double gz_hi = pc * c.gz0 + pz * c.gz1;
+    __builtin_nontemporal_store(lapl_cell(pc, pxm, pxp, pym, pyp, gz_lo, gz_hi, c, s), &out[i]);
     pc = pz;
     gz_lo = gz_hi;
   }
 }
 
+template <int TY, int ZC, int PD>
+__global__ __launch_bounds__(kTx * TY) void lapl_euler_staged_kernel(
+    double* __restrict__ out, const double* __restrict__ p, Tiles T, int X, int XY,
+    nebo_axis_coefs_t cc, double s)
+{
+  static_assert(TY >= 2 && (PD == 1 || PD == 2), "staged tiles need two rows and a prefetch of 1 or 2");
+  __shared__ double sp[2][TY + 2][kTx];
+
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kTx - 1);
+  const int wy = threadIdx.x / kTx;
+  const int x0 = T.lo[0] + tx * kTx, y0 = T.lo[1] + ty * TY;
+  const int x = x0 + lane, y = y0 + wy;
+  const bool active = x < T.hi[0] && y < T.hi[1];
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
+  const StageGeom g = stage_geom<TY>(T, X, x0, y0, lane, wy);
+  const LaplCoefs c = lapl_coefs(cc);
+
+  struct Plane { double p, pe, ph; };
+  auto load = [&](int z) {
+    const int b = XY * z;
+    return Plane{p[b + g.oc], p[b + g.oe], p[b + g.oh]};
+  };
+  auto stage = [&](int buf, const Plane& v) {
+    sp[buf][wy + 1][lane] = v.p;
+    if (g.halo_row >= 0) sp[buf][g.halo_row][lane] = v.ph;
+  };
+
+  // register slots renamed by an unrolled period, as in divkgrad_staged_kernel
+  constexpr int P = PD + 1;
+  Plane S[P];
+  S[0] = load(z0);
+  const double pzm = p[XY * (z0 - 1) + g.oc];
+#pragma unroll
+  for (int j = 1; j < P; ++j) S[j] = load(min(z0 + j, z1));
+  stage(0, S[0]);
+  __syncthreads();
+
+  double gz_lo = pzm * c.gz0 + S[0].p * c.gz1;
+  int buf = 0;
+  const int periods = (z1 - z0 + P - 1) / P;
+  for (int q = 0; q < periods; ++q) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int z = z0 + q * P + j;
+      const double pc = S[j].p, pe = S[j].pe;
+      const Plane& B1 = S[(j + 1) % P];
+      const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
+      const double pxm = lane == 0 ? pe : pl, pxp = lane == kTx - 1 ? pe : pu;
+      const double pym = sp[buf][wy][lane], pyp = sp[buf][wy + 2][lane];
+      const double gz_hi = pc * c.gz0 + B1.p * c.gz1;
+      const double v = lapl_cell(pc, pxm, pxp, pym, pyp, gz_lo, gz_hi, c, s);
+      if (active && z < z1) __builtin_nontemporal_store(v, &out[x + X * y + XY * z]);
+      S[j] = load(min(z + P, z1));
+      stage(buf ^ 1, B1);
+      __syncthreads();
+      buf ^= 1;
+      gz_lo = gz_hi;
+    }
+  }
+}
+
 // ---- tile table -------------------------------------------------------------
 
-struct TileShape { int ty, zc; };
-constexpr TileShape kShapes[] = {{4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {8, 64},
-                                 {16, 16}, {16, 32}, {16, 64}, {2, 64}};
+// variant 0: direct loads; 1, 2: staged with 1 or 2 planes of prefetch
+struct TileShape { int ty, zc, variant; };
+constexpr TileShape kShapes[] = {
+    {4, 16, 0},  {4, 32, 0},  {8, 16, 0},  {8, 32, 0},  {16, 16, 0}, {2, 64, 0},
+    {4, 32, 1},  {8, 16, 1},  {8, 32, 1},  {8, 64, 1},  {16, 32, 1},
+    {4, 48, 2},  {8, 24, 2},  {8, 48, 2},  {16, 24, 2}, {16, 48, 2}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kDefaultShape = 0;
 
@@ -163,37 +406,50 @@ using DivkFn = void (*)(double*, const double*, const double*, Tiles, int, int, 
 using LaplFn = void (*)(double*, const double*, Tiles, int, int, nebo_axis_coefs_t, double);
 
 template <int FORM, int S>
-constexpr DivkFn divk_fn() { return divkgrad_kernel<FORM, kShapes[S].ty, kShapes[S].zc>; }
+constexpr DivkFn divk_fn()
+{
+  constexpr TileShape sh = kShapes[S];
+  if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
+  else return divkgrad_staged_kernel<FORM, sh.ty, sh.zc, sh.variant>;
+}
+
 template <int S>
-constexpr LaplFn lapl_fn() { return lapl_euler_kernel<kShapes[S].ty, kShapes[S].zc>; }
-
-template <int FORM>
-DivkFn divk_table(int s)
+constexpr LaplFn lapl_fn()
 {
-  static const DivkFn t[kNumShapes] = {divk_fn<FORM, 0>(), divk_fn<FORM, 1>(), divk_fn<FORM, 2>(),
-                                       divk_fn<FORM, 3>(), divk_fn<FORM, 4>(), divk_fn<FORM, 5>(),
-                                       divk_fn<FORM, 6>(), divk_fn<FORM, 7>(), divk_fn<FORM, 8>(),
-                                       divk_fn<FORM, 9>()};
+  constexpr TileShape sh = kShapes[S];
+  if constexpr (sh.variant == 0) return lapl_euler_kernel<sh.ty, sh.zc>;
+  else return lapl_euler_staged_kernel<sh.ty, sh.zc, sh.variant>;
+}
+
+template <int FORM, size_t... S>
+DivkFn divk_pick(int s, std::index_sequence<S...>)
+{
+  static const DivkFn t[] = {divk_fn<FORM, (int)S>()...};
   return t[s];
 }
 
-LaplFn lapl_table(int s)
+template <size_t... S>
+LaplFn lapl_pick(int s, std::index_sequence<S...>)
 {
-  static const LaplFn t[kNumShapes] = {lapl_fn<0>(), lapl_fn<1>(), lapl_fn<2>(), lapl_fn<3>(),
-                                       lapl_fn<4>(), lapl_fn<5>(), lapl_fn<6>(), lapl_fn<7>(),
-                                       lapl_fn<8>(), lapl_fn<9>()};
+  static const LaplFn t[] = {lapl_fn<(int)S>()...};
   return t[s];
 }
 
-static_assert(kNumShapes == 10, "tables list every shape");
+DivkFn divk_table(int form, int s)
+{
+  using Seq = std::make_index_sequence<kNumShapes>;
+  return form == 0 ? divk_pick<0>(s, Seq{}) : form == 1 ? divk_pick<1>(s, Seq{}) : divk_pick<2>(s, Seq{});
+}
+
+LaplFn lapl_table(int s) { return lapl_pick(s, std::make_index_sequence<kNumShapes>{}); }
 
 std::atomic<int> g_divk_shape{kDefaultShape};
 std::atomic<int> g_lapl_shape{kDefaultShape};
 
-int find_shape(int ty, int zc)
+int find_shape(int ty, int zc, int variant)
 {
   for (int s = 0; s < kNumShapes; ++s)
-    if (kShapes[s].ty == ty && kShapes[s].zc == zc) return s;
+    if (kShapes[s].ty == ty && kShapes[s].zc == zc && kShapes[s].variant == variant) return s;
   return -1;
 }
 
@@ -205,9 +461,20 @@ int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_fie
   Tiles T = make_tiles(lim, sh.ty, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
-  DivkFn fn = form == 0 ? divk_table<0>(shape) : form == 1 ? divk_table<1>(shape) : divk_table<2>(shape);
-  hipLaunchKernelGGL(fn, dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream, frame_origin(r),
-                     frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
+  hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream,
+                     frame_origin(r), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
+  return check_hip(hipGetLastError(), "Nebo Assignment");
+}
+
+int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* phi,
+                      const nebo_axis_coefs_t& c, double sc, const nebo_limits_t& lim, void* stream)
+{
+  const TileShape sh = kShapes[shape];
+  Tiles T = make_tiles(lim, sh.ty, sh.zc);
+  const int nb = T.nx * T.ny * T.nz;
+  const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
+  hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream,
+                     frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
   return check_hip(hipGetLastError(), "Nebo Assignment");
 }
 
@@ -269,13 +536,7 @@ int launch_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
                       const nebo_axis_coefs_t& c, double sc, const nebo_limits_t& lim,
                       void* stream)
 {
-  const TileShape sh = kShapes[g_lapl_shape.load()];
-  Tiles T = make_tiles(lim, sh.ty, sh.zc);
-  const int nb = T.nx * T.ny * T.nz;
-  const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
-  hipLaunchKernelGGL(lapl_table(g_lapl_shape.load()), dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream,
-                     frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
-  return check_hip(hipGetLastError(), "Nebo Assignment");
+  return launch_lapl_shape(g_lapl_shape.load(), out, phi, c, sc, lim, stream);
 }
 
 int fill_ghosts_periodic(const nebo_field_t* f, void* stream)
@@ -381,54 +642,67 @@ extern "C" int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
   return launch_lapl_euler(out, phi, *c, s, L, stream);
 }
 
-extern "C" int nebo_stencil_set_tile(int kernel, int32_t ty, int32_t zchunk)
+extern "C" int nebo_stencil_set_tile(int kernel, int32_t ty, int32_t zchunk, int32_t variant)
 {
   clear_error();
-  const int s = find_shape(ty, zchunk);
+  const int s = find_shape(ty, zchunk, variant);
   if (s < 0 || (kernel != 0 && kernel != 1)) {
     std::ostringstream m;
-    m << "no stencil kernel instantiated for kernel " << kernel << ", tile 64x" << ty << "x" << zchunk;
+    m << "no stencil kernel instantiated for kernel " << kernel << ", tile 64x" << ty << "x" << zchunk
+      << ", variant " << variant;
     return set_error(NEBO_ERR_INVALID, "Nebo Stencil Tile", m.str());
   }
   (kernel == 0 ? g_divk_shape : g_lapl_shape).store(s);
   return NEBO_OK;
 }
 
-extern "C" int nebo_stencil_get_tile(int kernel, int32_t* ty, int32_t* zchunk)
+extern "C" int nebo_stencil_get_tile(int kernel, int32_t* ty, int32_t* zchunk, int32_t* variant)
 {
   clear_error();
   if (kernel != 0 && kernel != 1) return set_error(NEBO_ERR_INVALID, "Nebo Stencil Tile", "unknown kernel");
   const TileShape sh = kShapes[(kernel == 0 ? g_divk_shape : g_lapl_shape).load()];
   if (ty) *ty = sh.ty;
   if (zchunk) *zchunk = sh.zc;
+  if (variant) *variant = sh.variant;
   return NEBO_OK;
 }
 
-extern "C" int nebo_stencil_autotune(const nebo_field_t* r, const nebo_field_t* k,
+extern "C" int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const nebo_field_t* k,
                                      const nebo_field_t* phi, const nebo_axis_coefs_t* c,
-                                     void* stream, int32_t* ty, int32_t* zchunk, float* times_ms)
+                                     void* stream, int32_t* ty, int32_t* zchunk, int32_t* variant,
+                                     float* times_ms)
 {
   static const char* ctx = "Nebo Stencil Tile";
   clear_error();
-  if (!r || !k || !phi || !c) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  if (kernel != 0 && kernel != 1) return set_error(NEBO_ERR_INVALID, ctx, "unknown kernel");
+  if (!r || !phi || !c || (kernel == 0 && !k)) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
   int rc;
-  if ((rc = check_same_shape(r, k, ctx)) || (rc = check_same_shape(r, phi, ctx))) return rc;
-  if (r->values == k->values || r->values == phi->values)
+  if ((rc = check_same_shape(r, phi, ctx))) return rc;
+  if (kernel == 0 && (rc = check_same_shape(r, k, ctx))) return rc;
+  if (r->values == phi->values || (kernel == 0 && r->values == k->values))
     return set_error(NEBO_ERR_INVALID, ctx, "result must not alias an operand");
   nebo_limits_t L;
   int empty = 0;
-  stencil_range(k, &L, &empty);
+  stencil_range(phi, &L, &empty);
   if (empty) return set_error(NEBO_ERR_INVALID, ctx, "no interior cells to time");
+  auto run = [&](int sh) {
+    return kernel == 0 ? launch_divk_shape(sh, 0, r, k, phi, *c, 0.0, L, stream)
+                       : launch_lapl_shape(sh, r, phi, *c, 0.5, L, stream);
+  };
   hipStream_t s = (hipStream_t)stream;
   hipEvent_t e0, e1;
   if ((rc = check_hip(hipEventCreate(&e0), ctx))) return rc;
-  if ((rc = check_hip(hipEventCreate(&e1), ctx))) return rc;
-  int best = g_divk_shape.load();
+  if ((rc = check_hip(hipEventCreate(&e1), ctx))) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  std::atomic<int>& slot = kernel == 0 ? g_divk_shape : g_lapl_shape;
+  int best = slot.load();
   float best_ms = 1e30f;
   for (int sh = 0; sh < kNumShapes && !rc; ++sh) {
-    rc = launch_divk_shape(sh, 0, r, k, phi, *c, 0.0, L, stream);   // warm
+    rc = run(sh);   // warm
     if (!rc) rc = check_hip(hipEventRecord(e0, s), ctx);
-    for (int rep = 0; rep < 5 && !rc; ++rep) rc = launch_divk_shape(sh, 0, r, k, phi, *c, 0.0, L, stream);
+    for (int rep = 0; rep < 5 && !rc; ++rep) rc = run(sh);
     if (!rc) rc = check_hip(hipEventRecord(e1, s), ctx);
     if (!rc) rc = check_hip(hipEventSynchronize(e1), ctx);
     float ms = 0.f;
@@ -443,19 +717,21 @@ extern "C" int nebo_stencil_autotune(const nebo_field_t* r, const nebo_field_t* 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  g_divk_shape.store(best);
+  slot.store(best);
   if (ty) *ty = kShapes[best].ty;
   if (zchunk) *zchunk = kShapes[best].zc;
+  if (variant) *variant = kShapes[best].variant;
   return NEBO_OK;
 }
 
-extern "C" int nebo_stencil_tiles(int32_t* n, int32_t* ty, int32_t* zchunk)
+extern "C" int nebo_stencil_tiles(int32_t* n, int32_t* ty, int32_t* zchunk, int32_t* variant)
 {
   clear_error();
   if (n) *n = kNumShapes;
   for (int s = 0; s < kNumShapes; ++s) {
     if (ty) ty[s] = kShapes[s].ty;
     if (zchunk) zchunk[s] = kShapes[s].zc;
+    if (variant) variant[s] = kShapes[s].variant;
   }
   return NEBO_OK;
 }

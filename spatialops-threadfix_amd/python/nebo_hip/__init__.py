@@ -12,7 +12,7 @@ from .api import (lower_stats, assign, interior_assign, masked_assign, assign_li
                   field_max, field_min, field_sum, field_norm, field_max_interior,
                   field_min_interior, field_sum_interior, field_norm_interior,
                   fill_ghosts_periodic, divkgrad, lapl_euler, device_count, stencil_tiles,
-                  stencil_set_tile, stencil_get_tile, stencil_autotune)
+                  stencil_set_tile, stencil_get_tile, stencil_autotune, KERNEL_DIVKGRAD, KERNEL_LAPL)
 from .batch import Batch, alloc_patches  # noqa: F401,E402
 from . import adjoint  # noqa: F401,E402
 from . import workloads  # noqa: F401,E402

@@ -137,6 +137,31 @@ def test_diffusion_step_and_adjoint_bit_exact(n):
     assert np.array_equal(gk.cpu().numpy(), rg)
 
 
+def test_diffusion_step_every_tile_bit_exact():
+    """The Euler form of the diffusion kernel under every stencil tile."""
+    n = (70, 21, 29)
+    L = nh.Layout.build(n, 1)
+    c = nh.axis_coefs(nh.build_stencils(*n, 1.0, 1.0, 1.0))
+    rng = np.random.default_rng(77)
+    N = n[0] * n[1] * n[2]
+    k = _periodic(L, rng.uniform(0.5, 2.0, N))
+    phi = _periodic(L, rng.standard_normal(N))
+    dt = 1e-3
+    ro = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+    orc.diffusion_step(ro, nh.Field(L, host=phi), nh.Field(L, host=k), c, dt)
+    gp, gkk = torch.from_numpy(phi).cuda(), torch.from_numpy(k).cuda()
+    saved = nh.stencil_get_tile(0)
+    try:
+        for tile in nh.stencil_tiles():
+            nh.stencil_set_tile(0, *tile)
+            out = torch.zeros_like(gp)
+            adj.diffusion_step(L, out, gp, gkk, c, dt)
+            torch.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy(), ro.host), tile
+    finally:
+        nh.stencil_set_tile(0, *saved)
+
+
 def _trajectory_case(n, steps, seed=5):
     L = nh.Layout.build((n, n, n), 1)
     db = nh.build_stencils(n, n, n, 1.0, 1.0, 1.0)

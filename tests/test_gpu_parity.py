@@ -239,21 +239,29 @@ def test_every_stencil_tile_bit_exact():
     c = nh.axis_coefs(db)
     ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
     orc.divkgrad(ref, k, phi, c, 0)
+    ref1 = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+    orc.divkgrad(ref1, k, phi, c, 1)
     lref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
     orc.lapl_euler(lref, phi, c, 0.01)
     saved = nh.stencil_get_tile(0), nh.stencil_get_tile(1)
     try:
-        for ty, zc in nh.stencil_tiles():
-            nh.stencil_set_tile(0, ty, zc)
-            nh.stencil_set_tile(1, ty, zc)
-            r = gpu_empty_like(L, 0.0)
-            nh.divkgrad(r, k, phi, c, 0)
-            assert np.array_equal(fetch(r), ref.host), (ty, zc)
+        for tile in nh.stencil_tiles():
+            nh.stencil_set_tile(0, *tile)
+            nh.stencil_set_tile(1, *tile)
+            for form in (0, 1):
+                r = gpu_empty_like(L, 0.0)
+                nh.divkgrad(r, k, phi, c, form)
+                if form == 0:
+                    assert np.array_equal(fetch(r), ref.host), tile
+                else:
+                    assert np.array_equal(fetch(r), ref1.host), tile
             o = gpu_empty_like(L, 0.0)
             nh.lapl_euler(o, phi, c, 0.01)
-            assert np.array_equal(fetch(o), lref.host), (ty, zc)
+            assert np.array_equal(fetch(o), lref.host), tile
         best, times = nh.stencil_autotune(gpu_empty_like(L), k, phi, c)
         assert best in nh.stencil_tiles() and len(times) == len(nh.stencil_tiles())
+        best, times = nh.stencil_autotune(gpu_empty_like(L), None, phi, c, kernel=nh.KERNEL_LAPL)
+        assert best in nh.stencil_tiles() and nh.stencil_get_tile(1) == best
     finally:
         nh.stencil_set_tile(0, *saved[0])
         nh.stencil_set_tile(1, *saved[1])
