@@ -173,24 +173,29 @@ def adjoint_leg(n=128, steps=100, every=10):
             "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3)}
 
 
-def transport_leg(n=512, steps=20):
+def transport_leg(n=512, steps=20, every=5):
     """Scalar transport at 512^3: upwinded convective flux (cond on the sign of the
-    face velocity) plus div(k grad phi), SSP-RK3, 20 steps; each stage is one fused
-    assignment on the generic evaluator followed by a periodic ghost fill."""
-    import math
+    face velocity) plus div(k grad phi), SSP-RK3, 20 steps.  Timed two ways: each
+    stage as one fused assignment on the generic evaluator, and on the hand-written
+    stage kernel (the two must agree bit for bit); then the adjoint of the 20 steps
+    with respect to k and phi0 with checkpoints every 5 steps."""
     import torch
     import nebo_hip as nh
     from nebo_hip import workloads as W
+    from nebo_hip import transport as tr
     L, Ls, ops, k, phi, u, dt = W.transport_problem_gpu(n)
+    c = nh.axis_coefs(nh.build_stencils(n, n, n, 1.0, 1.0, 1.0))
     K = nh.Field(L, gpu=k)
     U = [nh.Field(l, gpu=x) for l, x in zip(Ls, u)]
-    P = [nh.Field(L, gpu=phi), nh.Field(L, gpu=phi.clone()), nh.Field(L, gpu=phi.clone())]
+    P = [nh.Field(L, gpu=phi.clone()), nh.Field(L, gpu=phi.clone()), nh.Field(L, gpu=phi.clone())]
     lo, hi = W.upwind_ops()
     s1, s2, s3 = W.rk3_stage_exprs(ops, P[0], K, U, dt, lo, hi)
     e1, e2, e3 = s1(), s2(P[1]), s3(P[2])
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    abs0 = nh.field_sum_interior(nh.abs_(P[0]))
     sum0 = nh.field_sum_interior(P[0])
 
-    def step():
+    def generic_step():
         nh.assign(P[1], e1)
         nh.fill_ghosts_periodic(P[1])
         nh.assign(P[2], e2)
@@ -198,22 +203,56 @@ def transport_leg(n=512, steps=20):
         nh.assign(P[0], e3)
         nh.fill_ghosts_periodic(P[0])
 
-    step()
+    H = [phi.clone(), torch.empty_like(phi), torch.empty_like(phi), torch.empty_like(phi)]
+
+    def hand_step(cur):
+        nxt = 3 - cur if cur in (0, 3) else 0
+        tr.stage(L, H[1], H[cur], None, k, u, c, dt, 1, face_layouts=Ls)
+        tr.stage(L, H[2], H[cur], H[1], k, u, c, dt, 2, face_layouts=Ls)
+        tr.stage(L, H[nxt], H[cur], H[2], k, u, c, dt, 3, face_layouts=Ls)
+        return nxt
+
+    generic_step()
+    cur = hand_step(0)
     torch.cuda.synchronize()
-    e0, e1_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+    a0, a1, b0, b1 = ev(), ev(), ev(), ev()
+    a0.record()
     for _ in range(steps - 1):
-        step()
-    e1_.record()
+        generic_step()
+    a1.record()
+    b0.record()
+    for _ in range(steps - 1):
+        cur = hand_step(cur)
+    b1.record()
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1_) / (steps - 1)
+    gen_ms = a0.elapsed_time(a1) / (steps - 1)
+    hand_ms = b0.elapsed_time(b1) / (steps - 1)
+    same = bool(torch.equal(P[0].gpu, H[cur]))
     sum1 = nh.field_sum_interior(P[0])
+    # adjoint: forward with checkpoints, then the reverse sweep
+    phiT, ckpt = tr.forward(L, phi, k, u, c, dt, steps, every, face_layouts=Ls)
+    tr.backward(L, phiT.clone(), k, u, c, dt, steps, every, ckpt, face_layouts=Ls)
+    torch.cuda.synchronize()
+    f0, f1, f2 = ev(), ev(), ev()
+    f0.record()
+    phiT, ckpt = tr.forward(L, phi, k, u, c, dt, steps, every, face_layouts=Ls)
+    f1.record()
+    lam0, gk = tr.backward(L, phiT.clone(), k, u, c, dt, steps, every, ckpt, face_layouts=Ls)
+    f2.record()
+    torch.cuda.synchronize()
+    fwd_ms, bwd_ms = f0.elapsed_time(f1), f1.elapsed_time(f2)
     cells = n ** 3
-    return {"n": n, "steps": steps, "ms_per_rk3_step": round(ms, 3),
-            "gcells_per_s_per_stage": round(3 * cells / (ms * 1e-3) / 1e9, 2),
-            "stage_hbm_frac": round(3 * 56 * cells / (ms * 1e-3) / HBM_PEAK, 4),
-            "relative_sum_drift": abs(sum1 - sum0) / max(1e-300, abs(sum0)) if sum0 else abs(sum1),
-            "program_insns": nh.lower_stats(e3, where="gpu")[0]}
+    return {"n": n, "steps": steps,
+            "generic_ms_per_rk3_step": round(gen_ms, 3),
+            "kernel_ms_per_rk3_step": round(hand_ms, 3),
+            "kernel_gcells_per_s_per_stage": round(3 * cells / (hand_ms * 1e-3) / 1e9, 2),
+            "kernel_stage_hbm_frac": round(3 * 56 * cells / (hand_ms * 1e-3) / HBM_PEAK, 4),
+            "generic_stage_hbm_frac": round(3 * 56 * cells / (gen_ms * 1e-3) / HBM_PEAK, 4),
+            "kernel_equals_generic": same,
+            "sum_drift_over_sum_abs": abs(sum1 - sum0) / abs0,
+            "program_insns": nh.lower_stats(e3, where="gpu")[0],
+            "adjoint": {"checkpoint_every": every, "forward_ms": round(fwd_ms, 2),
+                        "backward_ms": round(bwd_ms, 2), "backward_over_forward": round(bwd_ms / fwd_ms, 3)}}
 
 
 def pmc_probe(n, tile=None):

@@ -373,6 +373,55 @@ int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk,
                             int32_t n_steps, int32_t every, const double* ckpt,
                             double* work, void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* scalar transport chain: upwinded convection + var-coefficient diffusion,   */
+/* SSP-RK3, periodic, and its adjoint                                         */
+/* ------------------------------------------------------------------------- */
+
+/* One SSP-RK3 stage of R(s) = div(interp(k)*grad(s)) - div(u*upwind(s)), where
+ * upwind(s) on a face is cond(u > 0, s of the cell below)(s of the cell above):
+ *   stage 1: out = phin + dt*R(phin)                  (s unused, may be NULL)
+ *   stage 2: out = 0.75*phin + 0.25*(s + dt*R(s))
+ *   stage 3: out = (1/3)*phin + (2/3)*(s + dt*R(s))
+ * over the interior, then out's periodic ghosts.  u points to the three face
+ * velocity fields (x, y, z faces).  Every operand has one periodic ghost layer
+ * and the same window.  Bit-identical to nebo_assign of the fused stage
+ * expression (built from the reference's Divergence/Interpolant/Gradient
+ * stencils and cond, NeboStencilBuilder.h:134-148, NeboCond.h:3801-4465)
+ * followed by nebo_fill_ghosts_periodic. */
+int nebo_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
+                         const nebo_field_t* s, const nebo_field_t* k,
+                         const nebo_field_t* u, const nebo_axis_coefs_t* c, double dt,
+                         int stage, void* stream);
+/* Transpose of one stage for adjoint a (periodic ghosts), with X = a + dt*R_s^T a:
+ *   stage 3: out = (2/3)*X,        acc_out = (1/3)*a,          gk += (2/3)*dt*R_k(s)^T a
+ *   stage 2: out = 0.25*X,         acc_out = acc_in + 0.75*a,  gk += 0.25*dt*R_k(s)^T a
+ *   stage 1: out = acc_in + X,                                  gk += dt*R_k(s)^T a
+ * (R_s: R as a linear map of s; R_k(s): its diffusion part as a linear map of k.)
+ * acc_in / acc_out / gk are blocks of the field's shape (interior cells used);
+ * out's periodic ghosts are filled.  No reference counterpart (the reference has
+ * no adjoint). */
+int nebo_transport_adjoint_stage(const nebo_field_t* out, double* acc_out, double* gk,
+                                 const nebo_field_t* a, const double* acc_in,
+                                 const nebo_field_t* s, const nebo_field_t* k,
+                                 const nebo_field_t* u, const nebo_axis_coefs_t* c,
+                                 double dt, int stage, void* stream);
+/* n_steps RK3 steps from phi0 (its periodic ghosts are filled in the first
+ * checkpoint), keeping phi0 and every `every`-th state: ckpt holds
+ * n_steps/every + 1 blocks.  phiT receives the final state. */
+int nebo_transport_forward(const nebo_field_t* phiT, const nebo_field_t* phi0,
+                           const nebo_field_t* k, const nebo_field_t* u,
+                           const nebo_axis_coefs_t* c, double dt, int32_t n_steps,
+                           int32_t every, double* ckpt, void* stream);
+/* Reverse sweep for lamT = dL/d phiT: each segment is recomputed from its
+ * checkpoint (stage states kept), then swept back stage by stage.  On return
+ * lam0 = dL/d phi0 and gk = dL/dk on interior cells (ghosts zero).  work holds
+ * 3*every + 5 blocks. */
+int nebo_transport_backward(const nebo_field_t* lam0, double* gk, const nebo_field_t* lamT,
+                            const nebo_field_t* k, const nebo_field_t* u,
+                            const nebo_axis_coefs_t* c, double dt, int32_t n_steps,
+                            int32_t every, const double* ckpt, double* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -745,3 +745,142 @@ void oracle_fill_normal(double* dst, size_t n, uint64_t seed)
     if (i + 1 < n) dst[i + 1] = r * sin(6.283185307179586 * u2);
   }
 }
+
+/* ------------------------------------------------------------------------- */
+/* scalar transport: upwinded convection + var-coefficient diffusion, SSP-RK3 */
+/* ------------------------------------------------------------------------- */
+
+/* The right-hand side R(s) = div(interp(k)*grad(s)) - div(u*upwind(s)) at one
+ * interior cell, in the order the fused expression
+ *   (Dx(Ix(k)*Gx(s)) + Dy(..)) + Dz(..)  -  ((Dx(c_x*u_x) + Dy(..)) + Dz(..)),
+ *   c_d = cond(u_d > 0, s[-1 along d])(s[0])
+ * is evaluated by the tree walk (NeboStencils.h:407-448 fold, NeboCond.h:408-412
+ * select, NeboOperators.h SumOp/DiffOp/ProdOp), where every one-point shift
+ * stencil multiplies by its coefficient 1.0. */
+static double transport_rhs_cell(const nebo_field_t* s, const nebo_field_t* k,
+                                 const nebo_field_t* u, const nebo_axis_coefs_t* c,
+                                 int x, int y, int z)
+{
+  double f[3], v[3];
+  for (int d = 0; d < 3; ++d) {
+    int dx = d == 0, dy = d == 1, dz = d == 2;
+    double lo = face_flux(k, s, c, d, x, y, z);
+    double hi = face_flux(k, s, c, d, x + dx, y + dy, z + dz);
+    f[d] = lo * c->div[d][0] + hi * c->div[d][1];
+    const nebo_field_t* ud = &u[d];
+    double ul = AT(ud, x, y, z), uh = AT(ud, x + dx, y + dy, z + dz);
+    double cl = ul > 0.0 ? AT(s, x - dx, y - dy, z - dz) * 1.0 : AT(s, x, y, z) * 1.0;
+    double ch = uh > 0.0 ? AT(s, x, y, z) * 1.0 : AT(s, x + dx, y + dy, z + dz) * 1.0;
+    v[d] = (cl * ul) * c->div[d][0] + (ch * uh) * c->div[d][1];
+  }
+  return ((f[0] + f[1]) + f[2]) - ((v[0] + v[1]) + v[2]);
+}
+
+void oracle_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
+                            const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
+                            const nebo_axis_coefs_t* c, double dt, int stage)
+{
+  nebo_limits_t L;
+  interior_limits(out, &L);
+  double* o = (double*)out->values;
+  for (int z = L.lo[2]; z < L.hi[2]; ++z)
+    for (int y = L.lo[1]; y < L.hi[1]; ++y)
+      for (int x = L.lo[0]; x < L.hi[0]; ++x) {
+        double r = transport_rhs_cell(s, k, u, c, x, y, z);
+        double pn = AT(phin, x, y, z), sv = AT(s, x, y, z);
+        double v;
+        if (stage == 1) v = pn + dt * r;
+        else if (stage == 2) v = 0.75 * pn + 0.25 * (sv + dt * r);
+        else v = (1.0 / 3.0) * pn + (2.0 / 3.0) * (sv + dt * r);
+        o[field_index(out, x, y, z)] = v;
+      }
+  oracle_fill_ghosts_periodic(out);
+}
+
+/* Transpose of one SSP-RK3 stage.  With R_s the (linear in s) right-hand side
+ * for fixed k and u, and R_k(s) its (linear in k) diffusion part for fixed s:
+ *   R_s^T a = sum_d G_d^T(kf_d .* w_d) - sum_d U_d^T(u_d .* w_d),  w_d = D_d^T a
+ *   R_k(s)^T a = sum_d I_d^T(w_d .* G_d s)
+ * with, for face f = low face of cell f along d,
+ *   (D^T a)_f = a_f*d0 + a_{f-1}*d1        (G^T v)_c = v_{c+1}*g0 + v_c*g1
+ *   (I^T v)_c = v_{c+1}*i0 + v_c*i1        (U^T v)_c = [u_{c+1} > 0] v_{c+1} + [!(u_c > 0)] v_c
+ * and X = a + dt*R_s^T a, the stage outputs are
+ *   stage 3: out = (2/3)*X,          acc_out = (1/3)*a,        gk += ((2/3)*dt) * R_k(s)^T a
+ *   stage 2: out = 0.25*X,           acc_out = acc_in + 0.75*a, gk += (0.25*dt) * R_k(s)^T a
+ *   stage 1: out = acc_in + X,                                  gk += dt * R_k(s)^T a
+ * Separate passes over face temporaries, one per axis; the periodic ghosts of a,
+ * s and k hold the wrapped interior, so faces 0 and n are one physical face. */
+void oracle_transport_adjoint_stage(const nebo_field_t* out, double* acc_out, double* gk,
+                                    const nebo_field_t* a, const double* acc_in,
+                                    const nebo_field_t* s, const nebo_field_t* k,
+                                    const nebo_field_t* u, const nebo_axis_coefs_t* c, double dt,
+                                    int stage)
+{
+  nebo_limits_t L;
+  interior_limits(out, &L);
+  int nx = L.hi[0], ny = L.hi[1], nz = L.hi[2];
+  size_t ncell = (size_t)nx * ny * nz;
+  double* A[3];
+  double* V[3];
+  double* B[3];
+  for (int d = 0; d < 3; ++d) {
+    int fx = nx + (d == 0), fy = ny + (d == 1), fz = nz + (d == 2);
+    size_t nf = (size_t)fx * fy * fz;
+    double* kw = (double*)malloc(nf * sizeof(double));
+    double* wg = (double*)malloc(nf * sizeof(double));
+    double* uw = (double*)malloc(nf * sizeof(double));
+    double* uf = (double*)malloc(nf * sizeof(double));
+    int dx = d == 0, dy = d == 1, dz = d == 2;
+    for (int z = 0; z < fz; ++z)
+      for (int y = 0; y < fy; ++y)
+        for (int x = 0; x < fx; ++x) {
+          size_t fi = (size_t)x + (size_t)fx * (y + (size_t)fy * z);
+          double w = AT(a, x, y, z) * c->div[d][0] + AT(a, x - dx, y - dy, z - dz) * c->div[d][1];
+          double gs = AT(s, x - dx, y - dy, z - dz) * c->grad[d][0] + AT(s, x, y, z) * c->grad[d][1];
+          double kf = AT(k, x - dx, y - dy, z - dz) * c->interp[d][0] + AT(k, x, y, z) * c->interp[d][1];
+          uf[fi] = AT(&u[d], x, y, z);
+          kw[fi] = kf * w;
+          wg[fi] = w * gs;
+          uw[fi] = uf[fi] * w;
+        }
+    A[d] = (double*)malloc(ncell * sizeof(double));
+    V[d] = (double*)malloc(ncell * sizeof(double));
+    B[d] = (double*)malloc(ncell * sizeof(double));
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          size_t ci = (size_t)x + (size_t)nx * (y + (size_t)ny * z);
+          size_t flo = (size_t)x + (size_t)fx * (y + (size_t)fy * z);
+          size_t fhi = (size_t)(x + dx) + (size_t)fx * ((y + dy) + (size_t)fy * (z + dz));
+          A[d][ci] = kw[fhi] * c->grad[d][0] + kw[flo] * c->grad[d][1];
+          V[d][ci] = (uf[fhi] > 0.0 ? uw[fhi] : 0.0) + (uf[flo] > 0.0 ? 0.0 : uw[flo]);
+          B[d][ci] = wg[fhi] * c->interp[d][0] + wg[flo] * c->interp[d][1];
+        }
+    free(kw); free(wg); free(uw); free(uf);
+  }
+  const double cs = stage == 3 ? 2.0 / 3.0 : stage == 2 ? 0.25 : 1.0;
+  const double cdt = cs * dt;
+  double* o = (double*)out->values;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        size_t ci = (size_t)x + (size_t)nx * (y + (size_t)ny * z);
+        size_t fi = field_index(out, x, y, z);
+        double av = AT(a, x, y, z);
+        double rt = ((A[0][ci] + A[1][ci]) + A[2][ci]) - ((V[0][ci] + V[1][ci]) + V[2][ci]);
+        double X = av + dt * rt;
+        if (stage == 3) {
+          o[fi] = cs * X;
+          acc_out[fi] = (1.0 / 3.0) * av;
+        } else if (stage == 2) {
+          o[fi] = cs * X;
+          acc_out[fi] = acc_in[fi] + 0.75 * av;
+        } else {
+          o[fi] = acc_in[fi] + X;
+        }
+        size_t gi = field_index(k, x, y, z);
+        gk[gi] = gk[gi] + cdt * ((B[0][ci] + B[1][ci]) + B[2][ci]);
+      }
+  for (int d = 0; d < 3; ++d) { free(A[d]); free(V[d]); free(B[d]); }
+  oracle_fill_ghosts_periodic(out);
+}

@@ -72,6 +72,20 @@ void oracle_diffusion_adjoint_step(const nebo_field_t* lam_out, double* gk,
                                    const nebo_field_t* k, const nebo_axis_coefs_t* c,
                                    double dt);
 
+/* Scalar transport (upwinded convection by the sign of a face velocity plus
+ * var-coefficient diffusion), periodic, one ghost layer: one SSP-RK3 stage
+ * (stage 1: phin + dt*R(phin); 2: 0.75*phin + 0.25*(s + dt*R(s));
+ * 3: (1/3)*phin + (2/3)*(s + dt*R(s))) and its transpose.  u points to the three
+ * face-velocity fields (x, y, z).  Both fill the output's periodic ghosts. */
+void oracle_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
+                            const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
+                            const nebo_axis_coefs_t* c, double dt, int stage);
+void oracle_transport_adjoint_stage(const nebo_field_t* out, double* acc_out, double* gk,
+                                    const nebo_field_t* a, const double* acc_in,
+                                    const nebo_field_t* s, const nebo_field_t* k,
+                                    const nebo_field_t* u, const nebo_axis_coefs_t* c, double dt,
+                                    int stage);
+
 /* Deterministic inputs shared by CPU and GPU runs. */
 uint64_t oracle_splitmix64(uint64_t* state);
 void oracle_fill_uniform(double* dst, size_t n, uint64_t seed, double lo, double hi);

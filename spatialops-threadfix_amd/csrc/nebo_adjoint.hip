@@ -17,29 +17,55 @@
 // (48 bytes per cell); the z faces are carried in registers.
 
 #include "nebo_internal.h"
+#include "nebo_device.h"
 
 #include <hip/hip_runtime.h>
 
 #include <sstream>
 
 namespace nebo {
+
+int check_periodic_operand(const nebo_field_t* f, const nebo_field_t* ref, const char* ctx)
+{
+  for (int d = 0; d < 3; ++d) {
+    if (f->glob[d] != ref->glob[d] || f->extent[d] != ref->extent[d] || f->offset[d] != ref->offset[d])
+      return set_error(NEBO_ERR_INVALID, ctx, "operands must share one memory window");
+    if (f->ghost_minus[d] != 1 || f->ghost_plus[d] != 1 || f->valid_minus[d] < 1 || f->valid_plus[d] < 1)
+      return set_error(NEBO_ERR_GHOST, ctx, "periodic operands need exactly one ghost layer per face");
+    if (f->extent[d] - 2 < 1)
+      return set_error(NEBO_ERR_INVALID, ctx, "empty interior");
+  }
+  if ((long)f->glob[0] * f->glob[1] * f->glob[2] >= (1L << 31))
+    return set_error(NEBO_ERR_TOO_LARGE, ctx, "field has 2^31 or more points");
+  if (f->device < 0 || f->device != ref->device)
+    return set_error(NEBO_ERR_DEVICE, ctx, "operands must be resident on one GPU");
+  return NEBO_OK;
+}
+
+nebo_limits_t interior_of(const nebo_field_t* f)
+{
+  nebo_limits_t L;
+  for (int d = 0; d < 3; ++d) {
+    L.lo[d] = 0;
+    L.hi[d] = f->extent[d] - 2;
+  }
+  return L;
+}
+
+nebo_field_t with_values(const nebo_field_t* f, void* v)
+{
+  nebo_field_t g = *f;
+  g.values = v;
+  return g;
+}
+
+size_t field_npts(const nebo_field_t* f) { return (size_t)f->glob[0] * f->glob[1] * f->glob[2]; }
+
 namespace {
 
 constexpr int kTx = 64;
 constexpr int kTy = 4;
 constexpr int kZChunk = 16;
-
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, k = b >> 3;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
-struct Box3 {
-  int n[3];
-  int nx, ny, nz;  // tile counts
-};
 
 struct FaceQ {
   double kw;  // kf * w
@@ -105,41 +131,10 @@ __global__ __launch_bounds__(kTx * kTy) void adjoint_kernel(
 
 const char* kCtx = "Nebo Adjoint";
 
-// All operands share one glob block shape, one ghost layer and a full periodic
-// window (offset 0, extent = glob), and sit on one GPU.
 int check_periodic_field(const nebo_field_t* f, const nebo_field_t* ref)
 {
-  for (int d = 0; d < 3; ++d) {
-    if (f->glob[d] != ref->glob[d] || f->extent[d] != ref->extent[d] || f->offset[d] != ref->offset[d])
-      return set_error(NEBO_ERR_INVALID, kCtx, "operands must share one memory window");
-    if (f->ghost_minus[d] != 1 || f->ghost_plus[d] != 1 || f->valid_minus[d] < 1 || f->valid_plus[d] < 1)
-      return set_error(NEBO_ERR_GHOST, kCtx, "periodic diffusion needs exactly one ghost layer per face");
-    if (f->extent[d] - 2 < 1)
-      return set_error(NEBO_ERR_INVALID, kCtx, "empty interior");
-  }
-  if (f->device < 0 || f->device != ref->device)
-    return set_error(NEBO_ERR_DEVICE, kCtx, "operands must be resident on one GPU");
-  return NEBO_OK;
+  return check_periodic_operand(f, ref, kCtx);
 }
-
-nebo_limits_t interior_of(const nebo_field_t* f)
-{
-  nebo_limits_t L;
-  for (int d = 0; d < 3; ++d) {
-    L.lo[d] = 0;
-    L.hi[d] = f->extent[d] - 2;
-  }
-  return L;
-}
-
-nebo_field_t with_values(const nebo_field_t* f, void* v)
-{
-  nebo_field_t g = *f;
-  g.values = v;
-  return g;
-}
-
-size_t field_npts(const nebo_field_t* f) { return (size_t)f->glob[0] * f->glob[1] * f->glob[2]; }
 
 // out = phi + dt*div(interp(k)*grad(phi)), then out's periodic ghosts.  phi's
 // ghosts must already hold the periodic images.

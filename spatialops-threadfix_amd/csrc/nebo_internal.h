@@ -134,6 +134,17 @@ int launch_diffusion_euler(const nebo_field_t* out, const nebo_field_t* k, const
 // Periodic copy-fill of every allocated ghost layer of f.
 int fill_ghosts_periodic(const nebo_field_t* f, void* stream);
 
+// Periodic operands (diffusion and transport chains): f shares ref's memory
+// window and device, has exactly one ghost layer per face (valid) and fewer than
+// 2^31 points.  Sets the error with context ctx.
+int check_periodic_operand(const nebo_field_t* f, const nebo_field_t* ref, const char* ctx);
+// [0, extent - 2) per axis: the interior of a one-ghost periodic field.
+nebo_limits_t interior_of(const nebo_field_t* f);
+// f's descriptor with another values pointer (same window).
+nebo_field_t with_values(const nebo_field_t* f, void* v);
+// glob[0] * glob[1] * glob[2]
+size_t field_npts(const nebo_field_t* f);
+
 // Pointer of the first valid-ghost-excluded point of a field (the frame origin).
 inline double* frame_origin(const nebo_field_t* f)
 {

@@ -15,4 +15,5 @@ from .api import (lower_stats, assign, interior_assign, masked_assign, assign_li
                   stencil_set_tile, stencil_get_tile, stencil_autotune, KERNEL_DIVKGRAD, KERNEL_LAPL)
 from .batch import Batch, alloc_patches  # noqa: F401,E402
 from . import adjoint  # noqa: F401,E402
+from . import transport  # noqa: F401,E402
 from . import workloads  # noqa: F401,E402

@@ -8,6 +8,8 @@ import ctypes as C
 import os
 import sys
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "spatialops-threadfix_amd", "python"))
 
@@ -40,6 +42,11 @@ _SIGS = {
                                      C.c_double]),
     "oracle_diffusion_adjoint_step": (None, [_P(_abi.FieldT), _vp, _P(_abi.FieldT), _P(_abi.FieldT),
                                              _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double]),
+    "oracle_transport_stage": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT),
+                                      _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double, C.c_int]),
+    "oracle_transport_adjoint_stage": (None, [_P(_abi.FieldT), _vp, _vp, _P(_abi.FieldT), _vp, _P(_abi.FieldT),
+                                              _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double,
+                                              C.c_int]),
     "oracle_splitmix64": (C.c_uint64, [_P(C.c_uint64)]),
     "oracle_fill_uniform": (None, [_vp, C.c_size_t, C.c_uint64, C.c_double, C.c_double]),
     "oracle_fill_normal": (None, [_vp, C.c_size_t, C.c_uint64]),
@@ -139,3 +146,62 @@ def diffusion_adjoint_step(lam_out, gk, lam, phi, k, coefs, dt):
     lib().oracle_diffusion_adjoint_step(C.byref(dlo), gk.ctypes.data, C.byref(dl), C.byref(dp), C.byref(dk),
                                         C.byref(coefs), float(dt))
     return lam_out
+
+
+def _face_array(us):
+    arr = (_abi.FieldT * 3)()
+    for d in range(3):
+        arr[d] = us[d].desc("host")
+    return arr
+
+
+def transport_stage(out, phin, s, k, us, coefs, dt, stage):
+    """One SSP-RK3 stage of the transport right-hand side (fills out's periodic ghosts)."""
+    do, dn, ds, dk = out.desc("host"), phin.desc("host"), s.desc("host"), k.desc("host")
+    lib().oracle_transport_stage(C.byref(do), C.byref(dn), C.byref(ds), C.byref(dk), _face_array(us),
+                                 C.byref(coefs), float(dt), int(stage))
+    return out
+
+
+def transport_adjoint_stage(out, acc_out, gk, a, acc_in, s, k, us, coefs, dt, stage):
+    """Transpose of one stage; acc_out / acc_in / gk are numpy arrays of the field's size
+    (acc_out unused for stage 1, acc_in unused for stage 3)."""
+    do, da, ds, dk = out.desc("host"), a.desc("host"), s.desc("host"), k.desc("host")
+    ptr = lambda x: None if x is None else x.ctypes.data
+    lib().oracle_transport_adjoint_stage(C.byref(do), ptr(acc_out), ptr(gk), C.byref(da), ptr(acc_in),
+                                         C.byref(ds), C.byref(dk), _face_array(us), C.byref(coefs),
+                                         float(dt), int(stage))
+    return out
+
+
+def transport_forward_cpu(L, phi0, k, us, coefs, dt, steps):
+    """SSP-RK3 trajectory through the oracle's stages.  Returns the per-step
+    (phi_n, s1, s2) states (host arrays) and phi_T."""
+    from nebo_hip import Field
+    hf = lambda a: Field(L, host=np.asarray(a, dtype=np.float64).copy())
+    states, phi = [], np.asarray(phi0, dtype=np.float64).copy()
+    for _ in range(steps):
+        s1, s2, nxt = hf(np.zeros(L.npts)), hf(np.zeros(L.npts)), hf(np.zeros(L.npts))
+        transport_stage(s1, hf(phi), hf(phi), k, us, coefs, dt, 1)
+        transport_stage(s2, hf(phi), s1, k, us, coefs, dt, 2)
+        transport_stage(nxt, hf(phi), s2, k, us, coefs, dt, 3)
+        states.append((phi, s1.host, s2.host))
+        phi = nxt.host
+    return states, phi
+
+
+def transport_backward_cpu(L, lamT, k, us, coefs, dt, states):
+    """Reverse sweep through the oracle's adjoint stages: (dL/dphi_0, dL/dk)."""
+    from nebo_hip import Field
+    hf = lambda a: Field(L, host=np.asarray(a, dtype=np.float64).copy())
+    gk = np.zeros(L.npts)
+    lam = np.asarray(lamT, dtype=np.float64).copy()
+    for phin, s1, s2 in reversed(states):
+        acc = np.zeros(L.npts)
+        mu2, mu1, out = hf(np.zeros(L.npts)), hf(np.zeros(L.npts)), hf(np.zeros(L.npts))
+        transport_adjoint_stage(mu2, acc, gk, hf(lam), None, hf(s2), k, us, coefs, dt, 3)
+        acc2 = np.zeros(L.npts)
+        transport_adjoint_stage(mu1, acc2, gk, mu2, acc, hf(s1), k, us, coefs, dt, 2)
+        transport_adjoint_stage(out, None, gk, mu1, acc2, hf(phin), k, us, coefs, dt, 1)
+        lam = out.host
+    return lam, gk
