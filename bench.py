@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-tune", action="store_true", help="keep the default tile (no autotune)")
     p.add_argument("--tile", default=None,
-                   help="ty,zchunk,variant of the diffusion kernel (implies --no-tune)")
+                   help="tx,ty,zchunk,variant of the diffusion kernel (implies --no-tune)")
     p.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
     p.add_argument("--pmc-probe", action="store_true",
                    help="only launch 3 known-size copies and 3 kernels (for rocprofv3 --pmc runs)")
@@ -43,8 +43,10 @@ def parse():
 
 
 def _tile_name(t):
-    """64 x ty x zchunk, with the schedule: direct loads or LDS-staged + prefetch depth."""
-    return f"64x{t[0]}x{t[1]}" + ("" if t[2] == 0 else f"/staged-pf{t[2]}")
+    """tx x ty x zchunk with the schedule: direct loads, LDS-staged + prefetch depth,
+    or the column schedule (ty rows per lane)."""
+    sched = {0: "", 1: "/staged-pf1", 2: "/staged-pf2", 3: "/column"}[t[3]]
+    return f"{t[0]}x{t[1]}x{t[2]}{sched}"
 
 
 def cpu_baseline(L, kh, ph, coefs, budget_s):

@@ -270,24 +270,28 @@ int nebo_divkgrad(const nebo_field_t* r, const nebo_field_t* k,
 int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
                     const nebo_axis_coefs_t* c, double s, void* stream);
 
-/* Tile of the hand-written stencil kernels: a workgroup covers 64 (x) by ty (y)
- * columns and marches zchunk planes.  variant 0 reads every neighbour straight
- * from memory; variants 1 and 2 stage each plane's rows in LDS, take x
- * neighbours from the adjacent lanes and prefetch 1 or 2 planes ahead.
- * kernel 0 = the var-coefficient diffusion kernels (nebo_divkgrad,
- * nebo_diffusion_step), 1 = Laplacian+Euler.  The tile changes only the
- * schedule, never the results.  The reference has no counterpart (its CUDA
- * launch shape is fixed at 16x16, NeboLhs.h:413-427). */
-int nebo_stencil_tiles(int32_t* n, int32_t* ty, int32_t* zchunk, int32_t* variant); /* arrays of n */
-int nebo_stencil_set_tile(int kernel, int32_t ty, int32_t zchunk, int32_t variant);
-int nebo_stencil_get_tile(int kernel, int32_t* ty, int32_t* zchunk, int32_t* variant);
+/* Tile of the hand-written stencil kernels: a workgroup covers tx (x) by ty (y)
+ * columns and marches zchunk planes.  variant 0 (tx = 64) reads every neighbour
+ * straight from memory; variants 1 and 2 (tx = 64) stage each plane's rows in
+ * LDS, take x neighbours from the adjacent lanes and prefetch 1 or 2 planes
+ * ahead; variant 3 (column schedule) puts tx/64 waves side by side in x, each
+ * lane computing ty rows of one column with no LDS and no barrier, loads issued
+ * two planes ahead.  kernel 0 = the var-coefficient diffusion kernels
+ * (nebo_divkgrad, nebo_diffusion_step), 1 = Laplacian+Euler.  The tile changes
+ * only the schedule, never the results.  The reference has no counterpart (its
+ * CUDA launch shape is fixed at 16x16, NeboLhs.h:413-427). */
+int nebo_stencil_tiles(int32_t* n, int32_t* tx, int32_t* ty, int32_t* zchunk,
+                       int32_t* variant);                               /* arrays of n */
+int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t zchunk, int32_t variant);
+int nebo_stencil_get_tile(int kernel, int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant);
 /* Time every instantiated tile of `kernel` on these operands (r is overwritten on
  * its interior; k is unused for kernel 1), keep the fastest for that kernel and
  * report it; times_ms (optional) receives one mean time per tile in
  * nebo_stencil_tiles order.  Synchronises stream. */
 int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const nebo_field_t* k,
                           const nebo_field_t* phi, const nebo_axis_coefs_t* c, void* stream,
-                          int32_t* ty, int32_t* zchunk, int32_t* variant, float* times_ms);
+                          int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant,
+                          float* times_ms);
 
 /* ------------------------------------------------------------------------- */
 /* batched patches: one persistent launch over a device patch table          */

@@ -58,11 +58,11 @@ struct Tiles {
   int nx, ny, nz;  // tile counts
 };
 
-Tiles make_tiles(const nebo_limits_t& L, int ty, int zc)
+Tiles make_tiles(const nebo_limits_t& L, int tx, int ty, int zc)
 {
   Tiles t;
   for (int d = 0; d < 3; ++d) { t.lo[d] = L.lo[d]; t.hi[d] = L.hi[d]; }
-  t.nx = (L.hi[0] - L.lo[0] + kTx - 1) / kTx;
+  t.nx = (L.hi[0] - L.lo[0] + tx - 1) / tx;
   t.ny = (L.hi[1] - L.lo[1] + ty - 1) / ty;
   t.nz = (L.hi[2] - L.lo[2] + zc - 1) / zc;
   return t;
@@ -272,6 +272,125 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_staged_kernel(
   }
 }
 
+// Lane j's value of v (v_readlane; j is uniform), as seen by every lane.
+__device__ __forceinline__ double lane_value(double v, int j)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
+
+// Loads of one column-schedule lane within a plane, clamped into [lo-1, hi] on
+// every axis: rows y0-1 .. y0+RY of its column, and one edge load: lanes 0..RY-1
+// take the left neighbours (x = xw0-1) of rows 0..RY-1, lanes 32..32+RY-1 the
+// right neighbours (x = xw0+64), other lanes re-read their row 0.
+template <int RY>
+struct ColumnGeom {
+  int row[RY + 2];
+  int edge;
+};
+
+template <int RY>
+__device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int xw0, int y0, int lane)
+{
+  ColumnGeom<RY> g;
+  const int xc = min(xw0 + lane, T.hi[0]);
+#pragma unroll
+  for (int r = 0; r < RY + 2; ++r) g.row[r] = xc + X * min(y0 - 1 + r, T.hi[1]);
+  // (a wave wholly beyond hi, in a partial last tile, has xw0 - 1 > hi: clamp too)
+  if (lane < RY) g.edge = min(xw0 - 1, T.hi[0]) + X * min(y0 + lane, T.hi[1]);
+  else if (lane >= 32 && lane < 32 + RY) g.edge = min(xw0 + kTx, T.hi[0]) + X * min(y0 + lane - 32, T.hi[1]);
+  else g.edge = g.row[1];
+  return g;
+}
+
+// Column schedule (variant 3): W waves side by side in x (a 64*W-wide strip, so
+// the loads in flight cover whole contiguous rows), each lane one x column of RY
+// rows, the wave marching ZC planes.  No LDS and no barrier: y neighbours are the
+// lane's own rows plus one halo row on each side, x neighbours come from the
+// adjacent lanes (DPP) and at the wave's ends from the edge load (readlane).
+// Three register slots hold planes z, z+1 and z+2; once plane z is done its slot
+// receives plane z+3, so loads are issued two planes ahead.  One period of three
+// steps is unrolled (slots are renamed, never copied) and the chunk is padded to
+// whole periods; padded steps load clamped planes and store nothing.
+template <int FORM, int W, int RY, int ZC>
+__global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
+    double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
+    Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
+{
+  static_assert(RY >= 1 && RY <= 16, "rows per lane");
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kTx - 1);
+  const int xw0 = T.lo[0] + tx * (kTx * W) + (threadIdx.x / kTx) * kTx;
+  const int x = xw0 + lane;
+  const int y0 = T.lo[1] + ty * RY;
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
+  const bool xin = x < T.hi[0];
+  const ColumnGeom<RY> g = column_geom<RY>(T, X, xw0, y0, lane);
+  const DivkCoefs c = divk_coefs(cc);
+
+  struct Col { double p[RY + 2], k[RY + 2], pe, ke; };
+  auto load = [&](int z) {
+    Col v;
+    const int b = XY * z;
+#pragma unroll
+    for (int q = 0; q < RY + 2; ++q) {
+      v.p[q] = p[b + g.row[q]];
+      v.k[q] = k[b + g.row[q]];
+    }
+    v.pe = p[b + g.edge];
+    v.ke = k[b + g.edge];
+    return v;
+  };
+
+  Col S[3];
+  S[0] = load(z0);
+  S[1] = load(min(z0 + 1, z1));
+  S[2] = load(min(z0 + 2, z1));
+  double fz[RY];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) {
+    const int b = XY * (z0 - 1) + g.row[j + 1];
+    fz[j] = face(k[b], S[0].k[j + 1], p[b], S[0].p[j + 1], c.iz0, c.iz1, c.gz0, c.gz1);
+  }
+  const int periods = (z1 - z0 + 2) / 3;
+  for (int q = 0; q < periods; ++q) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      const int z = z0 + 3 * q + jj;
+      const Col& C0 = S[jj];              // plane z
+      const Col& C1 = S[(jj + 1) % 3];    // plane z+1
+#pragma unroll
+      for (int j = 0; j < RY; ++j) {
+        const double pc = C0.p[j + 1], kc = C0.k[j + 1];
+        const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
+        const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
+        const double pel = lane_value(C0.pe, j), per = lane_value(C0.pe, 32 + j);
+        const double kel = lane_value(C0.ke, j), ker = lane_value(C0.ke, 32 + j);
+        const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
+        const double kxm = lane == 0 ? kel : kl, kxp = lane == kTx - 1 ? ker : ku;
+        const double fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+        const double fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+        const double fy_lo = face(C0.k[j], kc, C0.p[j], pc, c.iy0, c.iy1, c.gy0, c.gy1);
+        const double fy_hi = face(kc, C0.k[j + 2], pc, C0.p[j + 2], c.iy0, c.iy1, c.gy0, c.gy1);
+        const double fz_hi = face(kc, C1.k[j + 1], pc, C1.p[j + 1], c.iz0, c.iz1, c.gz0, c.gz1);
+        const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
+        const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
+        const double Dz = fz[j] * c.dz0 + fz_hi * c.dz1;
+        fz[j] = fz_hi;
+        const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
+        if (xin && y0 + j < T.hi[1] && z < z1) __builtin_nontemporal_store(v, &r[x + X * (y0 + j) + XY * z]);
+      }
+      S[jj] = load(min(z + 3, z1));       // plane z is consumed: its slot takes z+3
+    }
+  }
+}
+
 // ---- phi + s*lapl(phi) -----------------------------------------------------------
 
 // out = phi + s*((Lx + Ly) + Lz),  L_d = (phi[-1]*g0 + phi[0]*g1)*d0 + (phi[0]*g0 + phi[+1]*g1)*d1
@@ -391,14 +510,79 @@ __global__ __launch_bounds__(kTx * TY) void lapl_euler_staged_kernel(
   }
 }
 
+// Column schedule of the Laplacian + Euler update (see divkgrad_column_kernel).
+template <int W, int RY, int ZC>
+__global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
+    double* __restrict__ out, const double* __restrict__ p, Tiles T, int X, int XY,
+    nebo_axis_coefs_t cc, double s)
+{
+  static_assert(RY >= 1 && RY <= 16, "rows per lane");
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kTx - 1);
+  const int xw0 = T.lo[0] + tx * (kTx * W) + (threadIdx.x / kTx) * kTx;
+  const int x = xw0 + lane;
+  const int y0 = T.lo[1] + ty * RY;
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
+  const bool xin = x < T.hi[0];
+  const ColumnGeom<RY> g = column_geom<RY>(T, X, xw0, y0, lane);
+  const LaplCoefs c = lapl_coefs(cc);
+
+  struct Col { double p[RY + 2], pe; };
+  auto load = [&](int z) {
+    Col v;
+    const int b = XY * z;
+#pragma unroll
+    for (int q = 0; q < RY + 2; ++q) v.p[q] = p[b + g.row[q]];
+    v.pe = p[b + g.edge];
+    return v;
+  };
+
+  Col S[3];
+  S[0] = load(z0);
+  S[1] = load(min(z0 + 1, z1));
+  S[2] = load(min(z0 + 2, z1));
+  double gz[RY];
+#pragma unroll
+  for (int j = 0; j < RY; ++j) gz[j] = p[XY * (z0 - 1) + g.row[j + 1]] * c.gz0 + S[0].p[j + 1] * c.gz1;
+  const int periods = (z1 - z0 + 2) / 3;
+  for (int q = 0; q < periods; ++q) {
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      const int z = z0 + 3 * q + jj;
+      const Col& C0 = S[jj];
+      const Col& C1 = S[(jj + 1) % 3];
+#pragma unroll
+      for (int j = 0; j < RY; ++j) {
+        const double pc = C0.p[j + 1];
+        const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
+        const double pel = lane_value(C0.pe, j), per = lane_value(C0.pe, 32 + j);
+        const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
+        const double gz_hi = pc * c.gz0 + C1.p[j + 1] * c.gz1;
+        const double v = lapl_cell(pc, pxm, pxp, C0.p[j], C0.p[j + 2], gz[j], gz_hi, c, s);
+        gz[j] = gz_hi;
+        if (xin && y0 + j < T.hi[1] && z < z1) __builtin_nontemporal_store(v, &out[x + X * (y0 + j) + XY * z]);
+      }
+      S[jj] = load(min(z + 3, z1));
+    }
+  }
+}
+
 // ---- tile table -------------------------------------------------------------
 
-// variant 0: direct loads; 1, 2: staged with 1 or 2 planes of prefetch
-struct TileShape { int ty, zc, variant; };
+// variant 0: direct loads; 1, 2: LDS-staged with 1 or 2 planes of prefetch;
+// 3: column schedule (ty = rows per lane, tx = 64 * waves side by side)
+struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
-    {4, 16, 0},  {4, 32, 0},  {8, 16, 0},  {8, 32, 0},  {16, 16, 0}, {2, 64, 0},
-    {4, 32, 1},  {8, 16, 1},  {8, 32, 1},  {8, 64, 1},  {16, 32, 1},
-    {4, 48, 2},  {8, 24, 2},  {8, 48, 2},  {16, 24, 2}, {16, 48, 2}};
+    {64, 4, 16, 0},  {64, 4, 32, 0},  {64, 8, 16, 0},  {64, 8, 32, 0},  {64, 16, 16, 0}, {64, 2, 64, 0},
+    {64, 4, 32, 1},  {64, 8, 16, 1},  {64, 8, 32, 1},  {64, 8, 64, 1},  {64, 16, 32, 1},
+    {64, 8, 24, 2},  {64, 8, 48, 2},
+    {256, 2, 24, 3}, {256, 4, 24, 3}, {256, 2, 48, 3}, {256, 4, 48, 3}, {128, 4, 24, 3},
+    {64, 4, 24, 3},  {256, 1, 24, 3}, {128, 2, 48, 3}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kDefaultShape = 0;
 
@@ -409,7 +593,9 @@ template <int FORM, int S>
 constexpr DivkFn divk_fn()
 {
   constexpr TileShape sh = kShapes[S];
+  static_assert(sh.variant == 3 || sh.tx == kTx, "only the column schedule spans several waves in x");
   if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
+  else if constexpr (sh.variant == 3) return divkgrad_column_kernel<FORM, sh.tx / kTx, sh.ty, sh.zc>;
   else return divkgrad_staged_kernel<FORM, sh.ty, sh.zc, sh.variant>;
 }
 
@@ -418,6 +604,7 @@ constexpr LaplFn lapl_fn()
 {
   constexpr TileShape sh = kShapes[S];
   if constexpr (sh.variant == 0) return lapl_euler_kernel<sh.ty, sh.zc>;
+  else if constexpr (sh.variant == 3) return lapl_euler_column_kernel<sh.tx / kTx, sh.ty, sh.zc>;
   else return lapl_euler_staged_kernel<sh.ty, sh.zc, sh.variant>;
 }
 
@@ -446,22 +633,26 @@ LaplFn lapl_table(int s) { return lapl_pick(s, std::make_index_sequence<kNumShap
 std::atomic<int> g_divk_shape{kDefaultShape};
 std::atomic<int> g_lapl_shape{kDefaultShape};
 
-int find_shape(int ty, int zc, int variant)
+int find_shape(int tx, int ty, int zc, int variant)
 {
   for (int s = 0; s < kNumShapes; ++s)
-    if (kShapes[s].ty == ty && kShapes[s].zc == zc && kShapes[s].variant == variant) return s;
+    if (kShapes[s].tx == tx && kShapes[s].ty == ty && kShapes[s].zc == zc && kShapes[s].variant == variant)
+      return s;
   return -1;
 }
+
+// threads of one workgroup of a tile shape
+int tile_threads(const TileShape& sh) { return sh.variant == 3 ? sh.tx : sh.tx * sh.ty; }
 
 int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_field_t* k,
                       const nebo_field_t* phi, const nebo_axis_coefs_t& c, double dt,
                       const nebo_limits_t& lim, void* stream)
 {
   const TileShape sh = kShapes[shape];
-  Tiles T = make_tiles(lim, sh.ty, sh.zc);
+  Tiles T = make_tiles(lim, sh.tx, sh.ty, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
-  hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(tile_threads(sh)), 0, (hipStream_t)stream,
                      frame_origin(r), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
   return check_hip(hipGetLastError(), "Nebo Assignment");
 }
@@ -470,10 +661,10 @@ int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* ph
                       const nebo_axis_coefs_t& c, double sc, const nebo_limits_t& lim, void* stream)
 {
   const TileShape sh = kShapes[shape];
-  Tiles T = make_tiles(lim, sh.ty, sh.zc);
+  Tiles T = make_tiles(lim, sh.tx, sh.ty, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
-  hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(kTx * sh.ty), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(tile_threads(sh)), 0, (hipStream_t)stream,
                      frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
   return check_hip(hipGetLastError(), "Nebo Assignment");
 }
@@ -642,13 +833,13 @@ extern "C" int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
   return launch_lapl_euler(out, phi, *c, s, L, stream);
 }
 
-extern "C" int nebo_stencil_set_tile(int kernel, int32_t ty, int32_t zchunk, int32_t variant)
+extern "C" int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t zchunk, int32_t variant)
 {
   clear_error();
-  const int s = find_shape(ty, zchunk, variant);
+  const int s = find_shape(tx, ty, zchunk, variant);
   if (s < 0 || (kernel != 0 && kernel != 1)) {
     std::ostringstream m;
-    m << "no stencil kernel instantiated for kernel " << kernel << ", tile 64x" << ty << "x" << zchunk
+    m << "no stencil kernel instantiated for kernel " << kernel << ", tile " << tx << "x" << ty << "x" << zchunk
       << ", variant " << variant;
     return set_error(NEBO_ERR_INVALID, "Nebo Stencil Tile", m.str());
   }
@@ -656,11 +847,12 @@ extern "C" int nebo_stencil_set_tile(int kernel, int32_t ty, int32_t zchunk, int
   return NEBO_OK;
 }
 
-extern "C" int nebo_stencil_get_tile(int kernel, int32_t* ty, int32_t* zchunk, int32_t* variant)
+extern "C" int nebo_stencil_get_tile(int kernel, int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant)
 {
   clear_error();
   if (kernel != 0 && kernel != 1) return set_error(NEBO_ERR_INVALID, "Nebo Stencil Tile", "unknown kernel");
   const TileShape sh = kShapes[(kernel == 0 ? g_divk_shape : g_lapl_shape).load()];
+  if (tx) *tx = sh.tx;
   if (ty) *ty = sh.ty;
   if (zchunk) *zchunk = sh.zc;
   if (variant) *variant = sh.variant;
@@ -669,8 +861,8 @@ extern "C" int nebo_stencil_get_tile(int kernel, int32_t* ty, int32_t* zchunk, i
 
 extern "C" int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const nebo_field_t* k,
                                      const nebo_field_t* phi, const nebo_axis_coefs_t* c,
-                                     void* stream, int32_t* ty, int32_t* zchunk, int32_t* variant,
-                                     float* times_ms)
+                                     void* stream, int32_t* tx, int32_t* ty, int32_t* zchunk,
+                                     int32_t* variant, float* times_ms)
 {
   static const char* ctx = "Nebo Stencil Tile";
   clear_error();
@@ -718,17 +910,19 @@ extern "C" int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const ne
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   slot.store(best);
+  if (tx) *tx = kShapes[best].tx;
   if (ty) *ty = kShapes[best].ty;
   if (zchunk) *zchunk = kShapes[best].zc;
   if (variant) *variant = kShapes[best].variant;
   return NEBO_OK;
 }
 
-extern "C" int nebo_stencil_tiles(int32_t* n, int32_t* ty, int32_t* zchunk, int32_t* variant)
+extern "C" int nebo_stencil_tiles(int32_t* n, int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant)
 {
   clear_error();
   if (n) *n = kNumShapes;
   for (int s = 0; s < kNumShapes; ++s) {
+    if (tx) tx[s] = kShapes[s].tx;
     if (ty) ty[s] = kShapes[s].ty;
     if (zchunk) zchunk[s] = kShapes[s].zc;
     if (variant) variant[s] = kShapes[s].variant;

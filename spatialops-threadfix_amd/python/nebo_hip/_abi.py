@@ -140,11 +140,11 @@ _SIGS = {
     "nebo_fill_ghosts_periodic": (C.c_int, [P(FieldT), _vp]),
     "nebo_divkgrad": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), C.c_int, _vp]),
     "nebo_lapl_euler": (C.c_int, [P(FieldT), P(FieldT), P(AxisCoefsT), C.c_double, _vp]),
-    "nebo_stencil_tiles": (C.c_int, [P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
-    "nebo_stencil_set_tile": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int32]),
-    "nebo_stencil_get_tile": (C.c_int, [C.c_int, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+    "nebo_stencil_tiles": (C.c_int, [P(C.c_int32)] * 5),
+    "nebo_stencil_set_tile": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "nebo_stencil_get_tile": (C.c_int, [C.c_int] + [P(C.c_int32)] * 4),
     "nebo_stencil_autotune": (C.c_int, [C.c_int, P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), _vp,
-                                        P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_float)]),
+                                        P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_float)]),
     "nebo_batch_create": (C.c_int, [P(PatchT), C.c_int32, P(AxisCoefsT), C.c_double, P(C.c_int32), P(_vp)]),
     "nebo_batch_destroy": (C.c_int, [_vp]),
     "nebo_batch_step": (C.c_int, [_vp, C.c_int, _vp]),

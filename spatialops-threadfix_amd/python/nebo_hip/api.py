@@ -112,38 +112,38 @@ KERNEL_DIVKGRAD, KERNEL_LAPL = 0, 1
 
 
 def stencil_tiles():
-    """(ty, zchunk, variant) tiles instantiated for the hand-written stencil
+    """(tx, ty, zchunk, variant) tiles instantiated for the hand-written stencil
     kernels; variant 0 = direct loads, 1/2 = LDS-staged rows with 1/2 planes of
-    prefetch."""
+    prefetch, 3 = column schedule (tx/64 waves in x, ty rows per lane)."""
     n = C.c_int32()
-    _abi.call("nebo_stencil_tiles", C.byref(n), None, None, None)
-    ty, zc, var = ((C.c_int32 * n.value)() for _ in range(3))
-    _abi.call("nebo_stencil_tiles", C.byref(n), ty, zc, var)
-    return list(zip(ty, zc, var))
+    _abi.call("nebo_stencil_tiles", C.byref(n), None, None, None, None)
+    tx, ty, zc, var = ((C.c_int32 * n.value)() for _ in range(4))
+    _abi.call("nebo_stencil_tiles", C.byref(n), tx, ty, zc, var)
+    return list(zip(tx, ty, zc, var))
 
 
-def stencil_set_tile(kernel, ty, zchunk, variant=0):
-    _abi.call("nebo_stencil_set_tile", int(kernel), int(ty), int(zchunk), int(variant))
+def stencil_set_tile(kernel, tx, ty, zchunk, variant=0):
+    _abi.call("nebo_stencil_set_tile", int(kernel), int(tx), int(ty), int(zchunk), int(variant))
 
 
 def stencil_get_tile(kernel=0):
-    ty, zc, var = C.c_int32(), C.c_int32(), C.c_int32()
-    _abi.call("nebo_stencil_get_tile", int(kernel), C.byref(ty), C.byref(zc), C.byref(var))
-    return ty.value, zc.value, var.value
+    v = [C.c_int32() for _ in range(4)]
+    _abi.call("nebo_stencil_get_tile", int(kernel), *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def stencil_autotune(r: Field, k, phi: Field, coefs, stream=None, kernel=KERNEL_DIVKGRAD):
     """Time every tile of one stencil kernel (0: div(k grad phi) into r; 1:
     phi + s*lapl(phi) into r, k unused) and keep the fastest.  Returns
-    ((ty, zchunk, variant), {tile: ms})."""
+    ((tx, ty, zchunk, variant), {tile: ms})."""
     tiles = stencil_tiles()
     times = (C.c_float * len(tiles))()
-    ty, zc, var = C.c_int32(), C.c_int32(), C.c_int32()
+    v = [C.c_int32() for _ in range(4)]
     dr, dp = r.desc("gpu"), phi.desc("gpu")
     dk = C.byref(k.desc("gpu")) if k is not None else None
     _abi.call("nebo_stencil_autotune", int(kernel), C.byref(dr), dk, C.byref(dp), C.byref(coefs),
-              _stream(stream), C.byref(ty), C.byref(zc), C.byref(var), times)
-    return (ty.value, zc.value, var.value), dict(zip(tiles, list(times)))
+              _stream(stream), *[C.byref(x) for x in v], times)
+    return tuple(x.value for x in v), dict(zip(tiles, list(times)))
 
 
 def lower_stats(rhs, where="host"):
