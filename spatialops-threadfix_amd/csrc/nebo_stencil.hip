@@ -283,25 +283,50 @@ __device__ __forceinline__ double lane_value(double v, int j)
 // Loads of one column-schedule lane within a plane, clamped into [lo-1, hi] on
 // every axis: rows y0-1 .. y0+RY of its column, and one edge load: lanes 0..RY-1
 // take the left neighbours (x = xw0-1) of rows 0..RY-1, lanes 32..32+RY-1 the
-// right neighbours (x = xw0+64), other lanes re-read their row 0.
+// right neighbours (x = xw0+64), other lanes re-read their row 0.  Offsets are in
+// bytes from the plane's base (the frame origin of the plane, minus X + 1), so
+// they are non-negative 32-bit values and every load is base (uniform) + offset;
+// for an active lane the offset of row j+1 is also the store offset of row j.
 template <int RY>
 struct ColumnGeom {
-  int row[RY + 2];
-  int edge;
+  uint32_t row[RY + 2];
+  uint32_t edge;
+  uint32_t inner;   // lanes 0..RY-1: (xw0, row); lanes 32..32+RY-1: (xw0+63, row); else row 0
 };
 
 template <int RY>
 __device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int xw0, int y0, int lane)
 {
   ColumnGeom<RY> g;
+  auto off = [X](int x, int y) { return (uint32_t)((x + 1) + X * (y + 1)) * 8u; };
   const int xc = min(xw0 + lane, T.hi[0]);
 #pragma unroll
-  for (int r = 0; r < RY + 2; ++r) g.row[r] = xc + X * min(y0 - 1 + r, T.hi[1]);
+  for (int r = 0; r < RY + 2; ++r) g.row[r] = off(xc, min(y0 - 1 + r, T.hi[1]));
   // (a wave wholly beyond hi, in a partial last tile, has xw0 - 1 > hi: clamp too)
-  if (lane < RY) g.edge = min(xw0 - 1, T.hi[0]) + X * min(y0 + lane, T.hi[1]);
-  else if (lane >= 32 && lane < 32 + RY) g.edge = min(xw0 + kTx, T.hi[0]) + X * min(y0 + lane - 32, T.hi[1]);
+  if (lane < RY) g.edge = off(min(xw0 - 1, T.hi[0]), min(y0 + lane, T.hi[1]));
+  else if (lane >= 32 && lane < 32 + RY) g.edge = off(min(xw0 + kTx, T.hi[0]), min(y0 + lane - 32, T.hi[1]));
   else g.edge = g.row[1];
+  if (lane < RY) g.inner = off(min(xw0, T.hi[0]), min(y0 + lane, T.hi[1]));
+  else if (lane >= 32 && lane < 32 + RY) g.inner = off(min(xw0 + kTx - 1, T.hi[0]), min(y0 + lane - 32, T.hi[1]));
+  else g.inner = g.row[1];
   return g;
+}
+
+// Plane z's base: frame origin + XY*z - X - 1 (uniform across the wave).
+template <typename T>
+__device__ __forceinline__ T* plane_base(T* origin, int X, int XY, int z)
+{
+  return origin + ((long)XY * z - X - 1);
+}
+
+__device__ __forceinline__ double load_at(const double* base, uint32_t off)
+{
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+}
+
+__device__ __forceinline__ void store_nt_at(double* base, uint32_t off, double v)
+{
+  __builtin_nontemporal_store(v, reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off));
 }
 
 // Column schedule (variant 3): W waves side by side in x (a 64*W-wide strip, so
@@ -313,7 +338,14 @@ __device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int
 // receives plane z+3, so loads are issued two planes ahead.  One period of three
 // steps is unrolled (slots are renamed, never copied) and the chunk is padded to
 // whole periods; padded steps load clamped planes and store nothing.
-template <int FORM, int W, int RY, int ZC>
+//
+// XR (x-face reuse): each lane computes only its high x face; its low face is the
+// lower lane's high face (DPP).  The faces at the wave's two ends are computed for
+// all RY rows at once, once per plane: lane j (< RY) takes the left face of row j
+// from its edge load and an inner load (x = xw0), lane 32+j the right face from
+// its edge load and an inner load (x = xw0+63); each row reads its two end faces
+// back with readlane.  A face has the same value wherever it is computed.
+template <int FORM, int W, int RY, int ZC, bool XR>
 __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
     Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
@@ -334,17 +366,22 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
   const ColumnGeom<RY> g = column_geom<RY>(T, X, xw0, y0, lane);
   const DivkCoefs c = divk_coefs(cc);
 
-  struct Col { double p[RY + 2], k[RY + 2], pe, ke; };
+  struct Col { double p[RY + 2], k[RY + 2], pe, ke, pi, ki; };
   auto load = [&](int z) {
     Col v;
-    const int b = XY * z;
+    const double* pb = plane_base(p, X, XY, z);
+    const double* kb = plane_base(k, X, XY, z);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) {
-      v.p[q] = p[b + g.row[q]];
-      v.k[q] = k[b + g.row[q]];
+      v.p[q] = load_at(pb, g.row[q]);
+      v.k[q] = load_at(kb, g.row[q]);
     }
-    v.pe = p[b + g.edge];
-    v.ke = k[b + g.edge];
+    v.pe = load_at(pb, g.edge);
+    v.ke = load_at(kb, g.edge);
+    if constexpr (XR) {
+      v.pi = load_at(pb, g.inner);
+      v.ki = load_at(kb, g.inner);
+    }
     return v;
   };
 
@@ -353,10 +390,13 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
   S[1] = load(min(z0 + 1, z1));
   S[2] = load(min(z0 + 2, z1));
   double fz[RY];
+  {
+    const double* pb = plane_base(p, X, XY, z0 - 1);
+    const double* kb = plane_base(k, X, XY, z0 - 1);
 #pragma unroll
-  for (int j = 0; j < RY; ++j) {
-    const int b = XY * (z0 - 1) + g.row[j + 1];
-    fz[j] = face(k[b], S[0].k[j + 1], p[b], S[0].p[j + 1], c.iz0, c.iz1, c.gz0, c.gz1);
+    for (int j = 0; j < RY; ++j)
+      fz[j] = face(load_at(kb, g.row[j + 1]), S[0].k[j + 1], load_at(pb, g.row[j + 1]), S[0].p[j + 1], c.iz0,
+                   c.iz1, c.gz0, c.gz1);
   }
   const int periods = (z1 - z0 + 2) / 3;
   for (int q = 0; q < periods; ++q) {
@@ -365,26 +405,44 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
       const int z = z0 + 3 * q + jj;
       const Col& C0 = S[jj];              // plane z
       const Col& C1 = S[(jj + 1) % 3];    // plane z+1
+      // the y face between rows j-1 and j is computed once, as row j-1's high face
+      double fy_lo = face(C0.k[0], C0.k[1], C0.p[0], C0.p[1], c.iy0, c.iy1, c.gy0, c.gy1);
+      // XR: the wave's end faces of every row (lane j: left face of row j, lane 32+j: right face)
+      double fend = 0.0;
+      if constexpr (XR) {
+        const bool left = lane < 32;
+        fend = face(left ? C0.ke : C0.ki, left ? C0.ki : C0.ke, left ? C0.pe : C0.pi, left ? C0.pi : C0.pe,
+                    c.ix0, c.ix1, c.gx0, c.gx1);
+      }
 #pragma unroll
       for (int j = 0; j < RY; ++j) {
         const double pc = C0.p[j + 1], kc = C0.k[j + 1];
-        const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
-        const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
-        const double pel = lane_value(C0.pe, j), per = lane_value(C0.pe, 32 + j);
-        const double kel = lane_value(C0.ke, j), ker = lane_value(C0.ke, 32 + j);
-        const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
-        const double kxm = lane == 0 ? kel : kl, kxp = lane == kTx - 1 ? ker : ku;
-        const double fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
-        const double fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
-        const double fy_lo = face(C0.k[j], kc, C0.p[j], pc, c.iy0, c.iy1, c.gy0, c.gy1);
+        double fx_lo, fx_hi;
+        if constexpr (XR) {
+          const double pu = from_upper_lane(pc), ku = from_upper_lane(kc);
+          const double own = face(kc, ku, pc, pu, c.ix0, c.ix1, c.gx0, c.gx1);
+          fx_hi = lane == kTx - 1 ? lane_value(fend, 32 + j) : own;
+          const double below = from_lower_lane(fx_hi);
+          fx_lo = lane == 0 ? lane_value(fend, j) : below;
+        } else {
+          const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
+          const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
+          const double pel = lane_value(C0.pe, j), per = lane_value(C0.pe, 32 + j);
+          const double kel = lane_value(C0.ke, j), ker = lane_value(C0.ke, 32 + j);
+          const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
+          const double kxm = lane == 0 ? kel : kl, kxp = lane == kTx - 1 ? ker : ku;
+          fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+          fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+        }
         const double fy_hi = face(kc, C0.k[j + 2], pc, C0.p[j + 2], c.iy0, c.iy1, c.gy0, c.gy1);
         const double fz_hi = face(kc, C1.k[j + 1], pc, C1.p[j + 1], c.iz0, c.iz1, c.gz0, c.gz1);
         const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
         const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
         const double Dz = fz[j] * c.dz0 + fz_hi * c.dz1;
         fz[j] = fz_hi;
+        fy_lo = fy_hi;
         const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
-        if (xin && y0 + j < T.hi[1] && z < z1) __builtin_nontemporal_store(v, &r[x + X * (y0 + j) + XY * z]);
+        if (xin && y0 + j < T.hi[1] && z < z1) store_nt_at(plane_base(r, X, XY, z), g.row[j + 1], v);
       }
       S[jj] = load(min(z + 3, z1));       // plane z is consumed: its slot takes z+3
     }
@@ -535,10 +593,10 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
   struct Col { double p[RY + 2], pe; };
   auto load = [&](int z) {
     Col v;
-    const int b = XY * z;
+    const double* pb = plane_base(p, X, XY, z);
 #pragma unroll
-    for (int q = 0; q < RY + 2; ++q) v.p[q] = p[b + g.row[q]];
-    v.pe = p[b + g.edge];
+    for (int q = 0; q < RY + 2; ++q) v.p[q] = load_at(pb, g.row[q]);
+    v.pe = load_at(pb, g.edge);
     return v;
   };
 
@@ -548,7 +606,8 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
   S[2] = load(min(z0 + 2, z1));
   double gz[RY];
 #pragma unroll
-  for (int j = 0; j < RY; ++j) gz[j] = p[XY * (z0 - 1) + g.row[j + 1]] * c.gz0 + S[0].p[j + 1] * c.gz1;
+  for (int j = 0; j < RY; ++j)
+    gz[j] = load_at(plane_base(p, X, XY, z0 - 1), g.row[j + 1]) * c.gz0 + S[0].p[j + 1] * c.gz1;
   const int periods = (z1 - z0 + 2) / 3;
   for (int q = 0; q < periods; ++q) {
 #pragma unroll
@@ -565,7 +624,7 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
         const double gz_hi = pc * c.gz0 + C1.p[j + 1] * c.gz1;
         const double v = lapl_cell(pc, pxm, pxp, C0.p[j], C0.p[j + 2], gz[j], gz_hi, c, s);
         gz[j] = gz_hi;
-        if (xin && y0 + j < T.hi[1] && z < z1) __builtin_nontemporal_store(v, &out[x + X * (y0 + j) + XY * z]);
+        if (xin && y0 + j < T.hi[1] && z < z1) store_nt_at(plane_base(out, X, XY, z), g.row[j + 1], v);
       }
       S[jj] = load(min(z + 3, z1));
     }
@@ -575,14 +634,18 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
 // ---- tile table -------------------------------------------------------------
 
 // variant 0: direct loads; 1, 2: LDS-staged with 1 or 2 planes of prefetch;
-// 3: column schedule (ty = rows per lane, tx = 64 * waves side by side)
+// 3: column schedule (ty = rows per lane, tx = 64 * waves side by side);
+// 4: column schedule with x-face reuse (the Laplacian kernel has no x-face
+//    reuse and runs its column schedule for both 3 and 4)
 struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
-    {64, 4, 16, 0},  {64, 4, 32, 0},  {64, 8, 16, 0},  {64, 8, 32, 0},  {64, 16, 16, 0}, {64, 2, 64, 0},
-    {64, 4, 32, 1},  {64, 8, 16, 1},  {64, 8, 32, 1},  {64, 8, 64, 1},  {64, 16, 32, 1},
-    {64, 8, 24, 2},  {64, 8, 48, 2},
-    {256, 2, 24, 3}, {256, 4, 24, 3}, {256, 2, 48, 3}, {256, 4, 48, 3}, {128, 4, 24, 3},
-    {64, 4, 24, 3},  {256, 1, 24, 3}, {128, 2, 48, 3}};
+    {64, 4, 16, 0},  {64, 8, 16, 0},
+    {64, 8, 16, 1},  {64, 8, 64, 1},  {64, 16, 32, 1},
+    {64, 8, 24, 2},
+    {256, 2, 24, 3}, {256, 4, 24, 3}, {128, 4, 24, 3}, {64, 4, 24, 3},  {64, 2, 24, 3},
+    {64, 3, 24, 3},  {64, 4, 48, 3},  {64, 4, 12, 3},
+    {256, 4, 24, 4}, {128, 4, 24, 4}, {64, 4, 24, 4},  {64, 2, 24, 4},  {64, 3, 24, 4},
+    {64, 4, 48, 4},  {64, 4, 12, 4}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kDefaultShape = 0;
 
@@ -593,9 +656,10 @@ template <int FORM, int S>
 constexpr DivkFn divk_fn()
 {
   constexpr TileShape sh = kShapes[S];
-  static_assert(sh.variant == 3 || sh.tx == kTx, "only the column schedule spans several waves in x");
+  static_assert(sh.variant >= 3 || sh.tx == kTx, "only the column schedule spans several waves in x");
   if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
-  else if constexpr (sh.variant == 3) return divkgrad_column_kernel<FORM, sh.tx / kTx, sh.ty, sh.zc>;
+  else if constexpr (sh.variant >= 3)
+    return divkgrad_column_kernel<FORM, sh.tx / kTx, sh.ty, sh.zc, sh.variant == 4>;
   else return divkgrad_staged_kernel<FORM, sh.ty, sh.zc, sh.variant>;
 }
 
@@ -604,7 +668,7 @@ constexpr LaplFn lapl_fn()
 {
   constexpr TileShape sh = kShapes[S];
   if constexpr (sh.variant == 0) return lapl_euler_kernel<sh.ty, sh.zc>;
-  else if constexpr (sh.variant == 3) return lapl_euler_column_kernel<sh.tx / kTx, sh.ty, sh.zc>;
+  else if constexpr (sh.variant >= 3) return lapl_euler_column_kernel<sh.tx / kTx, sh.ty, sh.zc>;
   else return lapl_euler_staged_kernel<sh.ty, sh.zc, sh.variant>;
 }
 
@@ -642,7 +706,7 @@ int find_shape(int tx, int ty, int zc, int variant)
 }
 
 // threads of one workgroup of a tile shape
-int tile_threads(const TileShape& sh) { return sh.variant == 3 ? sh.tx : sh.tx * sh.ty; }
+int tile_threads(const TileShape& sh) { return sh.variant >= 3 ? sh.tx : sh.tx * sh.ty; }
 
 int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_field_t* k,
                       const nebo_field_t* phi, const nebo_axis_coefs_t& c, double dt,
