@@ -45,7 +45,8 @@ def parse():
 def _tile_name(t):
     """tx x ty x zchunk with the schedule: direct loads, LDS-staged + prefetch depth,
     or the column schedule (ty rows per lane)."""
-    sched = {0: "", 1: "/staged-pf1", 2: "/staged-pf2", 3: "/column", 4: "/column-xreuse"}[t[3]]
+    sched = {0: "", 1: "/staged-pf1", 2: "/staged-pf2", 3: "/column", 4: "/column-xreuse",
+             5: "/stacked-ry4", 6: "/stacked-ry2"}[t[3]]
     return f"{t[0]}x{t[1]}x{t[2]}{sched}"
 
 

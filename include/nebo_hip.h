@@ -277,7 +277,10 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  * ahead; variant 3 (column schedule) puts tx/64 waves side by side in x, each
  * lane computing ty rows of one column with no LDS and no barrier, loads issued
  * two planes ahead; variant 4 is variant 3 computing each x face once (the
- * Laplacian kernel runs variant 3 for it).  kernel 0 = the var-coefficient diffusion kernels
+ * Laplacian kernel runs variant 3 for it); variants 5 and 6 stack ty/4 resp.
+ * ty/2 such waves in y in one workgroup, which swap their boundary rows through
+ * LDS (the Laplacian kernel runs variant 3 with 4 resp. 2 rows per lane for
+ * them).  kernel 0 = the var-coefficient diffusion kernels
  * (nebo_divkgrad, nebo_diffusion_step), 1 = Laplacian+Euler.  The tile changes
  * only the schedule, never the results.  The reference has no counterpart (its
  * CUDA launch shape is fixed at 16x16, NeboLhs.h:413-427). */

@@ -77,22 +77,17 @@ __device__ __forceinline__ double face(double klo, double khi, double plo, doubl
   return ki * gr;
 }
 
-// Lane l receives lane l-1's value (DPP wave_shr:1); lane 0 keeps its own.
+// Lane l receives lane l-1's value (DPP wave_shr:1); lane 0 receives 0 (every
+// caller replaces lane 0's value).
 __device__ __forceinline__ double from_lower_lane(double v)
 {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
 }
 
-// Lane l receives lane l+1's value (DPP wave_shl:1); lane 63 keeps its own.
+// Lane l receives lane l+1's value (DPP wave_shl:1); lane 63 receives 0.
 __device__ __forceinline__ double from_upper_lane(double v)
 {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
 }
 
 // Where one thread of a staged tile loads within a plane.  Every coordinate is
@@ -319,14 +314,29 @@ __device__ __forceinline__ T* plane_base(T* origin, int X, int XY, int z)
   return origin + ((long)XY * z - X - 1);
 }
 
-__device__ __forceinline__ double load_at(const double* base, uint32_t off)
+// Raw buffer resource over one plane (base from plane_base): loads and stores
+// take the lane's 32-bit byte offset, so no per-lane address arithmetic is left
+// in the loop.  The range, (XY + 2X + 2) doubles, covers every offset the column
+// geometry forms (the hardware returns 0 for a load beyond it instead of
+// faulting; the launcher keeps it below 2^31 bytes).  Word 3 is the gfx9 raw
+// buffer format.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+using U2 = __attribute__((__vector_size__(2 * sizeof(unsigned int)))) unsigned int;
+
+__device__ __forceinline__ Rsrc plane_rsrc(const double* base, int X, int XY)
 {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, (XY + 2 * X + 2) * 8, 0x00020000);
 }
 
-__device__ __forceinline__ void store_nt_at(double* base, uint32_t off, double v)
+__device__ __forceinline__ double load_at(Rsrc rs, uint32_t off)
 {
-  __builtin_nontemporal_store(v, reinterpret_cast<double*>(reinterpret_cast<char*>(base) + off));
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+}
+
+// nontemporal store (cache-policy bit 1: nt)
+__device__ __forceinline__ void store_nt_at(Rsrc rs, uint32_t off, double v)
+{
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, 2);
 }
 
 // Column schedule (variant 3): W waves side by side in x (a 64*W-wide strip, so
@@ -336,8 +346,11 @@ __device__ __forceinline__ void store_nt_at(double* base, uint32_t off, double v
 // adjacent lanes (DPP) and at the wave's ends from the edge load (readlane).
 // Three register slots hold planes z, z+1 and z+2; once plane z is done its slot
 // receives plane z+3, so loads are issued two planes ahead.  One period of three
-// steps is unrolled (slots are renamed, never copied) and the chunk is padded to
-// whole periods; padded steps load clamped planes and store nothing.
+// steps is unrolled (slots are renamed, never copied); the one or two steps left
+// after the whole periods run once after the loop, so a chunk of any length does
+// no padded work.  The chunk length sets the number of waves (for a 256^3 box at
+// 4 rows per lane, a 16-plane chunk gives 4096 waves, the whole GPU's residency
+// at 4 waves per SIMD), which hides the memory latency.
 //
 // XR (x-face reuse): each lane computes only its high x face; its low face is the
 // lower lane's high face (DPP).  The faces at the wave's two ends are computed for
@@ -345,8 +358,14 @@ __device__ __forceinline__ void store_nt_at(double* base, uint32_t off, double v
 // from its edge load and an inner load (x = xw0), lane 32+j the right face from
 // its edge load and an inner load (x = xw0+63); each row reads its two end faces
 // back with readlane.  A face has the same value wherever it is computed.
-template <int FORM, int W, int RY, int ZC, bool XR>
-__global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
+//
+// NW waves stacked in y per workgroup: a wave's halo rows are its y neighbours'
+// own rows, loaded at the same time on the same CU, so they hit L1; a halo row
+// that only another CU loaded comes back through L2 at about the cost of an HBM
+// row.  (Measured at 256^3 with 4 rows per lane: the halo rows cost ~13 us of an
+// ~84 us launch with one wave per workgroup, ~3 us with four stacked.)
+template <int FORM, int W, int NW, int RY, int ZC, bool XR>
+__global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
     Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
 {
@@ -357,9 +376,10 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
   const int ty = rest % T.ny;
   const int tz = rest / T.ny;
   const int lane = threadIdx.x & (kTx - 1);
-  const int xw0 = T.lo[0] + tx * (kTx * W) + (threadIdx.x / kTx) * kTx;
+  const int wave = threadIdx.x / kTx;
+  const int xw0 = T.lo[0] + tx * (kTx * W) + (wave % W) * kTx;
   const int x = xw0 + lane;
-  const int y0 = T.lo[1] + ty * RY;
+  const int y0 = T.lo[1] + ty * (RY * NW) + (wave / W) * RY;
   const int z0 = T.lo[2] + tz * ZC;
   const int z1 = min(z0 + ZC, T.hi[2]);
   const bool xin = x < T.hi[0];
@@ -369,8 +389,8 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
   struct Col { double p[RY + 2], k[RY + 2], pe, ke, pi, ki; };
   auto load = [&](int z) {
     Col v;
-    const double* pb = plane_base(p, X, XY, z);
-    const double* kb = plane_base(k, X, XY, z);
+    const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z), X, XY);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) {
       v.p[q] = load_at(pb, g.row[q]);
@@ -391,18 +411,17 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
   S[2] = load(min(z0 + 2, z1));
   double fz[RY];
   {
-    const double* pb = plane_base(p, X, XY, z0 - 1);
-    const double* kb = plane_base(k, X, XY, z0 - 1);
+    const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
 #pragma unroll
     for (int j = 0; j < RY; ++j)
       fz[j] = face(load_at(kb, g.row[j + 1]), S[0].k[j + 1], load_at(pb, g.row[j + 1]), S[0].p[j + 1], c.iz0,
                    c.iz1, c.gz0, c.gz1);
   }
-  const int periods = (z1 - z0 + 2) / 3;
-  for (int q = 0; q < periods; ++q) {
-#pragma unroll
-    for (int jj = 0; jj < 3; ++jj) {
-      const int z = z0 + 3 * q + jj;
+  // one step: plane z, whose slot is jj (a compile-time index)
+  auto step = [&](auto JJ, const int z) {
+    constexpr int jj = decltype(JJ)::value;
+    {
       const Col& C0 = S[jj];              // plane z
       const Col& C1 = S[(jj + 1) % 3];    // plane z+1
       // the y face between rows j-1 and j is computed once, as row j-1's high face
@@ -442,11 +461,21 @@ __global__ __launch_bounds__(kTx * W) void divkgrad_column_kernel(
         fz[j] = fz_hi;
         fy_lo = fy_hi;
         const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
-        if (xin && y0 + j < T.hi[1] && z < z1) store_nt_at(plane_base(r, X, XY, z), g.row[j + 1], v);
+        if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(r, X, XY, z), X, XY), g.row[j + 1], v);
       }
       S[jj] = load(min(z + 3, z1));       // plane z is consumed: its slot takes z+3
     }
+  };
+  // whole periods of three steps, then the one or two steps left over
+  const int full = (z1 - z0) / 3, rem = (z1 - z0) - 3 * full;
+  int z = z0;
+  for (int q = 0; q < full; ++q, z += 3) {
+    step(std::integral_constant<int, 0>{}, z);
+    step(std::integral_constant<int, 1>{}, z + 1);
+    step(std::integral_constant<int, 2>{}, z + 2);
   }
+  if (rem > 0) step(std::integral_constant<int, 0>{}, z);
+  if (rem > 1) step(std::integral_constant<int, 1>{}, z + 1);
 }
 
 // ---- phi + s*lapl(phi) -----------------------------------------------------------
@@ -569,8 +598,8 @@ __global__ __launch_bounds__(kTx * TY) void lapl_euler_staged_kernel(
 }
 
 // Column schedule of the Laplacian + Euler update (see divkgrad_column_kernel).
-template <int W, int RY, int ZC>
-__global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
+template <int W, int NW, int RY, int ZC>
+__global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
     double* __restrict__ out, const double* __restrict__ p, Tiles T, int X, int XY,
     nebo_axis_coefs_t cc, double s)
 {
@@ -581,9 +610,10 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
   const int ty = rest % T.ny;
   const int tz = rest / T.ny;
   const int lane = threadIdx.x & (kTx - 1);
-  const int xw0 = T.lo[0] + tx * (kTx * W) + (threadIdx.x / kTx) * kTx;
+  const int wave = threadIdx.x / kTx;
+  const int xw0 = T.lo[0] + tx * (kTx * W) + (wave % W) * kTx;
   const int x = xw0 + lane;
-  const int y0 = T.lo[1] + ty * RY;
+  const int y0 = T.lo[1] + ty * (RY * NW) + (wave / W) * RY;
   const int z0 = T.lo[2] + tz * ZC;
   const int z1 = min(z0 + ZC, T.hi[2]);
   const bool xin = x < T.hi[0];
@@ -593,7 +623,7 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
   struct Col { double p[RY + 2], pe; };
   auto load = [&](int z) {
     Col v;
-    const double* pb = plane_base(p, X, XY, z);
+    const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z), X, XY);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) v.p[q] = load_at(pb, g.row[q]);
     v.pe = load_at(pb, g.edge);
@@ -607,28 +637,33 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
   double gz[RY];
 #pragma unroll
   for (int j = 0; j < RY; ++j)
-    gz[j] = load_at(plane_base(p, X, XY, z0 - 1), g.row[j + 1]) * c.gz0 + S[0].p[j + 1] * c.gz1;
-  const int periods = (z1 - z0 + 2) / 3;
-  for (int q = 0; q < periods; ++q) {
+    gz[j] = load_at(plane_rsrc(plane_base(p, X, XY, z0 - 1), X, XY), g.row[j + 1]) * c.gz0 + S[0].p[j + 1] * c.gz1;
+  auto step = [&](auto JJ, const int z) {
+    constexpr int jj = decltype(JJ)::value;
+    const Col& C0 = S[jj];
+    const Col& C1 = S[(jj + 1) % 3];
 #pragma unroll
-    for (int jj = 0; jj < 3; ++jj) {
-      const int z = z0 + 3 * q + jj;
-      const Col& C0 = S[jj];
-      const Col& C1 = S[(jj + 1) % 3];
-#pragma unroll
-      for (int j = 0; j < RY; ++j) {
-        const double pc = C0.p[j + 1];
-        const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
-        const double pel = lane_value(C0.pe, j), per = lane_value(C0.pe, 32 + j);
-        const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
-        const double gz_hi = pc * c.gz0 + C1.p[j + 1] * c.gz1;
-        const double v = lapl_cell(pc, pxm, pxp, C0.p[j], C0.p[j + 2], gz[j], gz_hi, c, s);
-        gz[j] = gz_hi;
-        if (xin && y0 + j < T.hi[1] && z < z1) store_nt_at(plane_base(out, X, XY, z), g.row[j + 1], v);
-      }
-      S[jj] = load(min(z + 3, z1));
+    for (int j = 0; j < RY; ++j) {
+      const double pc = C0.p[j + 1];
+      const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
+      const double pel = lane_value(C0.pe, j), per = lane_value(C0.pe, 32 + j);
+      const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
+      const double gz_hi = pc * c.gz0 + C1.p[j + 1] * c.gz1;
+      const double v = lapl_cell(pc, pxm, pxp, C0.p[j], C0.p[j + 2], gz[j], gz_hi, c, s);
+      gz[j] = gz_hi;
+      if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(out, X, XY, z), X, XY), g.row[j + 1], v);
     }
+    S[jj] = load(min(z + 3, z1));
+  };
+  const int full = (z1 - z0) / 3, rem = (z1 - z0) - 3 * full;
+  int z = z0;
+  for (int q = 0; q < full; ++q, z += 3) {
+    step(std::integral_constant<int, 0>{}, z);
+    step(std::integral_constant<int, 1>{}, z + 1);
+    step(std::integral_constant<int, 2>{}, z + 2);
   }
+  if (rem > 0) step(std::integral_constant<int, 0>{}, z);
+  if (rem > 1) step(std::integral_constant<int, 1>{}, z + 1);
 }
 
 // ---- tile table -------------------------------------------------------------
@@ -636,18 +671,34 @@ __global__ __launch_bounds__(kTx * W) void lapl_euler_column_kernel(
 // variant 0: direct loads; 1, 2: LDS-staged with 1 or 2 planes of prefetch;
 // 3: column schedule (ty = rows per lane, tx = 64 * waves side by side);
 // 4: column schedule with x-face reuse (the Laplacian kernel has no x-face
-//    reuse and runs its column schedule for both 3 and 4)
+//    reuse and runs its column schedule for both 3 and 4);
+// 5, 6: column schedule with x-face reuse, 4 resp. 2 rows per lane, ty/4 resp.
+//    ty/2 waves stacked in y per workgroup (ty = the workgroup's rows).
 struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
     {64, 4, 16, 0},  {64, 8, 16, 0},
     {64, 8, 16, 1},  {64, 8, 64, 1},  {64, 16, 32, 1},
     {64, 8, 24, 2},
-    {256, 2, 24, 3}, {256, 4, 24, 3}, {128, 4, 24, 3}, {64, 4, 24, 3},  {64, 2, 24, 3},
-    {64, 3, 24, 3},  {64, 4, 48, 3},  {64, 4, 12, 3},
-    {256, 4, 24, 4}, {128, 4, 24, 4}, {64, 4, 24, 4},  {64, 2, 24, 4},  {64, 3, 24, 4},
-    {64, 4, 48, 4},  {64, 4, 12, 4}};
+    {64, 4, 16, 3},  {64, 4, 24, 3},  {64, 2, 32, 3},
+    {64, 4, 16, 4},  {64, 4, 24, 4},  {64, 4, 32, 4},  {64, 2, 32, 4},  {256, 4, 16, 4},
+    {64, 8, 32, 5},  {64, 12, 32, 5}, {64, 16, 16, 5}, {64, 16, 24, 5}, {64, 16, 32, 5},
+    {64, 16, 48, 5}, {64, 24, 32, 5}, {128, 16, 32, 5}, {64, 8, 32, 6}, {64, 16, 32, 6}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kDefaultShape = 0;
+
+// Rows per lane and waves stacked in y of a column tile.
+constexpr int rows_per_lane(const TileShape& sh)
+{
+  return sh.variant == 5 ? 4 : sh.variant == 6 ? 2 : sh.ty;
+}
+constexpr int stacked_waves(const TileShape& sh) { return sh.ty / rows_per_lane(sh); }
+
+// Launch geometry of one tile: rows per tile in y and threads per workgroup.
+struct Geometry { int tile_y, threads; };
+constexpr Geometry tile_geometry(const TileShape& sh)
+{
+  return sh.variant >= 3 ? Geometry{sh.ty, sh.tx * stacked_waves(sh)} : Geometry{sh.ty, sh.tx * sh.ty};
+}
 
 using DivkFn = void (*)(double*, const double*, const double*, Tiles, int, int, nebo_axis_coefs_t, double);
 using LaplFn = void (*)(double*, const double*, Tiles, int, int, nebo_axis_coefs_t, double);
@@ -657,9 +708,10 @@ constexpr DivkFn divk_fn()
 {
   constexpr TileShape sh = kShapes[S];
   static_assert(sh.variant >= 3 || sh.tx == kTx, "only the column schedule spans several waves in x");
+  static_assert(sh.ty % rows_per_lane(sh) == 0 && tile_geometry(sh).threads <= 1024, "whole waves, <= 1024 threads");
   if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
   else if constexpr (sh.variant >= 3)
-    return divkgrad_column_kernel<FORM, sh.tx / kTx, sh.ty, sh.zc, sh.variant == 4>;
+    return divkgrad_column_kernel<FORM, sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc, sh.variant != 3>;
   else return divkgrad_staged_kernel<FORM, sh.ty, sh.zc, sh.variant>;
 }
 
@@ -668,7 +720,8 @@ constexpr LaplFn lapl_fn()
 {
   constexpr TileShape sh = kShapes[S];
   if constexpr (sh.variant == 0) return lapl_euler_kernel<sh.ty, sh.zc>;
-  else if constexpr (sh.variant >= 3) return lapl_euler_column_kernel<sh.tx / kTx, sh.ty, sh.zc>;
+  else if constexpr (sh.variant >= 3)
+    return lapl_euler_column_kernel<sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc>;
   else return lapl_euler_staged_kernel<sh.ty, sh.zc, sh.variant>;
 }
 
@@ -705,18 +758,26 @@ int find_shape(int tx, int ty, int zc, int variant)
   return -1;
 }
 
-// threads of one workgroup of a tile shape
-int tile_threads(const TileShape& sh) { return sh.variant >= 3 ? sh.tx : sh.tx * sh.ty; }
+// The column schedule addresses a plane through a buffer range of
+// (XY + 2X + 2)*8 bytes, which must stay below 2^31; planes that large run the
+// default tile instead.
+int usable_shape(int shape, int X, int XY)
+{
+  if (kShapes[shape].variant >= 3 && ((long)XY + 2L * X + 2) * 8 >= (1L << 31)) return kDefaultShape;
+  return shape;
+}
 
 int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_field_t* k,
                       const nebo_field_t* phi, const nebo_axis_coefs_t& c, double dt,
                       const nebo_limits_t& lim, void* stream)
 {
+  shape = usable_shape(shape, r->glob[0], r->glob[0] * r->glob[1]);
   const TileShape sh = kShapes[shape];
-  Tiles T = make_tiles(lim, sh.tx, sh.ty, sh.zc);
+  const Geometry geo = tile_geometry(sh);
+  Tiles T = make_tiles(lim, sh.tx, geo.tile_y, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
-  hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(tile_threads(sh)), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(geo.threads), 0, (hipStream_t)stream,
                      frame_origin(r), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
   return check_hip(hipGetLastError(), "Nebo Assignment");
 }
@@ -724,11 +785,13 @@ int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_fie
 int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* phi,
                       const nebo_axis_coefs_t& c, double sc, const nebo_limits_t& lim, void* stream)
 {
+  shape = usable_shape(shape, out->glob[0], out->glob[0] * out->glob[1]);
   const TileShape sh = kShapes[shape];
-  Tiles T = make_tiles(lim, sh.tx, sh.ty, sh.zc);
+  const Geometry geo = tile_geometry(sh);
+  Tiles T = make_tiles(lim, sh.tx, geo.tile_y, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
-  hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(tile_threads(sh)), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(geo.threads), 0, (hipStream_t)stream,
                      frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
   return check_hip(hipGetLastError(), "Nebo Assignment");
 }
