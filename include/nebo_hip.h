@@ -98,6 +98,9 @@ enum nebo_opcode {
   NEBO_AND, NEBO_OR, NEBO_NOT,                             /* boolean operands     */
   NEBO_STENCIL,         /* a = argument node, idx = stencil table index          */
   NEBO_COND,            /* test a ? value b : otherwise c       NeboCond.h:408-412 */
+  NEBO_MASK_SHIFT,      /* a = NEBO_MASK node, idx = stencil table entry whose one
+                           point P is the shift: mask(x + P) (boolean)
+                           NeboMaskShift, NeboStencils.h:1048-1213             */
   NEBO_NUM_OPCODES
 };
 
@@ -114,7 +117,10 @@ typedef struct nebo_node {
 /* Stencil kinds (NeboStencils.h) */
 enum nebo_stencil_kind {
   NEBO_STENCIL_WEIGHTED = 0,  /* fold_p arg(x+p)*c_p, left fold in point order  :407-448 */
-  NEBO_STENCIL_SUM      = 1   /* fold_p arg(x+p), left fold in point order      :940-975 */
+  NEBO_STENCIL_SUM      = 1,  /* fold_p arg(x+p), left fold in point order      :940-975 */
+  NEBO_STENCIL_EDGELESS = 2   /* weighted, but the valid range does not shrink on the
+                                 side it reads (NeboEdgelessStencil :526-760); reads
+                                 outside the operand's allocated block give NaN      */
 };
 
 typedef struct nebo_stencil {
@@ -215,7 +221,12 @@ int nebo_lower_stats(const nebo_expr_t* rhs, int32_t* n_insns, int32_t* max_dept
  * checks every point against the LHS window and the RHS's valid ghost range,
  * then copies the list to the device).  The reference evaluates this on CPU only
  * (NeboLhs.h:164-283, NeboAssignment.h:246-313); this is its GPU counterpart,
- * with one evaluation per point. */
+ * with one evaluation per point.  The RHS may read the LHS through stencils
+ * (the boundary-condition inversion of NeboStencilBuilder.h:590-764 does) as long
+ * as no point reads a point that another point writes; such lists, and lists that
+ * repeat a point while the RHS reads the LHS, fail with NEBO_ERR_INVALID because
+ * their result would depend on the order of the writes.  Reads of edgeless
+ * stencils are checked point by point against each operand's allocated block. */
 int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points,
                        int32_t n_points, const nebo_expr_t* rhs, void* stream);
 

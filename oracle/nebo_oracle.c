@@ -19,7 +19,14 @@
  *                          + valid ghost minus); masks NeboMask.h:172-197 and
  *                          BitField.h:303-333 (window offset + total ghost minus)
  *   stencil fold ......... spatialops/NeboStencils.h:407-448 (left fold in point
- *                          definition order, value*coef); sum stencil :940-975
+ *                          definition order, value*coef); sum stencil :940-975;
+ *                          edgeless stencil :526-760 (same fold, ghosts by
+ *                          additive_point_to_ghost, GhostData.h:244-252)
+ *   mask shift ........... spatialops/NeboStencils.h:1048-1213 (mask(x + P),
+ *                          ghosts + additive_reductive_point_to_ghost(P))
+ *   boundary conditions .. spatialops/NeboStencilBuilder.h:504-526 (mask shift
+ *                          points) and :590-764 (cpu_apply: one pass over the
+ *                          mask points, phi(g - shift) = (gamma - sum) / coef)
  *   pointwise ops ........ spatialops/NeboOperators.h (SeqWalk eval bodies, e.g.
  *                          :181, :923, :1665, :2407, :4303, :6668, :6875, :13237,
  *                          :13992, :14710)
@@ -115,6 +122,21 @@ static oracle_ghost_t g_stencil_shift(const nebo_stencil_t* s)
   return g;
 }
 
+/* Edgeless stencils: additive_point_to_ghost(p) = (minus max(p,0), plus max(-p,0)),
+ * folded with min over the points (the reference compiles the one-point case). */
+static oracle_ghost_t g_edgeless_shift(const nebo_stencil_t* s)
+{
+  oracle_ghost_t g = g_uniform(0);
+  for (int i = 0; i < s->npts; ++i)
+    for (int d = 0; d < 3; ++d) {
+      int v = s->pts[i][d];
+      int mn = v > 0 ? v : 0, pl = v < 0 ? -v : 0;
+      if (i == 0 || mn < g.minus[d]) g.minus[d] = mn;
+      if (i == 0 || pl < g.plus[d]) g.plus[d] = pl;
+    }
+  return g;
+}
+
 int oracle_node_ghosts(const nebo_expr_t* e, int node, int with_bc, oracle_ghost_t* out)
 {
   if (node < 0 || node >= e->n_nodes) return NEBO_ERR_INVALID;
@@ -141,7 +163,13 @@ int oracle_node_ghosts(const nebo_expr_t* e, int node, int with_bc, oracle_ghost
     *out = g;
     return NEBO_OK;
   }
-  case NEBO_STENCIL:
+  case NEBO_STENCIL: {
+    const nebo_stencil_t* s = &e->stencils[n->idx];
+    if ((rc = oracle_node_ghosts(e, n->a, with_bc, &ga))) return rc;
+    *out = g_add(ga, s->kind == NEBO_STENCIL_EDGELESS ? g_edgeless_shift(s) : g_stencil_shift(s));
+    return NEBO_OK;
+  }
+  case NEBO_MASK_SHIFT:
     if ((rc = oracle_node_ghosts(e, n->a, with_bc, &ga))) return rc;
     *out = g_add(ga, g_stencil_shift(&e->stencils[n->idx]));
     return NEBO_OK;
@@ -181,7 +209,7 @@ static int node_extents(const nebo_expr_t* e, int node, int32_t ext[3])
     for (int d = 0; d < 3; ++d) ext[d] = m->extent[d] - m->ghost_minus[d] - m->ghost_plus[d];
     return 1;
   }
-  case NEBO_STENCIL: return node_extents(e, n->a, ext);
+  case NEBO_STENCIL: case NEBO_MASK_SHIFT: return node_extents(e, n->a, ext);
   case NEBO_COND:
     if (node_extents(e, n->a, ext)) return 1;
     if (node_extents(e, n->b, ext)) return 1;
@@ -256,10 +284,20 @@ double oracle_eval(const nebo_expr_t* e, int node, int x, int y, int z)
   switch (n->op) {
   case NEBO_SCALAR: return n->value;
   case NEBO_FIELD: {
+    /* reads outside the allocation (only an edgeless stencil can make them, at
+       points a cond does not select) give NaN instead of touching memory */
     const nebo_field_t* f = &e->fields[n->idx];
-    return ((const double*)f->values)[field_index(f, x, y, z)];
+    long i = (long)f->offset[0] + f->valid_minus[0] + x +
+             (long)f->glob[0] * ((long)f->offset[1] + f->valid_minus[1] + y +
+                                 (long)f->glob[1] * ((long)f->offset[2] + f->valid_minus[2] + z));
+    if (i < 0 || i >= (long)f->glob[0] * f->glob[1] * f->glob[2]) return NAN;
+    return ((const double*)f->values)[i];
   }
   case NEBO_MASK: return mask_bit(&e->masks[n->idx], x, y, z) ? 1.0 : 0.0;
+  case NEBO_MASK_SHIFT: {
+    const int32_t* P = e->stencils[n->idx].pts[0];
+    return mask_bit(&e->masks[e->nodes[n->a].idx], x + P[0], y + P[1], z + P[2]) ? 1.0 : 0.0;
+  }
   case NEBO_ADD: return oracle_eval(e, n->a, x, y, z) + oracle_eval(e, n->b, x, y, z);
   case NEBO_SUB: return oracle_eval(e, n->a, x, y, z) - oracle_eval(e, n->b, x, y, z);
   case NEBO_MUL: return oracle_eval(e, n->a, x, y, z) * oracle_eval(e, n->b, x, y, z);
@@ -302,7 +340,7 @@ double oracle_eval(const nebo_expr_t* e, int node, int x, int y, int z)
     double acc = 0.0;
     for (int i = 0; i < s->npts; ++i) {
       double v = oracle_eval(e, n->a, x + s->pts[i][0], y + s->pts[i][1], z + s->pts[i][2]);
-      if (s->kind == NEBO_STENCIL_WEIGHTED) v = v * s->coefs[i];
+      if (s->kind != NEBO_STENCIL_SUM) v = v * s->coefs[i];
       acc = (i == 0) ? v : acc + v;
     }
     return acc;
@@ -440,6 +478,52 @@ int oracle_masked_assign(const nebo_field_t* lhs, const int32_t* pts, int n,
       int x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
       out[field_index(lhs, x, y, z)] = oracle_eval(rhs, root, x, y, z);
     }
+  return NEBO_OK;
+}
+
+/* Boundary-condition inversion on the CPU (NeboBoundaryConditionBuilder,
+ * NeboStencilBuilder.h:590-764, shift points :504-526).  The operator's points
+ * are in definition order: the low point is the first defined (the reference's
+ * PointCollection::Last, coefficient coefs().last()), the high point the last
+ * (PointCollection::First, coefs().coef()).  Minus side: shift = (gamma offset <
+ * phi offset) per component, or e_axis for a same-location (FD) operator; plus
+ * side: shift = -(gamma offset > phi offset), or -e_axis.  For each mask point g
+ * (gamma frame) in list order, x = g - shift and
+ *   phi(x) = (gamma(x - p_side) * 1.0 - fold_{i != side} phi(x + p_i - p_side) * c_i) / c_side
+ * with the fold left to right in definition order, written at once (cpu_apply
+ * runs a single pass). */
+int oracle_bc_apply(const nebo_field_t* phi, const nebo_expr_t* gamma,
+                    const nebo_stencil_t* op, const int32_t phi_off[3],
+                    const int32_t gamma_off[3], int fd_axis, const int32_t* pts, int n,
+                    int minus)
+{
+  if (op->npts < 2) return NEBO_ERR_INVALID;
+  int side = minus ? 0 : op->npts - 1;
+  int32_t shift[3];
+  for (int d = 0; d < 3; ++d) {
+    if (fd_axis >= 0)
+      shift[d] = (d == fd_axis) ? (minus ? 1 : -1) : 0;
+    else
+      shift[d] = minus ? (gamma_off[d] < phi_off[d]) : -(gamma_off[d] > phi_off[d]);
+  }
+  const int32_t* ps = op->pts[side];
+  double* out = (double*)phi->values;
+  int root = gamma->n_nodes - 1;
+  for (int i = 0; i < n; ++i) {
+    int x = pts[3 * i] - shift[0], y = pts[3 * i + 1] - shift[1], z = pts[3 * i + 2] - shift[2];
+    double g = oracle_eval(gamma, root, x - ps[0], y - ps[1], z - ps[2]) * 1.0;
+    double acc = 0.0;
+    int first = 1;
+    for (int k = 0; k < op->npts; ++k) {
+      if (k == side) continue;
+      int qx = x + op->pts[k][0] - ps[0], qy = y + op->pts[k][1] - ps[1],
+          qz = z + op->pts[k][2] - ps[2];
+      double v = out[field_index(phi, qx, qy, qz)] * op->coefs[k];
+      acc = first ? v : acc + v;
+      first = 0;
+    }
+    out[field_index(phi, x, y, z)] = (g - acc) / op->coefs[side];
+  }
   return NEBO_OK;
 }
 

@@ -48,6 +48,16 @@ int oracle_masked_assign(const nebo_field_t* lhs, const int32_t* pts, int n,
 /* nebo_fold (NeboReductions.h:45-97) with max/min/sum and norm = sqrt(sum(pow(x,2))). */
 int oracle_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out);
 
+/* NeboBoundaryConditionBuilder::operator()(mask, phi, gamma, minus) on the CPU:
+ * phi is set at x = g - shift for each mask point g so that the operator `op`
+ * (phi's location -> gamma's location; points in definition order) applied to
+ * phi gives gamma(g) there.  gamma is an expression evaluated in the gamma frame;
+ * fd_axis >= 0 marks a same-location FD operator along that axis. */
+int oracle_bc_apply(const nebo_field_t* phi, const nebo_expr_t* gamma,
+                    const nebo_stencil_t* op, const int32_t phi_off[3],
+                    const int32_t gamma_off[3], int fd_axis, const int32_t* pts, int n,
+                    int minus);
+
 /* Periodic copy-fill of all allocated ghost layers (x, then y, then z sweep). */
 void oracle_fill_ghosts_periodic(const nebo_field_t* f);
 

@@ -96,6 +96,19 @@ inline NodeP stencil(std::shared_ptr<const StencilSpec> s, NodeP arg)
   return n;
 }
 
+// mask(x + P) of a mask leaf (NeboMaskShift, NeboStencils.h:1048-1213)
+inline NodeP mask_shift(NodeP mask, const std::array<int, 3>& P)
+{
+  auto s = std::make_shared<StencilSpec>();
+  s->pts.push_back(P);
+  s->coefs.push_back(1.0);
+  auto n = std::make_shared<Node>();
+  n->op = NEBO_MASK_SHIFT;
+  n->a = std::move(mask);
+  n->stencil = std::move(s);
+  return n;
+}
+
 }  // namespace nebo_detail
 
 // ---------------------------------------------------------------------------
@@ -114,6 +127,36 @@ class SpatialMask {
         valid_(prototype.get_valid_ghost_data()),
         st_(std::make_shared<State>())
   {
+    set_points(points);
+  }
+
+  // SpatialMask(window, bc, ghosts, points) (SpatialMask.h:99-114)
+  SpatialMask(const MemoryWindow& window, const BoundaryCellInfo& bc, const GhostData& ghosts,
+              const std::vector<IntVec>& points)
+      : window_(window),
+        bc_(bc.limit_by_extent(window.extent())),
+        ghosts_(ghosts.limit_by_extent(window.extent())),
+        valid_(ghosts_),
+        st_(std::make_shared<State>())
+  {
+    set_points(points);
+  }
+
+  // SpatialMask<FieldT>::build(prototype, points) (SpatialMask.h:173-181): the
+  // window a FieldT would have next to `prototype`, with its valid ghosts.
+  template <typename Proto>
+  static SpatialMask build(const Proto& prototype, const std::vector<IntVec>& points)
+  {
+    const IntVec n = prototype.window_without_ghost().extent();
+    const BoundaryCellInfo& pb = prototype.boundary_info();
+    const BoundaryCellInfo bc = BoundaryCellInfo::build<FieldT>(pb.has_bc(0), pb.has_bc(1), pb.has_bc(2));
+    const GhostData& g = prototype.get_valid_ghost_data();
+    return SpatialMask(get_window_with_ghost(n, g, bc), bc, g, points);
+  }
+
+ private:
+  void set_points(const std::vector<IntVec>& points)
+  {
     const IntVec g = window_.glob_dim();
     st_->words.assign((window_.glob_npts() + 31) / 32, 0u);
     for (const IntVec& p : points) {
@@ -127,6 +170,8 @@ class SpatialMask {
       st_->points.push_back(p[2]);
     }
   }
+
+ public:
 
   // BitField::add_consumer (BitField.h:466-481): a copy of the bits on device d
   void add_consumer(short d)
@@ -235,7 +280,7 @@ class Flat {
               "Nebo error in Nebo Mask:\nmask used on a GPU without a copy there (call add_consumer)");
         masks_.push_back(m);
       });
-    if (n->op == NEBO_STENCIL)
+    if (n->op == NEBO_STENCIL || n->op == NEBO_MASK_SHIFT)
       out.idx = table(stencil_ids_, n->stencil.get(), [&] {
         nebo_stencil_t s;
         std::memset(&s, 0, sizeof s);
@@ -553,15 +598,25 @@ SpatialField<L, T>& interior_assign(SpatialField<L, T>& lhs, const R& rhs)
   return lhs;
 }
 
+namespace nebo_detail {
+
+// lhs(p) = rhs(p) for each interior-relative point of `pts` (x, y, z triples)
+template <typename F>
+void assign_points(F& lhs, const std::vector<int32_t>& pts, const NodeP& rhs)
+{
+  nebo_field_t d = lhs.descriptor();
+  Flat flat(rhs, d.device);
+  nebo_expr_t e = flat.expr();
+  nebo_check(nebo_masked_assign(&d, pts.data(), (int32_t)(pts.size() / 3), &e, lhs.get_stream()));
+  lhs.mark_written();
+}
+
+}  // namespace nebo_detail
+
 template <typename F, typename R>
 F& masked_assign(const SpatialMask<F>& mask, F& lhs, const R& rhs)
 {
-  nebo_field_t d = lhs.descriptor();
-  nebo_detail::Flat flat(nebo_detail::Operand<R>::node(rhs), d.device);
-  nebo_expr_t e = flat.expr();
-  const std::vector<int32_t>& p = mask.flat_points();
-  nebo_check(nebo_masked_assign(&d, p.data(), (int32_t)(p.size() / 3), &e, lhs.get_stream()));
-  lhs.mark_written();
+  nebo_detail::assign_points(lhs, mask.flat_points(), nebo_detail::Operand<R>::node(rhs));
   return lhs;
 }
 

@@ -359,6 +359,155 @@ inline void build_stencils(unsigned nx, unsigned ny, unsigned nz, double Lx, dou
   reg_fd_all<ZVolField>(db, hc);
 }
 
+// ---------------------------------------------------------------------------
+// Boundary conditions (NeboStencilBuilder.h:504-764)
+// ---------------------------------------------------------------------------
+namespace nebo_detail {
+
+// Direction of an operator type tag (GradientX -> 0, ...), -1 when it has none.
+template <typename Op, typename = void>
+struct op_axis { static constexpr int value = -1; };
+template <typename Op>
+struct op_axis<Op, typename std::conditional<true, void, typename Op::DirT>::type> {
+  static constexpr int value = Op::DirT::value;
+};
+
+template <typename OpT, typename Src, typename Dest>
+struct MaskShiftPoints {
+  // MaskShiftPoints (:504-516): minus = (src < dest), plus = -(src > dest) per
+  // component of the location offsets; a same-location (FD) operator shifts by
+  // +e and -e along its direction (:518-526).
+  static std::array<int, 3> get(bool minus)
+  {
+    std::array<int, 3> p = {0, 0, 0};
+    if (std::is_same<Src, Dest>::value) {
+      static_assert(!std::is_same<Src, Dest>::value || op_axis<OpT>::value >= 0,
+                    "Nebo: a mask shift between fields of one type needs a directional operator");
+      p[op_axis<OpT>::value >= 0 ? op_axis<OpT>::value : 0] = minus ? 1 : -1;
+      return p;
+    }
+    const IntVec s = Src::Location::offset(), d = Dest::Location::offset();
+    for (int a = 0; a < 3; ++a) p[a] = minus ? (s[a] < d[a] ? 1 : 0) : -(s[a] > d[a] ? 1 : 0);
+    return p;
+  }
+};
+
+}  // namespace nebo_detail
+
+// NeboMaskShiftBuilder<Op, Src, Dest>: a mask of Src points as a boolean
+// expression over Dest points, mask(x + shift).
+template <typename OperatorT, typename SrcFieldT, typename DestFieldT>
+struct NeboMaskShiftBuilder {
+  typedef OperatorT OperatorType;
+  typedef SrcFieldT SrcFieldType;
+  typedef DestFieldT DestFieldType;
+
+  static std::array<int, 3> minus_point()
+  {
+    return nebo_detail::MaskShiftPoints<typename OperatorT::type, SrcFieldT, DestFieldT>::get(true);
+  }
+  static std::array<int, 3> plus_point()
+  {
+    return nebo_detail::MaskShiftPoints<typename OperatorT::type, SrcFieldT, DestFieldT>::get(false);
+  }
+  NeboBooleanExpression<DestFieldT> minus(const SpatialMask<SrcFieldT>& src) const
+  {
+    return NeboBooleanExpression<DestFieldT>(nebo_detail::mask_shift(src.node(), minus_point()));
+  }
+  NeboBooleanExpression<DestFieldT> plus(const SpatialMask<SrcFieldT>& src) const
+  {
+    return NeboBooleanExpression<DestFieldT>(nebo_detail::mask_shift(src.node(), plus_point()));
+  }
+};
+
+// NeboBoundaryConditionBuilder<Op>(op)(mask, phi, gamma, minus): sets, for each
+// gamma point g of the mask, the phi value at x = g - shift so that op(phi)(g)
+// equals gamma(g).  The low point is op's first point (coefficient coefs()[0]),
+// the high point its last:
+//   phi(x) = (gamma(x - p_side) * 1.0 - fold_{i != side} phi(x + p_i - p_side) * c_i) / c_side
+// with edgeless stencils.  The reference's CPU path loops once over the mask
+// points (:640-676); this runs that loop as one masked assignment on the GPU,
+// touching only the boundary points (its GPU path sweeps the whole field with a
+// cond, :708-714, which gives the same values).
+template <typename OperatorT>
+struct NeboBoundaryConditionBuilder {
+  typedef OperatorT OperatorType;
+  typedef typename OperatorT::SrcFieldType PhiFieldType;
+  typedef typename OperatorT::DestFieldType GammaFieldType;
+  typedef NeboMaskShiftBuilder<OperatorT, GammaFieldType, PhiFieldType> Shift;
+
+  explicit NeboBoundaryConditionBuilder(const OperatorT& op)
+  {
+    const nebo_detail::Points& p = op.points();
+    const std::vector<double>& c = op.coefs();
+    if (p.size() < 2)
+      throw std::runtime_error("Nebo error in Nebo Boundary Condition:\n"
+                               "the operator needs two or more points");
+    lowCoef_ = c.front();
+    highCoef_ = c.back();
+    auto sub = [](const std::array<int, 3>& a, const std::array<int, 3>& b) {
+      return std::array<int, 3>{a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    };
+    const std::array<int, 3> zero = {0, 0, 0};
+    minusGamma_ = edgeless({sub(zero, p.front())}, {1.0});
+    plusGamma_ = edgeless({sub(zero, p.back())}, {1.0});
+    nebo_detail::Points mp, pp;
+    for (size_t i = 1; i < p.size(); ++i) mp.push_back(sub(p[i], p.front()));
+    for (size_t i = 0; i + 1 < p.size(); ++i) pp.push_back(sub(p[i], p.back()));
+    minusPhi_ = edgeless(mp, std::vector<double>(c.begin() + 1, c.end()));
+    plusPhi_ = edgeless(pp, std::vector<double>(c.begin(), c.end() - 1));
+  }
+
+  void operator()(const SpatialMask<GammaFieldType>& mask, PhiFieldType& phi,
+                  const NeboExpression<GammaFieldType>& gamma, bool minus) const
+  {
+    apply(mask, phi, gamma.node(), minus);
+  }
+  void operator()(const SpatialMask<GammaFieldType>& mask, PhiFieldType& phi,
+                  const GammaFieldType& gamma, bool minus) const
+  {
+    apply(mask, phi, nebo_detail::field_leaf(gamma), minus);
+  }
+  void operator()(const SpatialMask<GammaFieldType>& mask, PhiFieldType& phi, double gamma,
+                  bool minus) const
+  {
+    apply(mask, phi, nebo_detail::scalar(gamma), minus);
+  }
+
+  // The inverted value at phi points, (gamma term - phi term) / coefficient.
+  NeboExpression<PhiFieldType> rhs(const PhiFieldType& phi, const nebo_detail::NodeP& gamma,
+                                   bool minus) const
+  {
+    using namespace nebo_detail;
+    NodeP g = stencil(minus ? minusGamma_ : plusGamma_, gamma);
+    NodeP f = stencil(minus ? minusPhi_ : plusPhi_, field_leaf(phi));
+    return NeboExpression<PhiFieldType>(
+        op2(NEBO_DIV, op2(NEBO_SUB, g, f), scalar(minus ? lowCoef_ : highCoef_)));
+  }
+
+ private:
+  static std::shared_ptr<const nebo_detail::StencilSpec> edgeless(nebo_detail::Points p,
+                                                                  std::vector<double> c)
+  {
+    auto s = std::make_shared<nebo_detail::StencilSpec>();
+    s->kind = NEBO_STENCIL_EDGELESS;
+    s->pts = std::move(p);
+    s->coefs = std::move(c);
+    return s;
+  }
+  void apply(const SpatialMask<GammaFieldType>& mask, PhiFieldType& phi, const nebo_detail::NodeP& gamma,
+             bool minus) const
+  {
+    const std::array<int, 3> sh = minus ? Shift::minus_point() : Shift::plus_point();
+    std::vector<int32_t> pts(mask.flat_points());
+    for (size_t i = 0; i < pts.size(); ++i) pts[i] -= sh[i % 3];
+    nebo_detail::assign_points(phi, pts, rhs(phi, gamma, minus).node());
+  }
+
+  double lowCoef_ = 0.0, highCoef_ = 0.0;
+  std::shared_ptr<const nebo_detail::StencilSpec> minusGamma_, plusGamma_, minusPhi_, plusPhi_;
+};
+
 // Grid (spatialops/structured/Grid.h): extents, lengths, spacings, coordinates
 class Grid {
  public:

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <sstream>
+#include <unordered_map>
 
 namespace nebo {
 namespace {
@@ -70,7 +71,7 @@ __device__ __forceinline__ int pick(const CellIdx<NCLS, V>& c, int cls, int v)
 template <int OP>
 constexpr int vm_pops()
 {
-  if constexpr (OP == VM_LOAD || OP == VM_CONST || OP == VM_MASK) return 0;
+  if constexpr (OP == VM_LOAD || OP == VM_LOADG || OP == VM_CONST || OP == VM_MASK) return 0;
   else if constexpr (OP >= VM_ADD && OP <= VM_OR) return 2;
   else if constexpr (OP == VM_SELECT) return 3;
   else return 1;
@@ -81,7 +82,7 @@ __device__ __forceinline__ void vm_exec(double (&S)[kVmMaxDepth][V], const VmPro
                                         int imm, const CellIdx<NCLS, V>& c)
 {
   constexpr int pops = vm_pops<OP>();
-  constexpr bool push = (OP == VM_LOAD || OP == VM_CONST || OP == VM_MASK);
+  constexpr bool push = (OP == VM_LOAD || OP == VM_LOADG || OP == VM_CONST || OP == VM_MASK);
   constexpr bool math = (OP == VM_POW || OP == VM_POWC || (OP >= VM_EXP && OP <= VM_ERFC));
   if constexpr (D < pops || (push && D >= kVmMaxDepth) || (math && !MATH)) {
     return;  // never emitted by the host for this instantiation
@@ -90,6 +91,15 @@ __device__ __forceinline__ void vm_exec(double (&S)[kVmMaxDepth][V], const VmPro
     const int cls = P.loads[imm].cls;
 #pragma unroll
     for (int v = 0; v < V; ++v) S[D][v] = b[pick(c, cls, v)];
+  } else if constexpr (OP == VM_LOADG) {
+    // an edgeless stencil may read past its operand's block at cells that a
+    // cond does not select; those reads yield NaN instead of touching memory
+    const VmLoad& L = P.loads[imm];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int i = pick(c, L.cls, v);
+      S[D][v] = (i >= L.glo && i < L.ghi) ? L.base[i] : __builtin_nan("");
+    }
   } else if constexpr (OP == VM_CONST) {
     const double k = P.consts[imm];
 #pragma unroll
@@ -162,10 +172,10 @@ __device__ __forceinline__ void vm_exec(double (&S)[kVmMaxDepth][V], const VmPro
   VM_CD(VM_AND) VM_CD(VM_OR) VM_CD(VM_NEG) VM_CD(VM_ABS) VM_CD(VM_SQRT)                \
   VM_CD(VM_SQUARE) VM_CD(VM_CUBE) VM_CD(VM_POWC) VM_CD(VM_EXP) VM_CD(VM_LOG)           \
   VM_CD(VM_LOG10) VM_CD(VM_SIN) VM_CD(VM_COS) VM_CD(VM_TAN) VM_CD(VM_TANH)             \
-  VM_CD(VM_ERF) VM_CD(VM_ERFC) VM_CD(VM_NOT) VM_CD(VM_SELECT)
+  VM_CD(VM_ERF) VM_CD(VM_ERFC) VM_CD(VM_NOT) VM_CD(VM_SELECT) VM_CD(VM_LOADG)
 
 static_assert(kVmMaxDepth == 10, "VM_CD covers depths 0..10");
-static_assert(VM_NUM_OPS == 36, "VM_ALL_CASES lists every opcode");
+static_assert(VM_NUM_OPS == 37, "VM_ALL_CASES lists every opcode");
 
 template <int NCLS, int V, bool MATH>
 __device__ __forceinline__ void vm_run(const VmProgram& P, double (&S)[kVmMaxDepth][V],
@@ -230,8 +240,8 @@ __global__ __launch_bounds__(kBlock) void vm_assign_kernel(const VmProgram P, Bo
 template <int NCLS, bool MATH>
 __global__ __launch_bounds__(kBlock) void vm_masked_kernel(const VmProgram P,
                                                            const int32_t* __restrict__ pts,
-                                                           int npts, double* __restrict__ out,
-                                                           int oX, int oXY)
+                                                           int npts, double* out, int oX,
+                                                           int oXY)
 {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= npts) return;
@@ -406,7 +416,7 @@ bool subtree_reads(const nebo_expr_t* e, int node, const void* values)
 {
   const nebo_node_t& n = e->nodes[node];
   switch (n.op) {
-  case NEBO_SCALAR: case NEBO_MASK: return false;
+  case NEBO_SCALAR: case NEBO_MASK: case NEBO_MASK_SHIFT: return false;
   case NEBO_FIELD: return e->fields[n.idx].values == values;
   default:
     if (subtree_reads(e, n.a, values)) return true;
@@ -429,6 +439,72 @@ int check_aliasing(const nebo_field_t* lhs, const nebo_expr_t* e, const char* ct
       return set_error(NEBO_ERR_INVALID, ctx,
                        "left-hand side is read by a stencil on the right-hand side; neighbouring "
                        "points would be read after they are written (assign into another field)");
+  }
+  return NEBO_OK;
+}
+
+// Flat index of interior-relative point q in field f's allocation.
+long flat_of(const nebo_field_t& f, const int32_t q[3])
+{
+  const long x = f.offset[0] + f.valid_minus[0] + q[0];
+  const long y = f.offset[1] + f.valid_minus[1] + q[1];
+  const long z = f.offset[2] + f.valid_minus[2] + q[2];
+  return x + (long)f.glob[0] * (y + (long)f.glob[1] * z);
+}
+
+std::string point_text(const int32_t* p)
+{
+  std::ostringstream m;
+  m << "(" << p[0] << ", " << p[1] << ", " << p[2] << ")";
+  return m.str();
+}
+
+// Reads of a masked assignment, point by point: edgeless reads must stay in
+// their field's block, and the LHS may only be read where no other point writes.
+int check_masked_reads(const nebo_field_t* lhs, const nebo_expr_t* e, const int32_t* points,
+                       int n_points, const char* ctx)
+{
+  std::vector<FieldRead> reads;
+  collect_reads(e, e->n_nodes - 1, 0, 0, 0, false, &reads);
+  bool reads_lhs = false;
+  for (const FieldRead& r : reads) {
+    const nebo_field_t& f = e->fields[r.field];
+    if (f.values == lhs->values) reads_lhs = true;
+    if (!r.edgeless) continue;
+    for (int i = 0; i < n_points; ++i)
+      for (int d = 0; d < 3; ++d) {
+        const long g = (long)f.offset[d] + f.valid_minus[d] + points[3 * i + d] + r.s[d];
+        if (g < 0 || g >= f.glob[d]) {
+          std::ostringstream m;
+          m << "mask point " << i << " " << point_text(points + 3 * i) << " reads field " << r.field
+            << " at offset (" << r.s[0] << ", " << r.s[1] << ", " << r.s[2]
+            << "), outside the field's allocated block";
+          return set_error(NEBO_ERR_GHOST, ctx, m.str());
+        }
+      }
+  }
+  if (!reads_lhs) return NEBO_OK;
+  std::unordered_map<long, int> written;
+  written.reserve((size_t)n_points * 2);
+  for (int i = 0; i < n_points; ++i)
+    if (!written.emplace(flat_of(*lhs, points + 3 * i), i).second)
+      return set_error(NEBO_ERR_INVALID, ctx,
+                       "mask lists point " + point_text(points + 3 * i) +
+                           " more than once while the right-hand side reads the left-hand side");
+  for (const FieldRead& r : reads) {
+    const nebo_field_t& f = e->fields[r.field];
+    if (f.values != lhs->values) continue;
+    for (int i = 0; i < n_points; ++i) {
+      const int32_t q[3] = {points[3 * i] + r.s[0], points[3 * i + 1] + r.s[1],
+                            points[3 * i + 2] + r.s[2]};
+      auto it = written.find(flat_of(f, q));
+      if (it != written.end() && it->second != i)
+        return set_error(NEBO_ERR_INVALID, ctx,
+                         "mask point " + point_text(points + 3 * i) + " reads the left-hand side at " +
+                             point_text(q) + ", which mask point " +
+                             point_text(points + 3 * it->second) +
+                             " writes; the result would depend on the order of the writes");
+    }
   }
   return NEBO_OK;
 }
@@ -597,7 +673,6 @@ extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points
   if (rc) return rc;
   if ((rc = check_devices(rhs, lhs->device, ctx))) return rc;
   if ((rc = check_index_range(rhs, lhs, ctx))) return rc;
-  if ((rc = check_aliasing(lhs, rhs, ctx))) return rc;
   if (n_points <= 0) return NEBO_OK;
   // every point must lie in the LHS window and in the RHS's valid range
   Ghost g;
@@ -618,6 +693,7 @@ extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points
   VmProgram prog;
   int depth = 0;
   if ((rc = lower_program(rhs, &prog, &depth))) return rc;
+  if ((rc = check_masked_reads(lhs, rhs, points, n_points, ctx))) return rc;
   double* out = frame_origin(lhs);
   const int oX = lhs->glob[0], oXY = lhs->glob[0] * lhs->glob[1];
   hipStream_t s = (hipStream_t)stream;

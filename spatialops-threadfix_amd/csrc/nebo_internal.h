@@ -54,6 +54,7 @@ enum VmOp : uint8_t {
   VM_NEG, VM_ABS, VM_SQRT, VM_SQUARE, VM_CUBE, VM_POWC,  // POWC: pow(top, const[imm])
   VM_EXP, VM_LOG, VM_LOG10, VM_SIN, VM_COS, VM_TAN, VM_TANH, VM_ERF, VM_ERFC, VM_NOT,
   VM_SELECT,     // c = pop, b = pop, a = pop, push (a != 0 ? b : c)
+  VM_LOADG,      // VM_LOAD that yields NaN where the read leaves the field's block
   VM_NUM_OPS
 };
 
@@ -72,6 +73,7 @@ struct VmInsn {
 struct VmLoad {
   const double* base;   // values + flat(offset + valid_minus + shift)
   int32_t cls;          // index class (glob shape)
+  int32_t glo, ghi;     // cell indices i with base + i inside the glob block: [glo, ghi)
   int32_t pad_;
 };
 
@@ -96,6 +98,18 @@ struct VmProgram {
 
 // Lower a tree to a stack program.  Also records the max stack depth.
 int lower_program(const nebo_expr_t* e, VmProgram* prog, int* max_depth);
+
+// One (field, offset) read of an expression; edgeless when an edgeless stencil
+// lies between the read and the root.
+struct FieldRead {
+  int field;
+  int s[3];
+  bool edgeless;
+};
+// Every distinct read of the subtree at `node` evaluated at offset (sx, sy, sz).
+// Call on trees that lower_program accepts (its size limits bound the walk).
+void collect_reads(const nebo_expr_t* e, int node, int sx, int sy, int sz, bool edgeless,
+                   std::vector<FieldRead>* out);
 
 // ---------------------------------------------------------------------------
 // recognised trees

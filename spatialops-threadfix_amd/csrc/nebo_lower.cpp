@@ -6,7 +6,10 @@
 // "infinite": an infinite left operand of + stays, an infinite right operand
 // wins), leaf ghosts of spatialops/NeboRhs.h:52-54,183-188, stencil ghosts of
 // spatialops/NeboStencils.h:222-239,311-317 (each point p moves the minus count
-// by p and the plus count by -p, folded with min), cond of NeboCond.h:86-91,284-289,
+// by p and the plus count by -p, folded with min), edgeless stencils of
+// NeboStencils.h:263-276,556-567 (additive_point_to_ghost: a point only widens the
+// side opposite to the one it reads), mask shifts of NeboStencils.h:1048-1051,
+// cond of NeboCond.h:86-91,284-289,
 // and the limits of calculate_actual_ghost (NeboBasic.h:61-100) and
 // NeboField<Initial>::assign (NeboLhs.h:67-88).
 
@@ -66,13 +69,30 @@ Ghost stencil_shift(const nebo_stencil_t& s)
   return g;
 }
 
+// Edgeless rule: additive_point_to_ghost(p) = (minus max(p, 0), plus max(-p, 0)),
+// folded with min over the points.  The reference only compiles the one-point
+// form (its multi-point fold at NeboStencils.h:231-233 calls a misspelt
+// function); the fold here is the one its other point collections use.
+Ghost edgeless_shift(const nebo_stencil_t& s)
+{
+  Ghost g = Ghost::uniform(0);
+  for (int i = 0; i < s.npts; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const int v = s.pts[i][d];
+      const int mn = v > 0 ? v : 0, pl = v < 0 ? -v : 0;
+      if (i == 0 || mn < g.m[d]) g.m[d] = mn;
+      if (i == 0 || pl < g.p[d]) g.p[d] = pl;
+    }
+  return g;
+}
+
 bool is_unary(int op)
 {
   switch (op) {
   case NEBO_NEG: case NEBO_ABS: case NEBO_SQRT: case NEBO_SQUARE: case NEBO_CUBE:
   case NEBO_POWINT: case NEBO_EXP: case NEBO_LOG: case NEBO_LOG10: case NEBO_SIN:
   case NEBO_COS: case NEBO_TAN: case NEBO_TANH: case NEBO_ERF: case NEBO_ERFC:
-  case NEBO_NOT: case NEBO_STENCIL:
+  case NEBO_NOT: case NEBO_STENCIL: case NEBO_MASK_SHIFT:
     return true;
   default:
     return false;
@@ -110,8 +130,20 @@ int validate_expr(const nebo_expr_t* e, const char* ctx)
       }
       const nebo_stencil_t& s = e->stencils[n.idx];
       if (s.npts < 1 || s.npts > NEBO_MAX_STENCIL_POINTS ||
-          (s.kind != NEBO_STENCIL_WEIGHTED && s.kind != NEBO_STENCIL_SUM)) {
+          (s.kind != NEBO_STENCIL_WEIGHTED && s.kind != NEBO_STENCIL_SUM &&
+           s.kind != NEBO_STENCIL_EDGELESS)) {
         msg << "stencil " << n.idx << " has " << s.npts << " points, kind " << s.kind;
+        return set_error(NEBO_ERR_INVALID, ctx, msg.str());
+      }
+    }
+    if (n.op == NEBO_MASK_SHIFT) {
+      if (n.idx < 0 || n.idx >= e->n_stencils || !e->stencils || e->stencils[n.idx].npts != 1) {
+        msg << "mask shift node " << i << " needs a one-point stencil entry (got index " << n.idx
+            << ")";
+        return set_error(NEBO_ERR_INVALID, ctx, msg.str());
+      }
+      if (!child_ok(n.a) || e->nodes[n.a].op != NEBO_MASK) {
+        msg << "mask shift node " << i << " must take a mask node that precedes it";
         return set_error(NEBO_ERR_INVALID, ctx, msg.str());
       }
     }
@@ -148,7 +180,13 @@ int node_ghosts(const nebo_expr_t* e, int node, bool with_bc, Ghost* out)
     *out = with_bc ? g_add(g, g_point(m.bc_extra)) : g;
     return NEBO_OK;
   }
-  case NEBO_STENCIL:
+  case NEBO_STENCIL: {
+    if ((rc = node_ghosts(e, n.a, with_bc, &a))) return rc;
+    const nebo_stencil_t& s = e->stencils[n.idx];
+    *out = g_add(a, s.kind == NEBO_STENCIL_EDGELESS ? edgeless_shift(s) : stencil_shift(s));
+    return NEBO_OK;
+  }
+  case NEBO_MASK_SHIFT:   // additive_reductive_point_to_ghost(P) = (P, -P)
     if ((rc = node_ghosts(e, n.a, with_bc, &a))) return rc;
     *out = g_add(a, stencil_shift(e->stencils[n.idx]));
     return NEBO_OK;
@@ -164,6 +202,36 @@ int node_ghosts(const nebo_expr_t* e, int node, bool with_bc, Ghost* out)
     if ((rc = node_ghosts(e, n.b, with_bc, &b))) return rc;
     *out = g_min(a, b);
     return NEBO_OK;
+  }
+}
+
+void collect_reads(const nebo_expr_t* e, int node, int sx, int sy, int sz, bool edgeless,
+                   std::vector<FieldRead>* out)
+{
+  const nebo_node_t& n = e->nodes[node];
+  switch (n.op) {
+  case NEBO_SCALAR: case NEBO_MASK: case NEBO_MASK_SHIFT:
+    return;
+  case NEBO_FIELD: {
+    for (const FieldRead& r : *out)
+      if (r.field == n.idx && r.s[0] == sx && r.s[1] == sy && r.s[2] == sz && r.edgeless == edgeless)
+        return;
+    out->push_back(FieldRead{n.idx, {sx, sy, sz}, edgeless});
+    return;
+  }
+  case NEBO_STENCIL: {
+    const nebo_stencil_t& s = e->stencils[n.idx];
+    const bool eg = edgeless || s.kind == NEBO_STENCIL_EDGELESS;
+    for (int i = 0; i < s.npts; ++i)
+      collect_reads(e, n.a, sx + s.pts[i][0], sy + s.pts[i][1], sz + s.pts[i][2], eg, out);
+    return;
+  }
+  default:
+    collect_reads(e, n.a, sx, sy, sz, edgeless, out);
+    if (is_unary(n.op)) return;
+    collect_reads(e, n.b, sx, sy, sz, edgeless, out);
+    if (n.op == NEBO_COND) collect_reads(e, n.c, sx, sy, sz, edgeless, out);
+    return;
   }
 }
 
@@ -259,6 +327,7 @@ struct Lowerer {
   VmProgram* p;
   int depth = 0;
   int max_depth = 0;
+  int guard = 0;   // > 0 inside an edgeless stencil: field reads are bounds-checked
   std::map<std::tuple<int, int, int, int>, int> load_slot;   // (field, sx, sy, sz)
   std::map<std::tuple<int, int, int, int>, int> mask_slot;
   std::map<std::pair<int, int>, int> classes;               // (X, Y) -> class
@@ -320,8 +389,15 @@ struct Lowerer {
     load_slot[key] = s;
     const nebo_field_t& fd = e->fields[f];
     double* o = frame_origin(&fd);
-    p->loads[s].base = o + sx + (long)fd.glob[0] * (sy + (long)fd.glob[1] * sz);
+    double* b = o + sx + (long)fd.glob[0] * (sy + (long)fd.glob[1] * sz);
+    p->loads[s].base = b;
     p->loads[s].cls = cls(fd.glob[0], fd.glob[1]);
+    // base - values and the block size both fit in int32: fields are checked to
+    // hold fewer than 2^31 points and stencil shifts are small
+    const long off = (long)(b - static_cast<double*>(fd.values));
+    const long npts = (long)fd.glob[0] * fd.glob[1] * fd.glob[2];
+    p->loads[s].glo = (int32_t)(-off);
+    p->loads[s].ghi = (int32_t)(npts - off);
     return s;
   }
 
@@ -378,15 +454,23 @@ struct Lowerer {
     const nebo_node_t& n = e->nodes[node];
     switch (n.op) {
     case NEBO_SCALAR: emit(VM_CONST, konst(n.value), 0, 1); return;
-    case NEBO_FIELD: emit(VM_LOAD, load(n.idx, sx, sy, sz), 0, 1); return;
+    case NEBO_FIELD: emit(guard ? VM_LOADG : VM_LOAD, load(n.idx, sx, sy, sz), 0, 1); return;
     case NEBO_MASK: emit(VM_MASK, mask(n.idx, sx, sy, sz), 0, 1); return;
+    case NEBO_MASK_SHIFT: {
+      const int32_t* P = e->stencils[n.idx].pts[0];
+      emit(VM_MASK, mask(e->nodes[n.a].idx, sx + P[0], sy + P[1], sz + P[2]), 0, 1);
+      return;
+    }
     case NEBO_STENCIL: {
       const nebo_stencil_t& s = e->stencils[n.idx];
+      const bool edgeless = s.kind == NEBO_STENCIL_EDGELESS;
+      guard += edgeless;
       for (int i = 0; i < s.npts; ++i) {
         gen(n.a, sx + s.pts[i][0], sy + s.pts[i][1], sz + s.pts[i][2]);
-        if (s.kind == NEBO_STENCIL_WEIGHTED) emit(VM_MULC, konst(s.coefs[i]), 1, 1);
+        if (s.kind != NEBO_STENCIL_SUM) emit(VM_MULC, konst(s.coefs[i]), 1, 1);
         if (i > 0) emit(VM_ADD, 0, 2, 1);
       }
+      guard -= edgeless;
       return;
     }
     case NEBO_COND:

@@ -31,12 +31,13 @@ OPCODES = [
     "SCALAR", "FIELD", "MASK", "ADD", "SUB", "MUL", "DIV", "NEG", "MAX", "MIN",
     "ABS", "SQRT", "SQUARE", "CUBE", "POW", "POWINT", "EXP", "LOG", "LOG10", "SIN",
     "COS", "TAN", "TANH", "ERF", "ERFC", "EQ", "NE", "LT", "LE", "GT", "GE",
-    "AND", "OR", "NOT", "STENCIL", "COND",
+    "AND", "OR", "NOT", "STENCIL", "COND", "MASK_SHIFT",
 ]
 OP = {name: i for i, name in enumerate(OPCODES)}
 
 STENCIL_WEIGHTED = 0
 STENCIL_SUM = 1
+STENCIL_EDGELESS = 2
 
 RED_MAX, RED_MIN, RED_SUM, RED_NORM = 0, 1, 2, 3
 

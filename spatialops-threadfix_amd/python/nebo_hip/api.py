@@ -57,12 +57,17 @@ def assign_plan(lhs: Field, rhs, use_ghost=True, where="gpu") -> int:
 
 
 def masked_assign(mask: Mask, lhs: Field, rhs, stream=None):
+    return assign_points(mask.points, lhs, rhs, stream)
+
+
+def assign_points(points, lhs: Field, rhs, stream=None):
+    """lhs(p) = rhs(p) for each interior-relative (x, y, z) point p of the list."""
     import numpy as np
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 3))
     s = serialize(E(rhs), "gpu")
     d = lhs.desc("gpu")
-    pts = np.ascontiguousarray(mask.points.reshape(-1), dtype=np.int32)
     _abi.call("nebo_masked_assign", C.byref(d), C.c_void_p(pts.ctypes.data),
-              int(len(mask.points)), s.ref, _stream(stream))
+              int(len(pts)), s.ref, _stream(stream))
     return lhs
 
 

@@ -13,7 +13,7 @@ import ctypes as C
 from . import _abi
 from .fields import Field, Mask
 
-_BOOL_OPS = {"EQ", "NE", "LT", "LE", "GT", "GE", "AND", "OR", "NOT", "MASK"}
+_BOOL_OPS = {"EQ", "NE", "LT", "LE", "GT", "GE", "AND", "OR", "NOT", "MASK", "MASK_SHIFT"}
 
 
 class Expr:
@@ -108,15 +108,19 @@ def cond(test, value):
 
 
 class Stencil:
-    """A weighted (sum_p arg(x+p)*c_p) or plain-sum stencil over fixed points."""
+    """A weighted (sum_p arg(x+p)*c_p), plain-sum or edgeless stencil over fixed
+    points.  An edgeless stencil folds like a weighted one but does not shrink the
+    valid range on the side it reads (NeboStencils.h:526-760)."""
 
-    def __init__(self, points, coefs=None, kind=_abi.STENCIL_WEIGHTED, src="SVol", dst="SVol", name=""):
+    def __init__(self, points, coefs=None, kind=_abi.STENCIL_WEIGHTED, src="SVol", dst="SVol", name="",
+                 axis=None):
         self.points = [tuple(int(v) for v in p) for p in points]
         self.coefs = [float(c) for c in coefs] if coefs is not None else [1.0] * len(self.points)
         self.kind = kind
         self.src = src
         self.dst = dst
         self.name = name
+        self.axis = axis   # direction of a same-location (FD) operator
         assert 1 <= len(self.points) <= _abi.NEBO_MAX_STENCIL_POINTS
         assert len(self.coefs) == len(self.points)
 
@@ -171,7 +175,7 @@ def serialize(root: Expr, where: str = "gpu") -> Serialized:
                 midx[k] = len(masks)
                 masks.append(e.mask.desc(where))
             n.idx = midx[k]
-        elif e.op == "STENCIL":
+        elif e.op in ("STENCIL", "MASK_SHIFT"):
             k = id(e.stencil)
             if k not in sidx:
                 sidx[k] = len(stencils)
@@ -191,6 +195,15 @@ def serialize(root: Expr, where: str = "gpu") -> Serialized:
     arr_m = (_abi.MaskT * max(1, len(masks)))(*masks)
     t = _abi.ExprT(len(nodes), arr_n, len(fields), arr_f, len(stencils), arr_s, len(masks), arr_m)
     return Serialized(t, (arr_n, arr_f, arr_s, arr_m, root))
+
+
+def mask_shift(mask, point) -> Expr:
+    """mask(x + point) as a boolean expression (NeboMaskShift,
+    NeboStencils.h:1048-1213)."""
+    m = E(mask)
+    if m.op != "MASK":
+        raise TypeError("mask_shift takes a Mask")
+    return Expr("MASK_SHIFT", (m,), stencil=Stencil([tuple(point)], name="MaskShift"))
 
 
 def _lift_methods(cls):

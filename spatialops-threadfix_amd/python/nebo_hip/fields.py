@@ -178,6 +178,16 @@ class Mask:
         self.device = device
         self.gpu = None
 
+    @staticmethod
+    def build(prototype, loc, points, device=0):
+        """SpatialMask<loc>::build(prototype, points) (SpatialMask.h:173-181): a mask of
+        location ``loc`` on the window a field of that location would have next to
+        ``prototype`` (a Field or Layout), with the prototype's valid ghosts."""
+        L = prototype.layout if isinstance(prototype, Field) else prototype
+        lay = Layout.build(L.interior, (L.valid_minus, L.valid_plus), loc,
+                           tuple(bool(b) for b in L.has_bc))
+        return Mask(lay, points, device)
+
     def add_consumer(self, device=0):
         """Copy the bit field to a GPU (BitField::add_consumer, BitField.h:466-481)."""
         self.device = device

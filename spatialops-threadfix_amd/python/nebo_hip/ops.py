@@ -122,8 +122,10 @@ def build_stencils(nx, ny, nz, Lx, Ly, Lz) -> OperatorDatabase:
         db[("Filter", loc, loc)] = Average(BOX27, loc, loc)
     for vol in VOLS:
         for d, ax in enumerate("XYZ"):
-            db[(f"Interpolant{ax}", vol, vol)] = Stencil(fd_points(d), half, src=vol, dst=vol, name="FD")
-            db[(f"Gradient{ax}", vol, vol)] = Stencil(fd_points(d), hcoef[d], src=vol, dst=vol, name="FD")
+            db[(f"Interpolant{ax}", vol, vol)] = Stencil(fd_points(d), half, src=vol, dst=vol, name="FD",
+                                                         axis=d)
+            db[(f"Gradient{ax}", vol, vol)] = Stencil(fd_points(d), hcoef[d], src=vol, dst=vol, name="FD",
+                                                      axis=d)
     return db
 
 

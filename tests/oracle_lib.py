@@ -32,6 +32,8 @@ _SIGS = {
     "oracle_assign_threads": (C.c_int, [_P(_abi.FieldT), _P(_abi.ExprT), C.c_int, C.c_int]),
     "oracle_masked_assign": (C.c_int, [_P(_abi.FieldT), _vp, C.c_int, _P(_abi.ExprT), C.c_int]),
     "oracle_reduce": (C.c_int, [_P(_abi.ExprT), C.c_int, C.c_int, _P(C.c_double)]),
+    "oracle_bc_apply": (C.c_int, [_P(_abi.FieldT), _P(_abi.ExprT), _P(_abi.StencilT), _vp, _vp, C.c_int,
+                                  _vp, C.c_int, C.c_int]),
     "oracle_fill_ghosts_periodic": (None, [_P(_abi.FieldT)]),
     "oracle_mask_bits": (None, [_P(_abi.MaskT), _vp, C.c_int, _vp]),
     "oracle_divkgrad": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_int]),
@@ -112,6 +114,32 @@ def masked_assign(mask, lhs, rhs_expr, passes=2):
     pts = np.ascontiguousarray(mask.points.reshape(-1), dtype=np.int32)
     check(lib().oracle_masked_assign(C.byref(d), pts.ctypes.data, len(mask.points), s.ref, passes))
     return lhs
+
+
+def masked_assign_points(points, lhs, rhs_expr, passes=1):
+    from nebo_hip.expr import serialize, E
+    s = serialize(E(rhs_expr), "host")
+    d = lhs.desc("host")
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1))
+    check(lib().oracle_masked_assign(C.byref(d), pts.ctypes.data, len(pts) // 3, s.ref, passes))
+    return lhs
+
+
+def bc_apply(op, mask_points, phi, gamma, minus):
+    """NeboBoundaryConditionBuilder<op>(mask, phi, gamma, minus) on host copies.
+    ``op`` is a nebo_hip Stencil with src = phi location, dst = gamma location."""
+    from nebo_hip.expr import serialize, E
+    from nebo_hip.fields import LOCATIONS
+    s = serialize(E(gamma), "host")
+    d = phi.desc("host")
+    st = op.struct()
+    po = (C.c_int32 * 3)(*LOCATIONS[op.src][0])
+    go = (C.c_int32 * 3)(*LOCATIONS[op.dst][0])
+    fd = -1 if op.src != op.dst else int(op.axis)
+    pts = np.ascontiguousarray(np.asarray(mask_points, dtype=np.int32).reshape(-1))
+    check(lib().oracle_bc_apply(C.byref(d), s.ref, C.byref(st), C.addressof(po), C.addressof(go), fd,
+                                pts.ctypes.data, len(pts) // 3, int(bool(minus))))
+    return phi
 
 
 def fill_periodic(f):

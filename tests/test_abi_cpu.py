@@ -86,8 +86,11 @@ def test_limits_agree_with_oracle_on_random_trees(seed):
     db = nh.build_stencils(max(n[0], 1), max(n[1], 1), max(n[2], 1), 1, 1, 1)
     stencils = [db[("Gradient", "SVol", "SSurfX")], db[("Divergence", "SSurfY", "SVol")],
                 db[("Interpolant", "SVol", "XSurfY")], db[("Filter", "SVol", "SVol")].sum,
-                db[("GradientZ", "SVol", "SVol")]]
-    masks = [nh.Mask(L, [(0, 0, 0)])]
+                db[("GradientZ", "SVol", "SVol")],
+                nh.Stencil([(1, 0, 0)], [1.0], kind=_abi.STENCIL_EDGELESS),
+                nh.Stencil([(0, -1, 0), (0, 0, 2)], [0.5, -2.0], kind=_abi.STENCIL_EDGELESS)]
+    m0 = nh.Mask(L, [(0, 0, 0)])
+    masks = [m0, nh.mask_shift(m0, (1, 0, -1)), nh.mask_shift(m0, (0, -2, 0))]
     for _ in range(5):
         e = random_tree(rng, leaves, stencils, masks, 4)
         for ug in (True, False):

@@ -34,6 +34,20 @@ void f(SVolField& r, const SVolField& k, const SVolField& phi, const OperatorDat
   double m = field_norm_interior(phi) + field_max(phi * k);
   (void)m;
 }
+void g(SVolField& phi, const SSurfXField& gamma, const SpatialMask<SSurfXField>& mask,
+       const OperatorDatabase& db) {
+  typedef BasicOpTypes<SVolField>::GradX OpT;
+  NeboBoundaryConditionBuilder<OpT> bc(*db.retrieve_operator<OpT>());
+  bc(mask, phi, gamma, true);
+  bc(mask, phi, 2.0, false);
+  NeboMaskShiftBuilder<OpT, SSurfXField, SVolField> shift;
+  SVolField out(phi);
+  out <<= cond(shift.minus(mask), 1.0)(0.0);
+  typedef OperatorTypeBuilder<GradientY, SVolField, SVolField>::type FD;
+  NeboBoundaryConditionBuilder<FD> fd(*db.retrieve_operator<FD>());
+  SpatialMask<SVolField> m2 = SpatialMask<SVolField>::build(phi, {IntVec(0, 0, 0)});
+  fd(m2, phi, phi * 2.0, true);
+}
 """
 
 BAD = r"""
@@ -55,7 +69,7 @@ def test_mixed_field_types_do_not_compile(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prog", ["stencil_test", "mask_test", "heat_equation", "reduction_test"])
+@pytest.mark.parametrize("prog", ["stencil_test", "mask_test", "heat_equation", "reduction_test", "bc_test"])
 def test_cpp_program(prog):
     exe = os.path.join(BIN, prog)
     assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
