@@ -426,9 +426,9 @@ struct NeboMaskShiftBuilder {
 // the high point its last:
 //   phi(x) = (gamma(x - p_side) * 1.0 - fold_{i != side} phi(x + p_i - p_side) * c_i) / c_side
 // with edgeless stencils.  The reference's CPU path loops once over the mask
-// points (:640-676); this runs that loop as one masked assignment on the GPU,
+// points (:646-680); this runs that loop as one masked assignment on the GPU,
 // touching only the boundary points (its GPU path sweeps the whole field with a
-// cond, :708-714, which gives the same values).
+// cond, :712-719, which gives the same values).
 template <typename OperatorT>
 struct NeboBoundaryConditionBuilder {
   typedef OperatorT OperatorType;

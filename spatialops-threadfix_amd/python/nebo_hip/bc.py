@@ -17,8 +17,8 @@ by +e and -e along its direction (:518-526).  There
 with both stencils edgeless (their valid range does not shrink).
 
 The reference applies this in two ways.  On the CPU it loops once over the mask
-points (``cpu_apply``, :640-676).  On a GPU it sweeps the whole field with
-``phi <<= cond(shift(mask), rhs)(phi)`` (:708-714).  Both give the same values at
+points (``cpu_apply``, :646-680).  On a GPU it sweeps the whole field with
+``phi <<= cond(shift(mask), rhs)(phi)`` (:712-719).  Both give the same values at
 the mask points and leave every other point as it was.  :class:`BoundaryCondition`
 runs the first form on the GPU: one masked assignment over the shifted points,
 which touches only the boundary.  :meth:`BoundaryCondition.cond_expr` builds the
