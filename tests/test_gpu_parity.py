@@ -227,15 +227,17 @@ def test_random_trees_bit_exact(seed):
             assert np.array_equal(got, ref.host)
 
 
-def test_every_stencil_tile_bit_exact():
+@pytest.mark.parametrize("lengths", [(1.0, 1.0, 1.0), (0.7, 0.37, 0.45)])
+def test_every_stencil_tile_bit_exact(lengths):
     """The tile shape changes only the schedule: every instantiated shape gives the
-    same bits, on a box that is not a multiple of any tile."""
+    same bits, on a box that is not a multiple of any tile, with one spacing per
+    axis and with one spacing for all axes."""
     n = (70, 37, 45)
     L = nh.Layout.build(n, 1)
     rng = np.random.default_rng(8)
     k = both(L, rng.uniform(0.5, 2.0, L.npts))
     phi = both(L, rng.standard_normal(L.npts))
-    db, ops = _ops(n)
+    db, ops = _ops(n, lengths)
     c = nh.axis_coefs(db)
     ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
     orc.divkgrad(ref, k, phi, c, 0)
