@@ -147,11 +147,9 @@ class SpatialMask {
   template <typename Proto>
   static SpatialMask build(const Proto& prototype, const std::vector<IntVec>& points)
   {
-    const IntVec n = prototype.window_without_ghost().extent();
-    const BoundaryCellInfo& pb = prototype.boundary_info();
-    const BoundaryCellInfo bc = BoundaryCellInfo::build<FieldT>(pb.has_bc(0), pb.has_bc(1), pb.has_bc(2));
-    const GhostData& g = prototype.get_valid_ghost_data();
-    return SpatialMask(get_window_with_ghost(n, g, bc), bc, g, points);
+    return SpatialMask(create_new_memory_window<FieldT, Proto>(prototype),
+                       create_new_boundary_cell_info<FieldT, Proto>(prototype),
+                       prototype.get_valid_ghost_data(), points);
   }
 
  private:

@@ -16,6 +16,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #define CPU_INDEX (-1)
@@ -179,6 +180,39 @@ class MemoryWindow {
 
 inline int get_dim_with_ghost(int n, int gm, int gp, int bc) { return n > 1 ? n + gm + gp + bc : 1; }
 
+// Host iterator over a window, x fastest then y then z (FieldIterator,
+// MemoryWindow.h:330-560): the i-th point of the window's extent.
+template <typename T>
+class WindowIterator {
+ public:
+  WindowIterator(T* values, const MemoryWindow& w, size_t i = 0) : v_(values), w_(w), i_(i) {}
+  T& operator*() const
+  {
+    const size_t ex = w_.extent(0), ey = w_.extent(1);
+    const size_t x = i_ % ex, y = (i_ / ex) % ey, z = i_ / (ex * ey);
+    return v_[(w_.offset(0) + x) + w_.glob_dim(0) * ((w_.offset(1) + y) + w_.glob_dim(1) * (w_.offset(2) + z))];
+  }
+  WindowIterator& operator++()
+  {
+    ++i_;
+    return *this;
+  }
+  WindowIterator operator++(int)
+  {
+    WindowIterator t(*this);
+    ++i_;
+    return t;
+  }
+  WindowIterator operator+(size_t n) const { return WindowIterator(v_, w_, i_ + n); }
+  bool operator==(const WindowIterator& o) const { return v_ == o.v_ && i_ == o.i_; }
+  bool operator!=(const WindowIterator& o) const { return !(*this == o); }
+
+ private:
+  T* v_;
+  MemoryWindow w_;
+  size_t i_;
+};
+
 inline MemoryWindow get_window_with_ghost(const IntVec& n, const GhostData& g, const BoundaryCellInfo& bc)
 {
   return MemoryWindow(IntVec(get_dim_with_ghost(n[0], g.get_minus(0), g.get_plus(0), bc.has_extra(0)),
@@ -318,6 +352,28 @@ class SpatialField {
         info_(std::make_shared<FieldStorage<T>>(window.glob_npts(), values, mode, devIdx))
   {}
 
+  // Copies share the storage (SpatialField.h:186-192); assignment copies the
+  // values of the whole block from the other field's active device to this
+  // field's active device (SpatialField.h:626-718).
+  SpatialField(const SpatialField&) = default;
+  SpatialField& operator=(const SpatialField& other)
+  {
+    if (window_ != other.window_)
+      throw std::runtime_error("Error : Attempted assignment between fields of unequal size!");
+    if (info_ == other.info_) return *this;
+    const short dd = active_device_index(), sd = other.active_device_index();
+    const size_t bytes = window_.glob_npts() * sizeof(T);
+    T* dst = info_->raw(dd);
+    const T* src = other.info_->const_values(sd);
+    if (IS_CPU_INDEX(dd) && IS_CPU_INDEX(sd)) std::memcpy(dst, src, bytes);
+    else if (IS_CPU_INDEX(dd)) nebo_check(nebo_memcpy(dst, src, bytes, 1, nullptr));
+    else if (IS_CPU_INDEX(sd)) nebo_check(nebo_memcpy(dst, src, bytes, 0, nullptr));
+    else if (dd == sd) nebo_check(nebo_memcpy(dst, src, bytes, 2, nullptr));
+    else nebo_check(nebo_memcpy_peer(dst, dd, src, sd, bytes, nullptr));
+    mark_written();
+    return *this;
+  }
+
   const MemoryWindow& window_with_ghost() const { return window_; }
   MemoryWindow window_without_ghost() const
   {
@@ -354,6 +410,17 @@ class SpatialField {
   }
   T& operator[](size_t i) { return field_values(CPU_INDEX)[i]; }
 
+  // host iterators over the window with ghosts and over the interior
+  // (SpatialField.h:448-499); the CPU copy must be the active one
+  typedef WindowIterator<T> iterator;
+  typedef WindowIterator<const T> const_iterator;
+  iterator begin() { return iterator(field_values(CPU_INDEX), window_); }
+  iterator end() { return begin() + window_.local_npts(); }
+  const_iterator begin() const { return const_iterator(field_values(CPU_INDEX), window_); }
+  const_iterator end() const { return begin() + window_.local_npts(); }
+  iterator interior_begin() { return iterator(field_values(CPU_INDEX), window_without_ghost()); }
+  iterator interior_end() { return interior_begin() + window_without_ghost().local_npts(); }
+
   // the engine-side description of this field on its active device
   nebo_field_t descriptor() const
   {
@@ -387,6 +454,25 @@ class SpatialField {
   void* stream_ = nullptr;
 };
 
+// A window / boundary info for a field of type FieldT next to a prototype of
+// another type: the prototype's window grown by the difference of the two
+// locations' BC extra cells on the axes that carry a boundary
+// (create_new_memory_window / create_new_boundary_cell_info, SpatialField.h:720-755).
+template <typename FieldT, typename ProtoT>
+BoundaryCellInfo create_new_boundary_cell_info(const ProtoT& prototype)
+{
+  return BoundaryCellInfo::build<FieldT>(prototype.boundary_info().has_bc());
+}
+
+template <typename FieldT, typename ProtoT>
+MemoryWindow create_new_memory_window(const ProtoT& prototype)
+{
+  const BoundaryCellInfo bc = create_new_boundary_cell_info<FieldT, ProtoT>(prototype);
+  const MemoryWindow& w = prototype.window_with_ghost();
+  const IntVec inc = bc.has_bc() * (FieldT::Location::bc_extra() - ProtoT::Location::bc_extra());
+  return MemoryWindow(w.glob_dim() + inc, w.offset(), w.extent() + inc);
+}
+
 // field typedefs (FVStaggeredFieldTypes.h)
 typedef SpatialField<SVol> SVolField;
 typedef SpatialField<SSurfX> SSurfXField;
@@ -404,6 +490,7 @@ typedef SpatialField<ZVol> ZVolField;
 typedef SpatialField<ZSurfX> ZSurfXField;
 typedef SpatialField<ZSurfY> ZSurfYField;
 typedef SpatialField<ZSurfZ> ZSurfZField;
+typedef SpatialField<SingleValue> SingleValueField;
 
 template <typename VolT> struct FaceTypes;
 template <> struct FaceTypes<SVolField> { typedef SSurfXField XFace; typedef SSurfYField YFace; typedef SSurfZField ZFace; };

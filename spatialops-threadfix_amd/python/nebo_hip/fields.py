@@ -85,6 +85,22 @@ class Layout:
         return Layout.build(self.interior, (self.ghost_minus, self.ghost_plus), loc,
                             tuple(bool(b) for b in self.has_bc))
 
+    def create_new(self, loc, use_valid=False):
+        """Layout of a field of location ``loc`` next to a field of this layout
+        (create_new_memory_window / create_new_boundary_cell_info,
+        SpatialField.h:720-755): the window grows by the difference of the two
+        locations' BC extra cells on the axes that carry a boundary.  Ghosts are
+        this layout's total ghosts, or its valid ghosts when ``use_valid``."""
+        new_extra, old_extra = LOCATIONS[loc][1], LOCATIONS[self.loc][1]
+        bc = tuple(int(b) for b in self.has_bc)
+        inc = tuple(bc[d] * (new_extra[d] - old_extra[d]) for d in range(3))
+        gm = self.valid_minus if use_valid else self.ghost_minus
+        gp = self.valid_plus if use_valid else self.ghost_plus
+        return Layout(glob=tuple(self.glob[d] + inc[d] for d in range(3)), offset=self.offset,
+                      extent=tuple(self.extent[d] + inc[d] for d in range(3)), ghost_minus=gm,
+                      ghost_plus=gp, valid_minus=gm, valid_plus=gp, has_bc=bc,
+                      bc_extra=tuple(new_extra[d] if bc[d] else 0 for d in range(3)), loc=loc)
+
     @property
     def interior(self):
         return tuple(self.extent[d] - self.ghost_minus[d] - self.ghost_plus[d] - self.bc_extra[d]
@@ -117,6 +133,18 @@ class Field:
         self.gpu = gpu
         self.device = device
         self.name = name
+
+    @staticmethod
+    def like(prototype: "Field", loc=None, where=None, device=None, name=""):
+        """SpatialFieldStore::get<loc>(prototype) (SpatialFieldStore.h:240-257): an
+        uninitialised field of location ``loc`` (default: the prototype's) shaped
+        after ``prototype``, where the prototype lives.  GPU storage comes from
+        torch's caching allocator, which keeps freed blocks for reuse as the
+        reference's store does."""
+        L = prototype.layout.create_new(loc or prototype.layout.loc)
+        if where is None:
+            where = "gpu" if prototype.gpu is not None else "cpu"
+        return Field.empty(L, where, prototype.device if device is None else device, name)
 
     @staticmethod
     def empty(layout: Layout, where="gpu", device=0, name=""):
@@ -184,9 +212,7 @@ class Mask:
         location ``loc`` on the window a field of that location would have next to
         ``prototype`` (a Field or Layout), with the prototype's valid ghosts."""
         L = prototype.layout if isinstance(prototype, Field) else prototype
-        lay = Layout.build(L.interior, (L.valid_minus, L.valid_plus), loc,
-                           tuple(bool(b) for b in L.has_bc))
-        return Mask(lay, points, device)
+        return Mask(L.create_new(loc, use_valid=True), points, device)
 
     def add_consumer(self, device=0):
         """Copy the bit field to a GPU (BitField::add_consumer, BitField.h:466-481)."""

@@ -15,21 +15,24 @@ const double length = 1.5;
 const double dx = length / dim[0];
 const double dy = length / dim[1];
 
+// a temporary from the store, as the reference test obtains its fields
 template <typename F>
-F make(short dev)
+SpatFldPtr<F> make()
 {
   const BoundaryCellInfo bc = BoundaryCellInfo::build<F>(true, true, true);
   const GhostData g(1);
-  return F(get_window_with_ghost(dim, g, bc), bc, g, nullptr, InternalStorage, dev);
+  return SpatialFieldStore::get_from_window<F>(get_window_with_ghost(dim, g, bc), bc, g);
 }
 
-// fill every point of the window with fn(i, j), i and j counted from -1
+// fill the window through the field iterator with fn(i, j), i and j counted
+// from -1 over the window's extent, as the test's loops do
 template <typename F, typename Fn>
 void fill(F& f, Fn fn)
 {
   const MemoryWindow& w = f.window_with_ghost();
-  for (int j = 0; j < (int)w.extent(1); ++j)
-    for (int i = 0; i < (int)w.extent(0); ++i) f(i, j, 0) = fn(i - 1, j - 1);
+  typename F::iterator it = f.begin();
+  for (int j = -1; j < (int)w.extent(1) - 1; ++j)
+    for (int i = -1; i < (int)w.extent(0) - 1; ++i) *it++ = fn(i, j);
 }
 
 template <typename F>
@@ -45,26 +48,26 @@ bool equal(F& a, F& b)
 template <typename Phi, typename Gamma, typename Op, typename Want>
 bool run(const std::vector<IntVec>& minusSet, const std::vector<IntVec>& plusSet, Want want)
 {
-  Phi test = make<Phi>(CPU_INDEX), ref = make<Phi>(CPU_INDEX);
-  Gamma gamma = make<Gamma>(CPU_INDEX);
-  fill(test, [](int i, int j) { return double(i + j * 5); });
-  fill(ref, want);
-  fill(gamma, [](int i, int j) { return double(25 + i + j * 5); });
-  test.set_device_as_active(GPU_INDEX);
-  gamma.set_device_as_active(GPU_INDEX);
+  SpatFldPtr<Phi> test = make<Phi>(), ref = make<Phi>();
+  SpatFldPtr<Gamma> gamma = make<Gamma>();
+  fill(*test, [](int i, int j) { return double(i + j * 5); });
+  fill(*ref, want);
+  fill(*gamma, [](int i, int j) { return double(25 + i + j * 5); });
+  test->set_device_as_active(GPU_INDEX);
+  gamma->set_device_as_active(GPU_INDEX);
 
   OperatorDatabase opdb;
   build_stencils(dim[0], dim[1], dim[2], length, length, length, opdb);
   const Op* const op = opdb.retrieve_operator<Op>();
   NeboBoundaryConditionBuilder<Op> BC(*op);
 
-  SpatialMask<Gamma> minus(gamma, minusSet);
+  SpatialMask<Gamma> minus(*gamma, minusSet);
   minus.add_consumer(GPU_INDEX);
-  BC(minus, test, gamma, true);
-  SpatialMask<Gamma> plus = SpatialMask<Gamma>::build(test, plusSet);
+  BC(minus, *test, *gamma, true);
+  SpatialMask<Gamma> plus = SpatialMask<Gamma>::build(*test, plusSet);
   plus.add_consumer(GPU_INDEX);
-  BC(plus, test, gamma, false);
-  return equal(test, ref);
+  BC(plus, *test, *gamma, false);
+  return equal(*test, *ref);
 }
 
 }  // namespace
