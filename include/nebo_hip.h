@@ -164,6 +164,12 @@ int nebo_free(int device, void* ptr);
 int nebo_pool_trim(int device);
 /* Bytes currently handed out / high-water mark (MemoryPool.h:49,82-90). */
 int nebo_pool_stats(int device, size_t* in_use, size_t* high_water);
+/* Page-locked host memory, so that host<->device copies on a stream run
+ * asynchronously (CUDADeviceInterface::get_pinned_pointer / releaseHost,
+ * ExternalAllocators.h:59-106; the reference's pool pins host blocks when CUDA is
+ * on, MemoryPool.cpp:132-133).  Fails with NEBO_ERR_HIP where there is no GPU. */
+int nebo_host_alloc(size_t bytes, void** out);
+int nebo_host_free(void* ptr);
 
 /* Copies.  kind: 0 host->device, 1 device->host, 2 device->device,
  * 3 peer (dst on dst_dev, src on src_dev).  stream NULL => synchronous

@@ -232,6 +232,7 @@ class FieldStorage {
     T* ptr = nullptr;
     bool valid = false;
     bool owned = false;
+    bool pinned = false;   // host copy in page-locked memory
   };
 
   FieldStorage(size_t n, T* external, StorageMode mode, short dev) : n_(n), active_(dev)
@@ -240,15 +241,14 @@ class FieldStorage {
     if (mode == ExternalStorage && external) {
       c.ptr = external;
     } else {
-      c.ptr = allocate(dev);
-      c.owned = true;
+      allocate(dev, c);
     }
     c.valid = true;
   }
   ~FieldStorage()
   {
     for (auto& kv : copies_)
-      if (kv.second.owned) release(kv.first, kv.second.ptr);
+      if (kv.second.owned) release(kv.first, kv.second);
   }
   FieldStorage(const FieldStorage&) = delete;
   FieldStorage& operator=(const FieldStorage&) = delete;
@@ -262,23 +262,23 @@ class FieldStorage {
   }
   bool is_available(short d) const { return copies_.count(d) != 0; }
 
-  void add_device(short d)
+  // Make the copy on d valid.  With a stream the copy is queued on it and the
+  // call returns at once (the *_async forms of FieldInfo.h:509-678): work queued
+  // later on that stream sees the data, the host waits with a stream sync.
+  void add_device(short d, void* stream = nullptr)
   {
     std::lock_guard<std::mutex> lk(mu_);
     Copy& c = copies_[d];
-    if (!c.ptr) {
-      c.ptr = allocate(d);
-      c.owned = true;
-    }
+    if (!c.ptr) allocate(d, c);
     if (!c.valid) {
-      copy(d, c.ptr, active_, copies_[active_].ptr);
+      copy(d, c.ptr, active_, copies_[active_].ptr, stream);
       c.valid = true;
     }
   }
-  void validate_device(short d) { add_device(d); }
-  void set_device_as_active(short d)
+  void validate_device(short d, void* stream = nullptr) { add_device(d, stream); }
+  void set_device_as_active(short d, void* stream = nullptr)
   {
-    add_device(d);
+    add_device(d, stream);
     active_ = d;
   }
   // write access: must be the active device; other copies become stale
@@ -304,27 +304,43 @@ class FieldStorage {
   }
 
  private:
-  T* allocate(short d)
+  // Host copies are page-locked when a GPU is present (so copies on a stream run
+  // asynchronously), plain heap memory otherwise; device copies come from the
+  // engine's pool.  Fresh storage is zeroed.
+  void allocate(short d, Copy& c)
   {
-    if (IS_CPU_INDEX(d)) return new T[n_]();
+    const size_t bytes = n_ * sizeof(T);
     void* p = nullptr;
-    nebo_check(nebo_alloc(d, n_ * sizeof(T), &p));
-    nebo_check(nebo_memset(p, 0, n_ * sizeof(T), nullptr));
-    return static_cast<T*>(p);
+    if (IS_CPU_INDEX(d)) {
+      int ndev = 0;
+      if (nebo_device_count(&ndev) == NEBO_OK && ndev > 0 && nebo_host_alloc(bytes, &p) == NEBO_OK && p) {
+        std::memset(p, 0, bytes);
+        c.pinned = true;
+        c.ptr = static_cast<T*>(p);
+      } else {
+        c.ptr = new T[n_]();
+      }
+    } else {
+      nebo_check(nebo_alloc(d, bytes, &p));
+      nebo_check(nebo_memset(p, 0, bytes, nullptr));
+      c.ptr = static_cast<T*>(p);
+    }
+    c.owned = true;
   }
-  void release(short d, T* p)
+  void release(short d, const Copy& c)
   {
-    if (IS_CPU_INDEX(d)) delete[] p;
-    else nebo_free(d, p);
+    if (!IS_CPU_INDEX(d)) nebo_free(d, c.ptr);
+    else if (c.pinned) nebo_host_free(c.ptr);
+    else delete[] c.ptr;
   }
-  void copy(short dd, T* dst, short sd, const T* src)
+  void copy(short dd, T* dst, short sd, const T* src, void* stream)
   {
     const size_t bytes = n_ * sizeof(T);
     if (IS_CPU_INDEX(dd) && IS_CPU_INDEX(sd)) std::memcpy(dst, src, bytes);
-    else if (IS_CPU_INDEX(dd)) nebo_check(nebo_memcpy(dst, src, bytes, 1, nullptr));
-    else if (IS_CPU_INDEX(sd)) nebo_check(nebo_memcpy(dst, src, bytes, 0, nullptr));
-    else if (dd == sd) nebo_check(nebo_memcpy(dst, src, bytes, 2, nullptr));
-    else nebo_check(nebo_memcpy_peer(dst, dd, src, sd, bytes, nullptr));
+    else if (IS_CPU_INDEX(dd)) nebo_check(nebo_memcpy(dst, src, bytes, 1, stream));
+    else if (IS_CPU_INDEX(sd)) nebo_check(nebo_memcpy(dst, src, bytes, 0, stream));
+    else if (dd == sd) nebo_check(nebo_memcpy(dst, src, bytes, 2, stream));
+    else nebo_check(nebo_memcpy_peer(dst, dd, src, sd, bytes, stream));
   }
 
   size_t n_;
@@ -387,12 +403,14 @@ class SpatialField {
   size_t allocated_bytes() const { return window_.glob_npts() * sizeof(T); }
 
   short active_device_index() const { return info_->active(); }
+  // the *_async forms queue the copy on the field's stream (set_stream) and
+  // return; wait_for_synchronization() waits for it (SpatialField.h:282-446)
   void add_device(short d) { info_->add_device(d); }
-  void add_device_async(short d) { info_->add_device(d); }
+  void add_device_async(short d) { info_->add_device(d, stream_); }
   void validate_device(short d) { info_->validate_device(d); }
-  void validate_device_async(short d) { info_->validate_device(d); }
+  void validate_device_async(short d) { info_->validate_device(d, stream_); }
   void set_device_as_active(short d) { info_->set_device_as_active(d); }
-  void set_device_as_active_async(short d) { info_->set_device_as_active(d); }
+  void set_device_as_active_async(short d) { info_->set_device_as_active(d, stream_); }
   bool is_valid(short d) const { return info_->is_valid(d); }
   bool is_available(short d) const { return info_->is_available(d); }
   T* field_values(short d = CPU_INDEX) { return info_->values(d); }

@@ -164,6 +164,22 @@ extern "C" int nebo_pool_stats(int device, size_t* in_use, size_t* high_water)
   return NEBO_OK;
 }
 
+extern "C" int nebo_host_alloc(size_t bytes, void** out)
+{
+  clear_error();
+  if (!out) return set_error(NEBO_ERR_INVALID, "Nebo Memory", "null argument");
+  *out = nullptr;
+  if (bytes == 0) return NEBO_OK;
+  return check_hip(hipHostMalloc(out, bytes, hipHostMallocDefault), "Nebo Memory");
+}
+
+extern "C" int nebo_host_free(void* ptr)
+{
+  clear_error();
+  if (!ptr) return NEBO_OK;
+  return check_hip(hipHostFree(ptr), "Nebo Memory");
+}
+
 extern "C" int nebo_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream)
 {
   clear_error();

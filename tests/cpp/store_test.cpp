@@ -7,8 +7,12 @@
 //  * GPU temporaries come from the engine's pool: releasing the last SpatFldPtr
 //    returns the block, and the next temporary of that size reuses it;
 //  * validity tracking: a GPU write makes the host copy stale until the field
-//    is moved back.
+//    is moved back;
+//  * asynchronous residency: copies queued on the field's stream with the
+//    *_async calls, ordered with the assignments on that stream.
 #include <spatialops/structured/FVStaggered.h>
+
+#include <hip/hip_runtime_api.h>
 
 #include <sstream>
 
@@ -88,6 +92,33 @@ int main()
     status(!svol1.is_valid(CPU_INDEX) && svol1.is_valid(GPU_INDEX), "host copy stale after a GPU write");
     svol1.set_device_as_active(CPU_INDEX);
     status(svol1.is_valid(CPU_INDEX) && svol1(2, 3, 4) == 9.0, "moved back to the host");
+  }
+
+  // asynchronous moves on the field's stream, ordered with its assignments
+  {
+    const IntVec n(40, 30, 20);
+    const GhostData g(1);
+    const BoundaryCellInfo bc = BoundaryCellInfo::build<SVolField>(false, false, false);
+    const MemoryWindow w = get_window_with_ghost(n, g, bc);
+    hipStream_t s = nullptr;
+    status(hipStreamCreate(&s) == hipSuccess, "stream created");
+    SVolField f(w, bc, g, nullptr, InternalStorage, CPU_INDEX);
+    f.set_stream(s);
+    for (size_t i = 0; i < w.glob_npts(); ++i) f[i] = double(i % 1000);
+    for (int rep = 0; rep < 3; ++rep) {
+      f.set_device_as_active_async(GPU_INDEX);   // queued host -> device copy
+      f <<= f * 2.0 + 1.0;                        // same stream: runs after the copy
+      f.set_device_as_active_async(CPU_INDEX);   // queued device -> host copy
+      f.wait_for_synchronization();
+    }
+    bool same = true;
+    for (size_t i = 0; i < w.glob_npts(); ++i) {
+      double v = double(i % 1000);
+      for (int rep = 0; rep < 3; ++rep) v = v * 2.0 + 1.0;
+      same = same && f[i] == v;
+    }
+    status(same, "async moves and assignments on one stream");
+    status(hipStreamDestroy(s) == hipSuccess, "stream destroyed");
   }
 
   // GPU temporaries reuse pooled blocks
