@@ -447,6 +447,54 @@ int nebo_transport_backward(const nebo_field_t* lam0, double* gk, const nebo_fie
                             const nebo_axis_coefs_t* c, double dt, int32_t n_steps,
                             int32_t every, const double* ckpt, double* work, void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* particles and point fields                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Constructor arguments of ParticleToCell / CellToParticle
+ * (spatialops/particles/ParticleOperators.h:151-166): mesh spacing and the
+ * coordinate of the first interior cell centre per axis.  A spacing <= 0 means
+ * the default (spacing 1, origin 0), as there. */
+typedef struct nebo_particle_geom {
+  double dx, xlo, dy, ylo, dz, zlo;
+} nebo_particle_geom_t;
+
+/* ParticleToCell<CellField>::apply_to_field (ParticleOperators.h:184-270): dest
+ * <<= 0, then each particle p (a cube of edge psize[p] centred at (px, py, pz))
+ * adds src[p] * (its volume fraction in the cell) to every cell of dest it
+ * overlaps.  px / py / pz may be NULL for an axis the particles do not resolve.
+ * Every cell receives its contributions in particle order, so the sums are those
+ * of the reference's serial loop bit for bit: the contributions are sorted by
+ * (cell, particle) and summed per cell.  A particle whose box begins a cell or
+ * more below the window on some axis contributes nothing (the reference's
+ * unsigned index conversion wraps there); `skipped` (device int, may be NULL)
+ * receives how many did.  All particle arrays hold np doubles on dest's GPU. */
+int nebo_particle_to_cell(const nebo_field_t* dest, const double* src, const double* px,
+                          const double* py, const double* pz, const double* psize, int64_t np,
+                          const nebo_particle_geom_t* geom, int32_t* skipped, void* stream);
+
+/* CellToParticle<CellField>::apply_to_field (ParticleOperators.h:305-395): dest[p]
+ * = sum over the cells the particle overlaps, in k, j, i order, of src(i,j,k) *
+ * (its volume fraction there); one thread per particle, same operations in the
+ * same order as the reference.  Skipped particles (as above) get 0. */
+int nebo_cell_to_particle(double* dest, const nebo_field_t* src, const double* px,
+                          const double* py, const double* pz, const double* psize, int64_t np,
+                          const nebo_particle_geom_t* geom, int32_t* skipped, void* stream);
+
+/* The index list of PointToField / FieldToPoint (spatialops/pointfield/
+ * PointOperators.h:40-160): flat indices into a field's block, uploaded once to
+ * `device`.  Where an index repeats, PointToField keeps the last point (the
+ * reference's loop writes in order).  Indices are taken from HOST memory. */
+typedef struct nebo_point_map nebo_point_map_t;
+int nebo_point_map_create(const int64_t* indices, int64_t n, int device, nebo_point_map_t** out);
+int nebo_point_map_destroy(nebo_point_map_t* map);
+/* PointToField::apply_to_field: dest.values[index[i]] = src[i] for i < n. */
+int nebo_point_to_field(const nebo_point_map_t* map, const nebo_field_t* dest, const double* src,
+                        void* stream);
+/* FieldToPoint::apply_to_field: dest[i] = src.values[index[i]] for i < n. */
+int nebo_field_to_point(const nebo_point_map_t* map, double* dest, const nebo_field_t* src,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,9 @@
  *   boundary conditions .. spatialops/NeboStencilBuilder.h:504-526 (mask shift
  *                          points) and :590-764 (cpu_apply: one pass over the
  *                          mask points, phi(g - shift) = (gamma - sum) / coef)
+ *   particles ............ spatialops/particles/ParticleOperators.h:151-395
+ *                          (serial particle loop, k/j/i cell loops, fractions)
+ *   point fields ......... spatialops/pointfield/PointOperators.h:95-160
  *   pointwise ops ........ spatialops/NeboOperators.h (SeqWalk eval bodies, e.g.
  *                          :181, :923, :1665, :2407, :4303, :6668, :6875, :13237,
  *                          :13992, :14710)
@@ -525,6 +528,131 @@ int oracle_bc_apply(const nebo_field_t* phi, const nebo_expr_t* gamma,
     out[field_index(phi, x, y, z)] = (g - acc) / op->coefs[side];
   }
   return NEBO_OK;
+}
+
+/* ParticleToCell / CellToParticle, restated from ParticleOperators.h:151-395.
+ * The reference converts the low cell index to size_t: exact (truncation) for
+ * t > -1; for t <= -1 the conversion is undefined and wraps in practice, which
+ * leaves the particle's loops empty.  The restatement takes that practical
+ * result: such a particle touches no cell. */
+typedef struct {
+  double h[3], face[3];
+  int has[3];
+  long lo[3], hi[3];
+  double plo[3], phi[3], pvol;
+  int skip;
+} pbox_t;
+
+static void particle_setup(pbox_t* b, const nebo_field_t* f, const nebo_particle_geom_t* g,
+                           const double* const c[3], const double* psize, long p)
+{
+  const double d[3] = {g->dx, g->dy, g->dz}, o[3] = {g->xlo, g->ylo, g->zlo};
+  const double size = psize[p], rp = size * 0.5;
+  b->skip = 0;
+  for (int a = 0; a < 3; ++a) {
+    /* constructor: dx_ = dx<=0 ? 1 : dx, xlo_ = dx<=0 ? 0 : xlo (:151-166) */
+    double h = d[a] <= 0 ? 1.0 : d[a], lo = d[a] <= 0 ? 0.0 : o[a];
+    b->h[a] = h;
+    /* xloface = xlo_ - dx_*ngm[0] - dx_/2 (:190-192) */
+    b->face[a] = lo - h * f->ghost_minus[a] - h / 2;
+    b->has[a] = c[a] != NULL;
+    b->plo[a] = b->has[a] ? c[a][p] - rp : 0;
+    b->phi[a] = b->has[a] ? b->plo[a] + size : 0;
+    double t = (b->plo[a] - b->face[a]) / h;
+    if (!(t > -1.0)) { b->skip = 1; t = 0; }
+    b->lo[a] = (long)t;
+    if (b->has[a]) {
+      double u = (b->phi[a] - b->face[a]) / h;
+      long ui = (long)u + 1;   /* int(u) + 1, then std::min with the window extent */
+      b->hi[a] = ui < f->extent[a] ? ui : f->extent[a];
+    } else {
+      b->hi[a] = b->lo[a] + 1;
+      if (b->hi[a] > f->extent[a]) b->hi[a] = f->extent[a];
+    }
+  }
+  b->pvol = (b->has[0] ? 2 * rp : 1) * (b->has[1] ? 2 * rp : 1) * (b->has[2] ? 2 * rp : 1);
+}
+
+/* dest(i,j,k) of a window index (MemoryWindow::flat_index, MemoryWindow.h:121-131) */
+static size_t window_index(const nebo_field_t* f, long i, long j, long k)
+{
+  long gi = f->glob[0] > 1 ? i + f->offset[0] : 0;
+  long gj = f->glob[1] > 1 ? j + f->offset[1] : 0;
+  long gk = f->glob[2] > 1 ? k + f->offset[2] : 0;
+  return (size_t)(gi + (long)f->glob[0] * (gj + (long)f->glob[1] * gk));
+}
+
+static double pmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+static double pmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+
+/* Sum (C2P, dest_cell == NULL) or scatter (P2C) one particle's fractions. */
+static double particle_pass(const pbox_t* b, const nebo_field_t* f, double* cells, double value)
+{
+  double acc = 0.0;
+  if (b->skip) return 0.0;
+  for (long k = b->lo[2]; k < b->hi[2]; ++k) {
+    double zcm = b->face[2] + k * b->h[2], zcp = zcm + b->h[2];
+    double zcont = b->has[2] ? pmin(zcp, b->phi[2]) - pmax(zcm, b->plo[2]) : 1;
+    for (long j = b->lo[1]; j < b->hi[1]; ++j) {
+      double ycm = b->face[1] + j * b->h[1], ycp = ycm + b->h[1];
+      double ycont = b->has[1] ? pmin(ycp, b->phi[1]) - pmax(ycm, b->plo[1]) : 1;
+      for (long i = b->lo[0]; i < b->hi[0]; ++i) {
+        double xcm = b->face[0] + i * b->h[0], xcp = xcm + b->h[0];
+        double xcont = b->has[0] ? pmin(xcp, b->phi[0]) - pmax(xcm, b->plo[0]) : 1;
+        double contribution = xcont * ycont * zcont / b->pvol;
+        size_t at = window_index(f, i, j, k);
+        if (cells == NULL)
+          acc += ((const double*)f->values)[at] * contribution;
+        else
+          cells[at] += value * contribution;
+      }
+    }
+  }
+  return acc;
+}
+
+int oracle_particle_to_cell(const nebo_field_t* dest, const double* src, const double* px,
+                            const double* py, const double* pz, const double* psize, int64_t np,
+                            const nebo_particle_geom_t* geom)
+{
+  /* dest <<= 0.0 */
+  nebo_node_t zero;
+  memset(&zero, 0, sizeof zero);
+  zero.op = NEBO_SCALAR;
+  nebo_expr_t e = {1, &zero, 0, NULL, 0, NULL, 0, NULL};
+  int rc = oracle_assign(dest, &e, 1);
+  if (rc) return rc;
+  const double* c[3] = {px, py, pz};
+  for (long p = 0; p < np; ++p) {
+    pbox_t b;
+    particle_setup(&b, dest, geom, c, psize, p);
+    particle_pass(&b, dest, (double*)dest->values, src[p]);
+  }
+  return NEBO_OK;
+}
+
+int oracle_cell_to_particle(double* dest, const nebo_field_t* src, const double* px,
+                            const double* py, const double* pz, const double* psize, int64_t np,
+                            const nebo_particle_geom_t* geom)
+{
+  const double* c[3] = {px, py, pz};
+  for (long p = 0; p < np; ++p) {
+    pbox_t b;
+    particle_setup(&b, src, geom, c, psize, p);
+    dest[p] = particle_pass(&b, src, NULL, 0.0);
+  }
+  return NEBO_OK;
+}
+
+/* PointToField / FieldToPoint (PointOperators.h:95-160): an in-order loop. */
+void oracle_point_to_field(double* field_values, const double* src, const int64_t* idx, int64_t n)
+{
+  for (int64_t i = 0; i < n; ++i) field_values[idx[i]] = src[i];
+}
+
+void oracle_field_to_point(double* dest, const double* field_values, const int64_t* idx, int64_t n)
+{
+  for (int64_t i = 0; i < n; ++i) dest[i] = field_values[idx[i]];
 }
 
 /* ------------------------------------------------------------------------- */

@@ -58,6 +58,17 @@ int oracle_bc_apply(const nebo_field_t* phi, const nebo_expr_t* gamma,
                     const int32_t gamma_off[3], int fd_axis, const int32_t* pts, int n,
                     int minus);
 
+/* ParticleToCell / CellToParticle::apply_to_field and the point-field
+ * operators, serial loops in the reference's order (host arrays). */
+int oracle_particle_to_cell(const nebo_field_t* dest, const double* src, const double* px,
+                            const double* py, const double* pz, const double* psize, int64_t np,
+                            const nebo_particle_geom_t* geom);
+int oracle_cell_to_particle(double* dest, const nebo_field_t* src, const double* px,
+                            const double* py, const double* pz, const double* psize, int64_t np,
+                            const nebo_particle_geom_t* geom);
+void oracle_point_to_field(double* field_values, const double* src, const int64_t* idx, int64_t n);
+void oracle_field_to_point(double* dest, const double* field_values, const int64_t* idx, int64_t n);
+
 /* Periodic copy-fill of all allocated ghost layers (x, then y, then z sweep). */
 void oracle_fill_ghosts_periodic(const nebo_field_t* f);
 

@@ -18,4 +18,5 @@ from .bc import MaskShift, BoundaryCondition, shift_points  # noqa: F401,E402
 from .batch import Batch, alloc_patches  # noqa: F401,E402
 from . import adjoint  # noqa: F401,E402
 from . import transport  # noqa: F401,E402
+from . import particles  # noqa: F401,E402
 from . import workloads  # noqa: F401,E402

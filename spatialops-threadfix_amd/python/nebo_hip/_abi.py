@@ -111,6 +111,11 @@ class AxisCoefsT(C.Structure):
     ]
 
 
+class ParticleGeomT(C.Structure):
+    _fields_ = [("dx", C.c_double), ("xlo", C.c_double), ("dy", C.c_double), ("ylo", C.c_double),
+                ("dz", C.c_double), ("zlo", C.c_double)]
+
+
 class PatchT(C.Structure):
     _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("n", I3), ("pad_", C.c_int32)]
 
@@ -169,6 +174,16 @@ _SIGS = {
                                          C.c_double, C.c_int32, C.c_int32, _vp, _vp]),
     "nebo_transport_backward": (C.c_int, [P(FieldT), _vp, P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT),
                                           C.c_double, C.c_int32, C.c_int32, _vp, _vp, _vp]),
+    "nebo_host_alloc": (C.c_int, [C.c_size_t, P(_vp)]),
+    "nebo_host_free": (C.c_int, [_vp]),
+    "nebo_particle_to_cell": (C.c_int, [P(FieldT), _vp, _vp, _vp, _vp, _vp, C.c_int64, P(ParticleGeomT), _vp,
+                                        _vp]),
+    "nebo_cell_to_particle": (C.c_int, [_vp, P(FieldT), _vp, _vp, _vp, _vp, C.c_int64, P(ParticleGeomT), _vp,
+                                        _vp]),
+    "nebo_point_map_create": (C.c_int, [_vp, C.c_int64, C.c_int, P(_vp)]),
+    "nebo_point_map_destroy": (C.c_int, [_vp]),
+    "nebo_point_to_field": (C.c_int, [_vp, P(FieldT), _vp, _vp]),
+    "nebo_field_to_point": (C.c_int, [_vp, _vp, P(FieldT), _vp]),
 }
 
 _lib = None

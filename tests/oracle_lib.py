@@ -49,6 +49,12 @@ _SIGS = {
     "oracle_transport_adjoint_stage": (None, [_P(_abi.FieldT), _vp, _vp, _P(_abi.FieldT), _vp, _P(_abi.FieldT),
                                               _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double,
                                               C.c_int]),
+    "oracle_particle_to_cell": (C.c_int, [_P(_abi.FieldT), _vp, _vp, _vp, _vp, _vp, C.c_int64,
+                                          _P(_abi.ParticleGeomT)]),
+    "oracle_cell_to_particle": (C.c_int, [_vp, _P(_abi.FieldT), _vp, _vp, _vp, _vp, C.c_int64,
+                                          _P(_abi.ParticleGeomT)]),
+    "oracle_point_to_field": (None, [_vp, _vp, _vp, C.c_int64]),
+    "oracle_field_to_point": (None, [_vp, _vp, _vp, C.c_int64]),
     "oracle_splitmix64": (C.c_uint64, [_P(C.c_uint64)]),
     "oracle_fill_uniform": (None, [_vp, C.c_size_t, C.c_uint64, C.c_double, C.c_double]),
     "oracle_fill_normal": (None, [_vp, C.c_size_t, C.c_uint64]),
@@ -140,6 +146,48 @@ def bc_apply(op, mask_points, phi, gamma, minus):
     check(lib().oracle_bc_apply(C.byref(d), s.ref, C.byref(st), C.addressof(po), C.addressof(go), fd,
                                 pts.ctypes.data, len(pts) // 3, int(bool(minus))))
     return phi
+
+
+def _arr(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def particle_to_cell(dest, src, px, py, pz, psize, geom):
+    """ParticleToCell::apply_to_field on host copies; geom = (dx, xlo, dy, ylo, dz, zlo)."""
+    keep = [_arr(v) for v in (src, px, py, pz, psize)]
+    ptr = [None if v is None else v.ctypes.data for v in keep]
+    g = _abi.ParticleGeomT(*[float(v) for v in geom])
+    d = dest.desc("host")
+    check(lib().oracle_particle_to_cell(C.byref(d), ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], len(keep[0]),
+                                        C.byref(g)))
+    return dest
+
+
+def cell_to_particle(src, px, py, pz, psize, geom, n):
+    keep = [_arr(v) for v in (px, py, pz, psize)]
+    ptr = [None if v is None else v.ctypes.data for v in keep]
+    out = np.zeros(n)
+    g = _abi.ParticleGeomT(*[float(v) for v in geom])
+    d = src.desc("host")
+    check(lib().oracle_cell_to_particle(out.ctypes.data, C.byref(d), ptr[0], ptr[1], ptr[2], ptr[3], n,
+                                        C.byref(g)))
+    return out
+
+
+def point_to_field(values, src, idx):
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    src = _arr(src)
+    lib().oracle_point_to_field(values.ctypes.data, src.ctypes.data, idx.ctypes.data, len(idx))
+
+
+def field_to_point(values, idx):
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = np.zeros(len(idx))
+    lib().oracle_field_to_point(out.ctypes.data, values.ctypes.data, idx.ctypes.data, len(idx))
+    return out
 
 
 def fill_periodic(f):

@@ -70,7 +70,7 @@ def test_mixed_field_types_do_not_compile(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prog", ["stencil_test", "mask_test", "heat_equation", "reduction_test", "bc_test",
-                                  "store_test"])
+                                  "store_test", "particle_test"])
 def test_cpp_program(prog):
     exe = os.path.join(BIN, prog)
     assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
