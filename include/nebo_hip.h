@@ -295,12 +295,15 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  * lane computing ty rows of one column with no LDS and no barrier, loads issued
  * two planes ahead; variant 4 is variant 3 computing each x face once (the
  * Laplacian kernel runs variant 3 for it); variants 5 and 6 stack ty/4 resp.
- * ty/2 such waves in y in one workgroup, which swap their boundary rows through
- * LDS (the Laplacian kernel runs variant 3 with 4 resp. 2 rows per lane for
- * them).  kernel 0 = the var-coefficient diffusion kernels
- * (nebo_divkgrad, nebo_diffusion_step), 1 = Laplacian+Euler.  The tile changes
- * only the schedule, never the results.  The reference has no counterpart (its
- * CUDA launch shape is fixed at 16x16, NeboLhs.h:413-427). */
+ * ty/2 such waves in y in one workgroup (4 resp. 2 rows per lane), so that a
+ * wave's halo rows are its neighbours' own rows, loaded on the same CU; variants
+ * 7 and 8 are variant 5 with loads issued 3 resp. 4 planes ahead (the Laplacian
+ * kernel runs variant 3 with the same rows per lane for 5-8).  Until a tile is
+ * set or tuned both kernels run 128 x 8 x 64, variant 7.  kernel 0 = the
+ * var-coefficient diffusion kernels (nebo_divkgrad, nebo_diffusion_step), 1 =
+ * Laplacian+Euler.  The tile changes only the schedule, never the results.  The
+ * reference has no counterpart (its CUDA launch shape is fixed at 16x16,
+ * NeboLhs.h:413-427). */
 int nebo_stencil_tiles(int32_t* n, int32_t* tx, int32_t* ty, int32_t* zchunk,
                        int32_t* variant);                               /* arrays of n */
 int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t zchunk, int32_t variant);

@@ -364,7 +364,14 @@ __device__ __forceinline__ void store_nt_at(Rsrc rs, uint32_t off, double v)
 // that only another CU loaded comes back through L2 at about the cost of an HBM
 // row.  (Measured at 256^3 with 4 rows per lane: the halo rows cost ~13 us of an
 // ~84 us launch with one wave per workgroup, ~3 us with four stacked.)
-template <int FORM, int W, int NW, int RY, int ZC, bool XR>
+// Run f(integral_constant<int, I>) for I = 0, 1, ... in order (compile-time steps).
+template <typename F, int... I>
+__device__ __forceinline__ void unroll_steps(F&& f, std::integer_sequence<int, I...>)
+{
+  (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int FORM, int W, int NW, int RY, int ZC, bool XR, int PF = 2>
 __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
     Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
@@ -405,10 +412,10 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
     return v;
   };
 
-  Col S[3];
-  S[0] = load(z0);
-  S[1] = load(min(z0 + 1, z1));
-  S[2] = load(min(z0 + 2, z1));
+  constexpr int NS = PF + 1;   // register slots: planes z .. z+PF
+  Col S[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
   double fz[RY];
   {
     const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z0 - 1), X, XY);
@@ -423,7 +430,7 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
     constexpr int jj = decltype(JJ)::value;
     {
       const Col& C0 = S[jj];              // plane z
-      const Col& C1 = S[(jj + 1) % 3];    // plane z+1
+      const Col& C1 = S[(jj + 1) % NS];   // plane z+1
       // the y face between rows j-1 and j is computed once, as row j-1's high face
       double fy_lo = face(C0.k[0], C0.k[1], C0.p[0], C0.p[1], c.iy0, c.iy1, c.gy0, c.gy1);
       // XR: the wave's end faces of every row (lane j: left face of row j, lane 32+j: right face)
@@ -463,19 +470,16 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
         const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
         if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(r, X, XY, z), X, XY), g.row[j + 1], v);
       }
-      S[jj] = load(min(z + 3, z1));       // plane z is consumed: its slot takes z+3
+      S[jj] = load(min(z + NS, z1));      // plane z is consumed: its slot takes z+NS
     }
   };
-  // whole periods of three steps, then the one or two steps left over
-  const int full = (z1 - z0) / 3, rem = (z1 - z0) - 3 * full;
+  // whole periods of NS steps, then the steps left over
+  const int full = (z1 - z0) / NS, rem = (z1 - z0) - NS * full;
   int z = z0;
-  for (int q = 0; q < full; ++q, z += 3) {
-    step(std::integral_constant<int, 0>{}, z);
-    step(std::integral_constant<int, 1>{}, z + 1);
-    step(std::integral_constant<int, 2>{}, z + 2);
-  }
-  if (rem > 0) step(std::integral_constant<int, 0>{}, z);
-  if (rem > 1) step(std::integral_constant<int, 1>{}, z + 1);
+  for (int q = 0; q < full; ++q, z += NS)
+    unroll_steps([&](auto J) { step(J, z + decltype(J)::value); }, std::make_integer_sequence<int, NS>{});
+  unroll_steps([&](auto J) { if (rem > decltype(J)::value) step(J, z + decltype(J)::value); },
+               std::make_integer_sequence<int, NS - 1>{});
 }
 
 // ---- phi + s*lapl(phi) -----------------------------------------------------------
@@ -673,24 +677,44 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
 // 4: column schedule with x-face reuse (the Laplacian kernel has no x-face
 //    reuse and runs its column schedule for both 3 and 4);
 // 5, 6: column schedule with x-face reuse, 4 resp. 2 rows per lane, ty/4 resp.
-//    ty/2 waves stacked in y per workgroup (ty = the workgroup's rows).
+//    ty/2 waves stacked in y per workgroup (ty = the workgroup's rows);
+// 7, 8: as 5 with loads issued 3 resp. 4 planes ahead instead of 2.
 struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
-    {64, 4, 16, 0},  {64, 8, 16, 0},
-    {64, 8, 16, 1},  {64, 8, 64, 1},  {64, 16, 32, 1},
+    {64, 4, 16, 0},   {64, 8, 16, 0},
+    {64, 8, 16, 1},   {64, 8, 64, 1},   {64, 16, 32, 1},
     {64, 8, 24, 2},
-    {64, 4, 16, 3},  {64, 4, 24, 3},  {64, 2, 32, 3},
-    {64, 4, 16, 4},  {64, 4, 24, 4},  {64, 4, 32, 4},  {64, 2, 32, 4},  {256, 4, 16, 4},
-    {64, 8, 32, 5},  {64, 12, 32, 5}, {64, 16, 16, 5}, {64, 16, 24, 5}, {64, 16, 32, 5},
-    {64, 16, 48, 5}, {64, 24, 32, 5}, {128, 16, 32, 5}, {64, 8, 32, 6}, {64, 16, 32, 6}};
+    {64, 4, 16, 3},   {64, 4, 24, 3},   {64, 2, 32, 3},
+    {64, 4, 16, 4},   {64, 4, 24, 4},   {64, 4, 32, 4},   {64, 2, 32, 4},   {256, 4, 16, 4},
+    {256, 4, 64, 4},
+    {64, 8, 32, 5},   {64, 16, 16, 5},  {64, 16, 32, 5},  {128, 8, 32, 5},  {128, 8, 64, 5},
+    {128, 16, 32, 5},
+    {64, 8, 32, 6},   {64, 16, 32, 6},
+    {128, 8, 32, 7},  {128, 8, 64, 7},  {64, 16, 64, 7},  {256, 4, 64, 7},  {128, 4, 128, 7},
+    {128, 8, 64, 8}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
-constexpr int kDefaultShape = 0;
+// The tile used until a caller sets or tunes one: 128 x 8 rows (2 x 2 waves)
+// marching 64 planes with loads 3 planes ahead, the fastest at 256^3 on MI355X
+// (about one workgroup per CU there).  Planes too large for the column
+// schedule's 32-bit offsets run the direct schedule (kFallbackShape).
+constexpr int find_shape_c(int tx, int ty, int zc, int variant, int s = 0)
+{
+  return s >= kNumShapes ? -1
+         : (kShapes[s].tx == tx && kShapes[s].ty == ty && kShapes[s].zc == zc && kShapes[s].variant == variant)
+             ? s
+             : find_shape_c(tx, ty, zc, variant, s + 1);
+}
+constexpr int kDefaultShape = find_shape_c(128, 8, 64, 7);
+constexpr int kFallbackShape = 0;
+static_assert(kDefaultShape >= 0 && kShapes[kFallbackShape].variant == 0, "default tiles");
 
 // Rows per lane and waves stacked in y of a column tile.
 constexpr int rows_per_lane(const TileShape& sh)
 {
-  return sh.variant == 5 ? 4 : sh.variant == 6 ? 2 : sh.ty;
+  return (sh.variant == 5 || sh.variant >= 7) ? 4 : sh.variant == 6 ? 2 : sh.ty;
 }
+// Planes of load prefetch of a column tile.
+constexpr int prefetch_planes(const TileShape& sh) { return sh.variant == 7 ? 3 : sh.variant == 8 ? 4 : 2; }
 constexpr int stacked_waves(const TileShape& sh) { return sh.ty / rows_per_lane(sh); }
 
 // Launch geometry of one tile: rows per tile in y and threads per workgroup.
@@ -711,7 +735,8 @@ constexpr DivkFn divk_fn()
   static_assert(sh.ty % rows_per_lane(sh) == 0 && tile_geometry(sh).threads <= 1024, "whole waves, <= 1024 threads");
   if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
   else if constexpr (sh.variant >= 3)
-    return divkgrad_column_kernel<FORM, sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc, sh.variant != 3>;
+    return divkgrad_column_kernel<FORM, sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc, sh.variant != 3,
+                                  prefetch_planes(sh)>;
   else return divkgrad_staged_kernel<FORM, sh.ty, sh.zc, sh.variant>;
 }
 
@@ -763,7 +788,7 @@ int find_shape(int tx, int ty, int zc, int variant)
 // default tile instead.
 int usable_shape(int shape, int X, int XY)
 {
-  if (kShapes[shape].variant >= 3 && ((long)XY + 2L * X + 2) * 8 >= (1L << 31)) return kDefaultShape;
+  if (kShapes[shape].variant >= 3 && ((long)XY + 2L * X + 2) * 8 >= (1L << 31)) return kFallbackShape;
   return shape;
 }
 

@@ -120,8 +120,8 @@ def stencil_tiles():
     """(tx, ty, zchunk, variant) tiles instantiated for the hand-written stencil
     kernels; variant 0 = direct loads, 1/2 = LDS-staged rows with 1/2 planes of
     prefetch, 3 = column schedule (tx/64 waves in x, ty rows per lane), 4 = column
-    schedule computing each x face once, 5/6 = ty/4 resp. ty/2 column waves
-    stacked in y sharing boundary rows through LDS."""
+    schedule computing each x face once, 5/6 = ty/4 resp. ty/2 column waves of 4
+    resp. 2 rows stacked in y, 7/8 = variant 5 with loads 3 resp. 4 planes ahead."""
     n = C.c_int32()
     _abi.call("nebo_stencil_tiles", C.byref(n), None, None, None, None)
     tx, ty, zc, var = ((C.c_int32 * n.value)() for _ in range(4))
