@@ -711,7 +711,7 @@ static_assert(kDefaultShape >= 0 && kShapes[kFallbackShape].variant == 0, "defau
 // Rows per lane and waves stacked in y of a column tile.
 constexpr int rows_per_lane(const TileShape& sh)
 {
-  return (sh.variant == 5 || sh.variant >= 7) ? 4 : sh.variant == 6 ? 2 : sh.ty;
+  return (sh.variant == 5 || sh.variant == 7 || sh.variant == 8) ? 4 : sh.variant == 6 ? 2 : sh.ty;
 }
 // Planes of load prefetch of a column tile.
 constexpr int prefetch_planes(const TileShape& sh) { return sh.variant == 7 ? 3 : sh.variant == 8 ? 4 : 2; }
