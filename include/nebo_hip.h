@@ -209,6 +209,20 @@ int nebo_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
 int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
                         int use_ghost, void* stream);
 
+/* The generic evaluator runs a tree as one kernel compiled at run time for that
+ * tree's shape (hiprtc, gfx950, the library's flags; cached per device and
+ * shape: field pointers, constants and sizes are arguments), or, when run-time
+ * compilation is switched off or fails, as the stack-program interpreter.  Both
+ * give the same bits.  No reference counterpart (the reference instantiates a
+ * kernel per expression type at its own compile time).
+ *   nebo_set_fused_jit(0 | 1): interpreter only / compiled kernels (default 1);
+ *   nebo_fused_jit_stats: kernels compiled, cache hits, failed compiles;
+ *   nebo_fused_source (host only): the kernel source generated for rhs
+ *   (*needed = bytes with the terminating 0; buf may be NULL). */
+int nebo_set_fused_jit(int enable);
+int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);
+int nebo_fused_source(const nebo_expr_t* rhs, char* buf, int32_t size, int32_t* needed);
+
 /* Which kernel nebo_assign would pick for this tree: 0 generic evaluator,
  * 1 var-coefficient diffusion, 2 Laplacian+Euler, 3 pointwise copy/scale.
  * Host-only. */

@@ -524,6 +524,9 @@ int launch_generic_assign(const nebo_field_t* lhs, const VmProgram& prog, int ma
                           const nebo_limits_t& lim, void* stream)
 {
   (void)max_depth;
+  bool done = false;
+  int rc = jit_assign(lhs, prog, lim, stream, &done);
+  if (rc || done) return rc;
   Box B = make_box(lim, kVmV);
   dim3 grid((B.nplane + kBlock - 1) / kBlock, B.zchunks);
   double* out = frame_origin(lhs);

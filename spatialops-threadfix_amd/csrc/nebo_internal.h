@@ -99,6 +99,12 @@ struct VmProgram {
 // Lower a tree to a stack program.  Also records the max stack depth.
 int lower_program(const nebo_expr_t* e, VmProgram* prog, int* max_depth);
 
+// Fused kernels compiled at run time (nebo_jit.hip): runs the program as one
+// generated kernel and sets *done, or leaves *done false when run-time
+// compilation is off or failed for this program (the interpreter then runs it).
+bool jit_enabled();
+int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done);
+
 // One (field, offset) read of an expression; edgeless when an edgeless stencil
 // lies between the read and the root.
 struct FieldRead {

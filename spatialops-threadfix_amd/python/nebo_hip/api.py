@@ -107,6 +107,29 @@ def lapl_euler(out: Field, phi: Field, coefs, s, stream=None):
     return out
 
 
+def set_fused_jit(enable: bool):
+    """Run generic trees as kernels compiled at run time (True, the default) or
+    on the stack-program interpreter (False); both give the same bits."""
+    _abi.call("nebo_set_fused_jit", int(bool(enable)))
+
+
+def fused_jit_stats():
+    """(kernels compiled, cache hits, failed compiles) of the run-time compiler."""
+    v = [C.c_int32() for _ in range(3)]
+    _abi.call("nebo_fused_jit_stats", *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def fused_source(rhs, where="host") -> str:
+    """The kernel source the run-time compiler generates for rhs."""
+    s = serialize(E(rhs), where)
+    need = C.c_int32()
+    _abi.call("nebo_fused_source", s.ref, None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value)
+    _abi.call("nebo_fused_source", s.ref, buf, need.value, C.byref(need))
+    return buf.value.decode()
+
+
 def device_count() -> int:
     n = C.c_int(0)
     _abi.call("nebo_device_count", C.byref(n))

@@ -191,8 +191,16 @@ def test_heat_equation_mask_step():
         assert np.array_equal(fetch(phi), href.host)
 
 
+@pytest.fixture(params=[True, False], ids=["compiled", "interpreter"])
+def fused_mode(request):
+    """Run a test on the kernels compiled at run time and on the interpreter."""
+    nh.set_fused_jit(request.param)
+    yield request.param
+    nh.set_fused_jit(True)
+
+
 @pytest.mark.parametrize("seed", range(12))
-def test_random_trees_bit_exact(seed):
+def test_random_trees_bit_exact(seed, fused_mode):
     rng = np.random.default_rng(seed)
     n = tuple(int(v) for v in rng.integers(3, 20, 3))
     L = nh.Layout.build(n, 2)
@@ -225,6 +233,49 @@ def test_random_trees_bit_exact(seed):
             assert ulp.max() <= 2.0
         else:
             assert np.array_equal(got, ref.host)
+
+
+def test_compiled_kernels_equal_interpreter():
+    """Random trees over every opcode (transcendentals, pow, masks, mask shifts,
+    edgeless stencils): the compiled kernel and the interpreter agree bit for bit,
+    and every tree compiled."""
+    import random
+    from test_fused_source import random_tree
+    n = (13, 9, 11)
+    L = nh.Layout.build(n, 2)
+    rng = np.random.default_rng(5)
+    f = both(L, rng.uniform(0.2, 2.0, L.npts))
+    g = both(L, rng.standard_normal(L.npts))
+    db = nh.build_stencils(*n, 1.0, 1.0, 1.0)
+    m = nh.Mask(L, rng.integers(-1, 5, (40, 3))).add_consumer(0)
+    stencils = [db[("Interpolant", "SVol", "XSurfY")], db[("GradientY", "SVol", "SVol")],
+                nh.Stencil([(1, 0, 0)], [2.0], kind=_abi.STENCIL_EDGELESS)]
+    r = random.Random(3)
+    c0, _, f0 = nh.fused_jit_stats()
+    trees = [random_tree(r, [f, g], stencils, [m, nh.mask_shift(m, (0, 1, 0))], 4) for _ in range(24)]
+    ran = 0
+    for i, t in enumerate(trees):
+        print(f"compiled-vs-interpreter tree {i}", flush=True)
+        try:
+            nh.lower_stats(t, "gpu")
+        except nh.NeboError:
+            continue
+        outs = []
+        for jit in (True, False):
+            nh.set_fused_jit(jit)
+            out = gpu_empty_like(L, 7.0)
+            try:
+                nh.assign(out, t)
+            except nh.NeboError:
+                out = None
+            outs.append(None if out is None else fetch(out))
+        nh.set_fused_jit(True)
+        assert (outs[0] is None) == (outs[1] is None)
+        if outs[0] is not None:
+            assert np.array_equal(outs[0], outs[1], equal_nan=True), i
+            ran += 1
+    c1, _, f1 = nh.fused_jit_stats()
+    assert ran >= 12 and f1 == f0 and c1 > c0
 
 
 def _all_operators():
