@@ -209,19 +209,24 @@ int nebo_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
 int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
                         int use_ghost, void* stream);
 
-/* The generic evaluator runs a tree as one kernel compiled at run time for that
- * tree's shape (hiprtc, gfx950, the library's flags; cached per device and
- * shape: field pointers, constants and sizes are arguments), or, when run-time
- * compilation is switched off or fails, as the stack-program interpreter.  Both
- * give the same bits.  No reference counterpart (the reference instantiates a
- * kernel per expression type at its own compile time).
+/* The generic evaluator and the reductions run a tree as one kernel compiled at
+ * run time for that tree's shape (hiprtc, gfx950, the library's flags; cached
+ * per device and shape: field pointers, constants and sizes are arguments), or,
+ * when run-time compilation is switched off or fails, as the stack-program
+ * interpreter.  Assignments give the same bits either way; reductions meet the
+ * same bounds (max/min exact, sum and norm within one rounding of the exact
+ * result) but may order their double-double partial sums differently.  No
+ * reference counterpart (the reference instantiates a kernel per expression
+ * type at its own compile time).
  *   nebo_set_fused_jit(0 | 1): interpreter only / compiled kernels (default 1);
  *   nebo_fused_jit_stats: kernels compiled, cache hits, failed compiles;
- *   nebo_fused_source (host only): the kernel source generated for rhs
+ *   nebo_fused_source (host only): the kernel source generated for rhs, the
+ *   assignment kernel for reduce_op -1, else the NEBO_RED_* reduction kernel
  *   (*needed = bytes with the terminating 0; buf may be NULL). */
 int nebo_set_fused_jit(int enable);
 int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);
-int nebo_fused_source(const nebo_expr_t* rhs, char* buf, int32_t size, int32_t* needed);
+int nebo_fused_source(const nebo_expr_t* rhs, int32_t reduce_op, char* buf, int32_t size,
+                      int32_t* needed);
 
 /* Which kernel nebo_assign would pick for this tree: 0 generic evaluator,
  * 1 var-coefficient diffusion, 2 Laplacian+Euler, 3 pointwise copy/scale.
@@ -259,7 +264,11 @@ enum nebo_reduce_op { NEBO_RED_MAX = 0, NEBO_RED_MIN = 1, NEBO_RED_SUM = 2, NEBO
  * of an expression.  Synchronises `stream` and writes the host scalar *out.
  * max/min are exact; sum/norm accumulate in double-double (error-free
  * transformations) and round once, so they are within one rounding of the
- * exact sum. */
+ * exact sum.  An expression that reads no field and no mask has no extent and
+ * "infinite" ghosts: with use_ghost=1 its region is unbounded (the reference's
+ * loop would walk [-GHOST_MAX, GHOST_MAX)^3) and the call fails with
+ * NEBO_ERR_INVALID; with use_ghost=0 the region is empty and *out is the
+ * fold's initial value (-inf, +inf, 0, 0). */
 int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
                 void* stream);
 

@@ -666,6 +666,11 @@ static int fold_limits(const nebo_expr_t* e, int use_ghost, nebo_limits_t* L)
   if (use_ghost) {
     int rc = oracle_node_ghosts(e, root, 0, &g);
     if (rc) return rc;
+    /* no field and no mask: "infinite" ghosts; the reference's loop would walk
+     * [-GHOST_MAX, GHOST_MAX)^3 (5.8e12 cells).  The engine refuses such a
+     * reduction, and so does the checker rather than run for hours. */
+    for (int d = 0; d < 3; ++d)
+      if (g.minus[d] >= NEBO_GHOST_MAX || g.plus[d] >= NEBO_GHOST_MAX) return NEBO_ERR_INVALID;
   }
   int32_t ext[3] = {0, 0, 0};
   node_extents(e, root, ext);

@@ -601,15 +601,38 @@ void launch_reduce_any(const VmProgram& prog, const Box& B, int op, int blocks, 
   }
 }
 
-int launch_reduce(const VmProgram& prog, const Box& B, int op, double* out_dev, hipStream_t s)
+void launch_reduce_final(int op, const double* partial, int blocks, double* out_dev, hipStream_t s)
 {
+  switch (op) {
+  case NEBO_RED_MAX: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MAX>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
+  case NEBO_RED_MIN: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MIN>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
+  case NEBO_RED_SUM: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_SUM>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
+  default: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_NORM>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
+  }
+}
+
+int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, double* out_dev, hipStream_t s)
+{
+  const Box B = make_box(lim, kVmV);
   const long items = (long)B.nplane * B.zchunks;
   int blocks = (int)((items + kBlock - 1) / kBlock);
   if (blocks > kRedMaxBlocks) blocks = kRedMaxBlocks;
   if (blocks < 1) blocks = 1;
   double* partial = nullptr;
-  int rc = check_hip(hipMallocAsync((void**)&partial, sizeof(double) * 2 * blocks, s), "Nebo Reduction");
+  // room for the largest grid of either kernel
+  int rc = check_hip(hipMallocAsync((void**)&partial, sizeof(double) * 2 * kRedMaxBlocks, s), "Nebo Reduction");
   if (rc) return rc;
+  bool done = false;
+  int jit_blocks = 0;
+  rc = jit_reduce(prog, lim, op, partial, kRedMaxBlocks, s, &jit_blocks, &done);
+  if (rc || done) {
+    if (done) {
+      launch_reduce_final(op, partial, jit_blocks, out_dev, s);
+      rc = check_hip(hipGetLastError(), "Nebo Reduction");
+    }
+    (void)hipFreeAsync(partial, s);
+    return rc;
+  }
   const bool one = prog.n_classes == 1, math = uses_math(prog);
   if (one && !math) launch_reduce_any<1, false>(prog, B, op, blocks, partial, out_dev, s);
   else if (one) launch_reduce_any<1, true>(prog, B, op, blocks, partial, out_dev, s);
@@ -644,8 +667,7 @@ int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void
   VmProgram prog;
   int depth = 0;
   if ((rc = lower_program(e, &prog, &depth))) return rc;
-  Box B = make_box(lim, kVmV);
-  return launch_reduce(prog, B, op, out_dev, s);
+  return launch_reduce(prog, lim, op, out_dev, s);
 }
 
 }  // namespace

@@ -104,6 +104,11 @@ int lower_program(const nebo_expr_t* e, VmProgram* prog, int* max_depth);
 // compilation is off or failed for this program (the interpreter then runs it).
 bool jit_enabled();
 int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done);
+// Reduction: writes one (hi, lo) partial per block into partial (room for
+// max_blocks) and the block count into *blocks; the caller folds the partials
+// with reduce_final_kernel.
+int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks, void* stream,
+               int* blocks, bool* done);
 
 // One (field, offset) read of an expression; edgeless when an edgeless stencil
 // lies between the read and the root.

@@ -14,6 +14,15 @@
 // compiler rewrite pow(x, 2.0), which the interpreter evaluates through the
 // math library.
 //
+// The argument struct is handed to hipModuleLaunchKernel as the kernel's
+// argument buffer (HIP_LAUNCH_PARAM_BUFFER_POINTER), which is passed to the
+// kernel as it is: the hidden arguments that follow the explicit ones in the
+// kernarg segment (block counts behind gridDim, group sizes behind blockDim,
+// global offsets) are not filled in.  Generated code therefore reads none of
+// gridDim, blockDim or the global offsets; the grid size a grid-stride loop
+// needs travels in JitArgs::nblocks.  Only blockIdx and threadIdx (hardware
+// registers) are used.
+//
 // If hiprtc is unavailable or a compile fails, nebo_assign runs the
 // interpreter instead (also a GPU kernel) and the failure is counted.
 
@@ -39,7 +48,7 @@ namespace {
   int oXY;                                                                  \
   int lo0, lo1, lo2, n0, n1, n2;                                            \
   int nplane;                                                               \
-  int pad0;                                                                 \
+  int nzc;                                                                  \
   int clsX[4];                                                              \
   int clsXY[4];                                                             \
   const double* base[40];                                                   \
@@ -49,7 +58,7 @@ namespace {
   int mo[6][3];                                                             \
   int mX[6];                                                                \
   int mY[6];                                                                \
-  int pad1;                                                                 \
+  int nblocks;                                                              \
   double c[48];
 
 struct JitArgs {
@@ -74,8 +83,10 @@ struct Entry {
 std::mutex g_mu;
 std::unordered_map<std::string, Entry> g_cache;   // key: device + source
 
-// Straight-line source for one program and V cells per thread.
-std::string jit_source(const VmProgram& P, int V)
+// Index set-up, reads and operations for V cells of one z column at (x, y,
+// z0 .. z{V-1}); the generated code expects x, y and the z variables in scope
+// and returns the name of each cell's value.
+std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, int V, const std::string& ind)
 {
   // which slots are read, and whether every read of a slot is bounds-checked
   bool used[kVmMaxLoads] = {}, guarded[kVmMaxLoads], mused[kVmMaxMasks] = {};
@@ -86,48 +97,38 @@ std::string jit_source(const VmProgram& P, int V)
     if (op == VM_LOADG) used[imm] = true;
     if (op == VM_MASK) mused[imm] = true;
   }
-  std::ostringstream o;
-  o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
-  o << "extern \"C\" __global__ __launch_bounds__(" << kJitBlock << ") void nebo_fused(const JitArgs A)\n{\n";
-  o << "  const int t = (int)blockIdx.x * " << kJitBlock << " + (int)threadIdx.x;\n";
-  o << "  if (t >= A.nplane) return;\n";
-  o << "  const int yy = t / A.n0;\n  const int x = A.lo0 + (t - yy * A.n0);\n  const int y = A.lo1 + yy;\n";
-  o << "  const int zb = A.lo2 + (int)blockIdx.y * " << V << ";\n";
-  o << "  const int left = A.lo2 + A.n2 - zb;\n  const int nv = left < " << V << " ? left : " << V << ";\n";
-  for (int v = 0; v < V; ++v) {
-    // cells past the box end repeat the last cell (in bounds); only stores are guarded
-    o << "  const int z" << v << " = zb + (" << v << " < nv ? " << v << " : nv - 1);\n";
+  for (int v = 0; v < V; ++v)
     for (int k = 0; k < P.n_classes; ++k)
-      o << "  const int i" << v << "_" << k << " = x + A.clsX[" << k << "] * y + A.clsXY[" << k << "] * z" << v
-        << ";\n";
-  }
+      o << ind << "const int i" << v << "_" << k << " = x + A.clsX[" << k << "] * y + A.clsXY[" << k << "] * z"
+        << v << ";\n";
   // phase 1: every read, for every cell
   for (int v = 0; v < V; ++v) {
     for (int s = 0; s < kVmMaxLoads; ++s) {
       if (!used[s]) continue;
       const std::string idx = "i" + std::to_string(v) + "_" + std::to_string(P.loads[s].cls);
       if (guarded[s])
-        o << "  double L" << v << "_" << s << " = __builtin_nan(\"\");\n  if (" << idx << " >= A.glo[" << s
-          << "] && " << idx << " < A.ghi[" << s << "]) L" << v << "_" << s << " = A.base[" << s << "][" << idx
-          << "];\n";
+        o << ind << "double L" << v << "_" << s << " = __builtin_nan(\"\");\n" << ind << "if (" << idx
+          << " >= A.glo[" << s << "] && " << idx << " < A.ghi[" << s << "]) L" << v << "_" << s << " = A.base["
+          << s << "][" << idx << "];\n";
       else
-        o << "  const double L" << v << "_" << s << " = A.base[" << s << "][" << idx << "];\n";
+        o << ind << "const double L" << v << "_" << s << " = A.base[" << s << "][" << idx << "];\n";
     }
     for (int m = 0; m < kVmMaxMasks; ++m) {
       if (!mused[m]) continue;
-      o << "  const int q" << v << "_" << m << " = (A.mo[" << m << "][0] + x) + A.mX[" << m << "] * ((A.mo[" << m
-        << "][1] + y) + A.mY[" << m << "] * (A.mo[" << m << "][2] + z" << v << "));\n";
-      o << "  const double M" << v << "_" << m << " = ((A.mbits[" << m << "][q" << v << "_" << m << " >> 5] >> (q"
-        << v << "_" << m << " & 31)) & 1u) ? 1.0 : 0.0;\n";
+      o << ind << "const int q" << v << "_" << m << " = (A.mo[" << m << "][0] + x) + A.mX[" << m
+        << "] * ((A.mo[" << m << "][1] + y) + A.mY[" << m << "] * (A.mo[" << m << "][2] + z" << v << "));\n";
+      o << ind << "const double M" << v << "_" << m << " = ((A.mbits[" << m << "][q" << v << "_" << m
+        << " >> 5] >> (q" << v << "_" << m << " & 31)) & 1u) ? 1.0 : 0.0;\n";
     }
   }
   // phase 2: the program, one SSA temporary per result
+  std::vector<std::string> results;
   for (int v = 0; v < V; ++v) {
     std::vector<std::string> st;
     int tmp = 0;
     auto fresh = [&](const std::string& expr) {
       const std::string name = "t" + std::to_string(v) + "_" + std::to_string(tmp++);
-      o << "  const double " << name << " = " << expr << ";\n";
+      o << ind << "const double " << name << " = " << expr << ";\n";
       st.push_back(name);
     };
     auto pop = [&]() {
@@ -193,11 +194,95 @@ std::string jit_source(const VmProgram& P, int V)
       }
       }
     }
-    // a bare leaf has no temporary: give the result a name
-    const std::string r = st.empty() ? std::string("0.0") : st.back();
-    o << "  if (" << v << " < nv) __builtin_nontemporal_store(" << r << ", &A.out[x + A.oX * y + A.oXY * z" << v
-      << "]);\n";
+    // a bare leaf has no temporary of its own; the leaf's name is the value
+    results.push_back(st.empty() ? std::string("0.0") : st.back());
   }
+  return results;
+}
+
+// Assignment kernel: one thread per (x, y) point of the box and V z cells.
+std::string jit_source(const VmProgram& P, int V)
+{
+  std::ostringstream o;
+  o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
+  o << "extern \"C\" __global__ __launch_bounds__(" << kJitBlock << ") void nebo_fused(const JitArgs A)\n{\n";
+  o << "  const int t = (int)blockIdx.x * " << kJitBlock << " + (int)threadIdx.x;\n";
+  o << "  if (t >= A.nplane) return;\n";
+  o << "  const int yy = t / A.n0;\n  const int x = A.lo0 + (t - yy * A.n0);\n  const int y = A.lo1 + yy;\n";
+  o << "  const int zb = A.lo2 + (int)blockIdx.y * " << V << ";\n";
+  o << "  const int left = A.lo2 + A.n2 - zb;\n  const int nv = left < " << V << " ? left : " << V << ";\n";
+  // cells past the box end repeat the last cell (in bounds); only stores are guarded
+  for (int v = 0; v < V; ++v)
+    o << "  const int z" << v << " = zb + (" << v << " < nv ? " << v << " : nv - 1);\n";
+  const std::vector<std::string> r = emit_cells(o, P, V, "  ");
+  for (int v = 0; v < V; ++v)
+    o << "  if (" << v << " < nv) __builtin_nontemporal_store(" << r[v] << ", &A.out[x + A.oX * y + A.oXY * z" << v
+      << "]);\n";
+  o << "}\n";
+  return o.str();
+}
+
+// Reduction kernel: a grid-stride loop over (x, y, z chunk) items, V z cells
+// per item, each thread folding its cells into a double-double (sum, norm) or
+// a select-form max/min exactly as the interpreter's vm_reduce_kernel does;
+// then a fixed-order block reduction writes one (hi, lo) partial per block to
+// A.out, which reduce_final_kernel folds.
+std::string jit_reduce_source(const VmProgram& P, int V, int op)
+{
+  std::ostringstream o;
+  o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
+  const char* init = op == NEBO_RED_MAX ? "-__builtin_inf()" : op == NEBO_RED_MIN ? "__builtin_inf()" : "0.0";
+  const bool dd = op == NEBO_RED_SUM || op == NEBO_RED_NORM;
+  // combine (hi, lo) with (h2, l2) into (hi, lo): TwoSum merge, or the select forms
+  std::string combine;
+  if (op == NEBO_RED_MAX) combine = "hi = (hi < h2) ? h2 : hi;";
+  else if (op == NEBO_RED_MIN) combine = "hi = (hi > h2) ? h2 : hi;";
+  else
+    combine = "{ const double s = hi + h2; const double bb = s - hi; "
+              "const double e = ((hi - (s - bb)) + (h2 - bb)) + (lo + l2); "
+              "const double h = s + e; lo = e - (h - s); hi = h; }";
+  o << "extern \"C\" __global__ __launch_bounds__(" << kJitBlock << ") void nebo_reduce(const JitArgs A)\n{\n";
+  o << "  double hi = " << init << ", lo = 0.0;\n";
+  o << "  const long items = (long)A.nplane * A.nzc;\n";
+  o << "  for (long w = (long)blockIdx.x * " << kJitBlock << " + threadIdx.x; w < items; w += (long)A.nblocks * "
+    << kJitBlock << ") {\n";
+  o << "    const int zc = (int)(w / A.nplane);\n    const int t = (int)(w - (long)zc * A.nplane);\n";
+  o << "    const int yy = t / A.n0;\n    const int x = A.lo0 + (t - yy * A.n0);\n    const int y = A.lo1 + yy;\n";
+  o << "    const int zb = A.lo2 + zc * " << V << ";\n";
+  o << "    const int left = A.lo2 + A.n2 - zb;\n    const int nv = left < " << V << " ? left : " << V << ";\n";
+  for (int v = 0; v < V; ++v)
+    o << "    const int z" << v << " = zb + (" << v << " < nv ? " << v << " : nv - 1);\n";
+  const std::vector<std::string> r = emit_cells(o, P, V, "    ");
+  for (int v = 0; v < V; ++v) {
+    o << "    if (" << v << " < nv) {\n      const double xv = " << r[v] << ";\n";
+    if (op == NEBO_RED_MAX) o << "      hi = (hi < xv) ? xv : hi;\n";
+    else if (op == NEBO_RED_MIN) o << "      hi = (hi > xv) ? xv : hi;\n";
+    else if (op == NEBO_RED_SUM)
+      o << "      const double s = hi + xv; const double bb = s - hi;\n"
+           "      lo = lo + ((hi - (s - bb)) + (xv - bb)); hi = s;\n";
+    else
+      // x*x exactly as p + err (Dekker split, no FMA)
+      o << "      const double p = xv * xv; const double t2 = 134217729.0 * xv;\n"
+           "      const double xh = t2 - (t2 - xv), xl = xv - xh;\n"
+           "      const double err = ((xh * xh - p) + 2.0 * xh * xl) + xl * xl;\n"
+           "      const double s = hi + p; const double bb = s - hi;\n"
+           "      lo = lo + ((hi - (s - bb)) + (p - bb)); hi = s; lo = lo + err;\n";
+    o << "    }\n";
+  }
+  o << "  }\n";
+  // fixed-order block reduction: xor-shuffle tree inside each wave, then the waves in order
+  o << "  for (int off = 32; off >= 1; off >>= 1) {\n";
+  o << "    const double h2 = __shfl_xor(hi, off, 64);\n";
+  o << "    const double l2 = " << (dd ? "__shfl_xor(lo, off, 64)" : "0.0") << ";\n";
+  o << "    " << combine << "\n  }\n";
+  o << "  __shared__ double sh[" << 2 * (kJitBlock / 64) << "];\n";
+  o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
+  o << "  if (lane == 0) { sh[2 * wave] = hi; sh[2 * wave + 1] = lo; }\n";
+  o << "  __syncthreads();\n";
+  o << "  if (threadIdx.x == 0) {\n    hi = sh[0]; lo = sh[1];\n";
+  o << "    for (int w = 1; w < " << kJitBlock / 64 << "; ++w) {\n";
+  o << "      const double h2 = sh[2 * w], l2 = sh[2 * w + 1];\n      " << combine << "\n    }\n";
+  o << "    A.out[2 * blockIdx.x] = hi;\n    A.out[2 * blockIdx.x + 1] = " << (dd ? "lo" : "0.0") << ";\n  }\n";
   o << "}\n";
   return o.str();
 }
@@ -216,7 +301,7 @@ int reads_of(const VmProgram& P)
   return n;
 }
 
-int compile(const std::string& src, int device, Entry* e)
+int compile(const std::string& src, const char* name, int device, Entry* e)
 {
   static const char* ctx = "Nebo Fused Kernel";
   hipDeviceProp_t prop;
@@ -244,12 +329,86 @@ int compile(const std::string& src, int device, Entry* e)
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
   if ((rc = check_hip(hipModuleLoadData(&e->mod, code.data()), ctx))) return rc;
-  if ((rc = check_hip(hipModuleGetFunction(&e->fn, e->mod, "nebo_fused"), ctx))) {
+  if ((rc = check_hip(hipModuleGetFunction(&e->fn, e->mod, name), ctx))) {
     (void)hipModuleUnload(e->mod);
     e->mod = nullptr;
     return rc;
   }
   return NEBO_OK;
+}
+
+// Compiled kernel for src, from the cache or compiled now; fn stays null when
+// compilation failed (remembered, so the interpreter runs such trees without
+// another attempt).
+int get_kernel(const std::string& src, const char* name, int V, Entry* out)
+{
+  static const char* ctx = "Nebo Fused Kernel";
+  int device = 0;
+  int rc = check_hip(hipGetDevice(&device), ctx);
+  if (rc) return rc;
+  const std::string key = std::to_string(device) + "\n" + src;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_cache.find(key);
+  if (it != g_cache.end()) {
+    *out = it->second;
+    ++g_hits;
+    return NEBO_OK;
+  }
+  Entry fresh;
+  fresh.V = V;
+  if (compile(src, name, device, &fresh)) {
+    ++g_failures;
+    clear_error();
+    fresh = Entry{};
+  } else {
+    ++g_compiled;
+  }
+  g_cache[key] = fresh;
+  *out = fresh;
+  return NEBO_OK;
+}
+
+int cells_per_thread(const VmProgram& P)
+{
+  const int nr = reads_of(P);
+  return nr <= 6 ? 4 : nr <= 16 ? 2 : 1;
+}
+
+void fill_args(const VmProgram& P, const nebo_limits_t& lim, int V, JitArgs* A)
+{
+  std::memset(A, 0, sizeof *A);
+  A->lo0 = lim.lo[0];
+  A->lo1 = lim.lo[1];
+  A->lo2 = lim.lo[2];
+  A->n0 = lim.hi[0] - lim.lo[0];
+  A->n1 = lim.hi[1] - lim.lo[1];
+  A->n2 = lim.hi[2] - lim.lo[2];
+  A->nplane = A->n0 * A->n1;
+  A->nzc = (A->n2 + V - 1) / V;
+  for (int k = 0; k < kVmMaxClasses; ++k) {
+    A->clsX[k] = P.cls_X[k];
+    A->clsXY[k] = P.cls_XY[k];
+  }
+  for (int s = 0; s < kVmMaxLoads; ++s) {
+    A->base[s] = P.loads[s].base;
+    A->glo[s] = P.loads[s].glo;
+    A->ghi[s] = P.loads[s].ghi;
+  }
+  for (int m = 0; m < kVmMaxMasks; ++m) {
+    A->mbits[m] = P.masks[m].bits;
+    for (int d = 0; d < 3; ++d) A->mo[m][d] = P.masks[m].o[d];
+    A->mX[m] = P.masks[m].X;
+    A->mY[m] = P.masks[m].Y;
+  }
+  for (int c = 0; c < kVmMaxConsts; ++c) A->c[c] = P.consts[c];
+}
+
+int launch(const Entry& e, JitArgs* A, unsigned gx, unsigned gy, void* stream)
+{
+  size_t bytes = sizeof *A;
+  void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &bytes, HIP_LAUNCH_PARAM_END};
+  return check_hip(hipModuleLaunchKernel(e.fn, gx, gy, 1, kJitBlock, 1, 1, 0, (hipStream_t)stream, nullptr, cfg),
+                   "Nebo Fused Kernel");
 }
 
 }  // namespace
@@ -258,80 +417,50 @@ bool jit_enabled() { return g_mode.load() != 0; }
 
 int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done)
 {
-  static const char* ctx = "Nebo Fused Kernel";
   *done = false;
   if (!jit_enabled()) return NEBO_OK;
-  const int nr = reads_of(P);
-  const int V = nr <= 6 ? 4 : nr <= 16 ? 2 : 1;
-  const std::string src = jit_source(P, V);
-  int device = 0;
-  int rc = check_hip(hipGetDevice(&device), ctx);
-  if (rc) return rc;
-  const std::string key = std::to_string(device) + "\n" + src;
+  const int V = cells_per_thread(P);
   Entry e;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_cache.find(key);
-    if (it != g_cache.end()) {
-      e = it->second;
-      ++g_hits;
-    } else {
-      Entry fresh;
-      fresh.V = V;
-      if (compile(src, device, &fresh)) {
-        ++g_failures;
-        clear_error();
-        g_cache[key] = fresh;   // fn == nullptr: remembered failure, the interpreter runs it
-        return NEBO_OK;
-      }
-      g_cache[key] = fresh;
-      e = fresh;
-      ++g_compiled;
-    }
-  }
-  if (!e.fn) return NEBO_OK;
+  int rc = get_kernel(jit_source(P, V), "nebo_fused", V, &e);
+  if (rc || !e.fn) return rc;
   JitArgs A;
-  std::memset(&A, 0, sizeof A);
+  fill_args(P, lim, V, &A);
   A.out = frame_origin(lhs);
   A.oX = lhs->glob[0];
   A.oXY = lhs->glob[0] * lhs->glob[1];
-  A.lo0 = lim.lo[0];
-  A.lo1 = lim.lo[1];
-  A.lo2 = lim.lo[2];
-  A.n0 = lim.hi[0] - lim.lo[0];
-  A.n1 = lim.hi[1] - lim.lo[1];
-  A.n2 = lim.hi[2] - lim.lo[2];
-  A.nplane = A.n0 * A.n1;
-  for (int k = 0; k < kVmMaxClasses; ++k) {
-    A.clsX[k] = P.cls_X[k];
-    A.clsXY[k] = P.cls_XY[k];
-  }
-  for (int s = 0; s < kVmMaxLoads; ++s) {
-    A.base[s] = P.loads[s].base;
-    A.glo[s] = P.loads[s].glo;
-    A.ghi[s] = P.loads[s].ghi;
-  }
-  for (int m = 0; m < kVmMaxMasks; ++m) {
-    A.mbits[m] = P.masks[m].bits;
-    for (int d = 0; d < 3; ++d) A.mo[m][d] = P.masks[m].o[d];
-    A.mX[m] = P.masks[m].X;
-    A.mY[m] = P.masks[m].Y;
-  }
-  for (int c = 0; c < kVmMaxConsts; ++c) A.c[c] = P.consts[c];
-  size_t bytes = sizeof A;
-  void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &bytes, HIP_LAUNCH_PARAM_END};
   const unsigned gx = (unsigned)((A.nplane + kJitBlock - 1) / kJitBlock);
-  const unsigned gy = (unsigned)((A.n2 + e.V - 1) / e.V);
-  rc = check_hip(hipModuleLaunchKernel(e.fn, gx, gy, 1, kJitBlock, 1, 1, 0, (hipStream_t)stream, nullptr, cfg),
-                 ctx);
-  if (!rc) *done = true;
-  return rc;
+  if ((rc = launch(e, &A, gx, (unsigned)A.nzc, stream))) return rc;
+  *done = true;
+  return NEBO_OK;
 }
 
-std::string jit_source_for(const VmProgram& P)
+int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks, void* stream,
+               int* blocks, bool* done)
 {
-  const int nr = reads_of(P);
-  return jit_source(P, nr <= 6 ? 4 : nr <= 16 ? 2 : 1);
+  *done = false;
+  if (!jit_enabled()) return NEBO_OK;
+  const int V = cells_per_thread(P);
+  Entry e;
+  int rc = get_kernel(jit_reduce_source(P, V, op), "nebo_reduce", V, &e);
+  if (rc || !e.fn) return rc;
+  JitArgs A;
+  fill_args(P, lim, V, &A);
+  A.out = partial;
+  const long items = (long)A.nplane * A.nzc;
+  long nb = (items + kJitBlock - 1) / kJitBlock;
+  if (nb > max_blocks) nb = max_blocks;
+  if (nb < 1) nb = 1;
+  A.nblocks = (int)nb;
+  if ((rc = launch(e, &A, (unsigned)nb, 1, stream))) return rc;
+  *blocks = (int)nb;
+  *done = true;
+  return NEBO_OK;
+}
+
+std::string jit_source_for(const VmProgram& P, int reduce_op)
+{
+  const int V = cells_per_thread(P);
+  return reduce_op < 0 ? jit_source(P, V) : jit_reduce_source(P, V, reduce_op);
 }
 
 }  // namespace nebo
@@ -352,15 +481,18 @@ extern "C" int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int3
   return NEBO_OK;
 }
 
-extern "C" int nebo_fused_source(const nebo_expr_t* rhs, char* buf, int32_t size, int32_t* needed)
+extern "C" int nebo_fused_source(const nebo_expr_t* rhs, int32_t reduce_op, char* buf, int32_t size,
+                                 int32_t* needed)
 {
   nebo::clear_error();
+  if (reduce_op != -1 && (reduce_op < NEBO_RED_MAX || reduce_op > NEBO_RED_NORM))
+    return nebo::set_error(NEBO_ERR_INVALID, "Nebo Fused Kernel", "unknown reduction");
   int rc = nebo::validate_expr(rhs, "Nebo Fused Kernel");
   if (rc) return rc;
   nebo::VmProgram prog;
   int depth = 0;
   if ((rc = nebo::lower_program(rhs, &prog, &depth))) return rc;
-  const std::string s = nebo::jit_source_for(prog);
+  const std::string s = nebo::jit_source_for(prog, reduce_op);
   if (needed) *needed = (int32_t)s.size() + 1;
   if (buf && size > 0) {
     const size_t n = s.size() < (size_t)size - 1 ? s.size() : (size_t)size - 1;

@@ -307,6 +307,14 @@ int reduce_limits(const nebo_expr_t* e, int use_ghost, nebo_limits_t* lim, int* 
   if (use_ghost && (rc = node_ghosts(e, e->n_nodes - 1, false, &g))) return rc;
   int ext[3] = {0, 0, 0};
   node_extents(e, e->n_nodes - 1, ext);
+  // A tree that reads no field and no mask has "infinite" ghosts and no extent:
+  // the reference's fold (NeboReductions.h:58-90) would walk
+  // [-GHOST_MAX, GHOST_MAX)^3, 5.8e12 cells of one constant.  Refused here.
+  for (int d = 0; d < 3; ++d)
+    if (use_ghost && (g.m[d] >= NEBO_GHOST_MAX || g.p[d] >= NEBO_GHOST_MAX))
+      return set_error(NEBO_ERR_INVALID, "Nebo Reduction",
+                       "the expression reads no field or mask, so its region with ghost cells is unbounded; "
+                       "reduce over the interior or include a field");
   bool vol = true;
   for (int d = 0; d < 3; ++d) {
     lim->lo[d] = -g.m[d];

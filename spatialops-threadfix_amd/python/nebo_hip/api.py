@@ -120,13 +120,16 @@ def fused_jit_stats():
     return tuple(x.value for x in v)
 
 
-def fused_source(rhs, where="host") -> str:
-    """The kernel source the run-time compiler generates for rhs."""
+def fused_source(rhs, where="host", reduce_op=None) -> str:
+    """The kernel source the run-time compiler generates for rhs: the
+    assignment kernel, or with reduce_op (RED_MAX, RED_MIN, RED_SUM, RED_NORM)
+    the reduction kernel."""
     s = serialize(E(rhs), where)
+    op = -1 if reduce_op is None else int(reduce_op)
     need = C.c_int32()
-    _abi.call("nebo_fused_source", s.ref, None, 0, C.byref(need))
+    _abi.call("nebo_fused_source", s.ref, op, None, 0, C.byref(need))
     buf = C.create_string_buffer(need.value)
-    _abi.call("nebo_fused_source", s.ref, buf, need.value, C.byref(need))
+    _abi.call("nebo_fused_source", s.ref, op, buf, need.value, C.byref(need))
     return buf.value.decode()
 
 

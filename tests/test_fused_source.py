@@ -1,10 +1,19 @@
 """The kernels the run-time compiler generates for generic trees compile for
 gfx950 with the library's flags (hiprtc works without a GPU), for random trees
 over every opcode, masks, mask shifts and edgeless stencils.  The GPU tests
-check their results against the oracle bit for bit."""
+check their results against the oracle bit for bit.
+
+The library launches these kernels with their argument struct as the whole
+argument buffer, so a kernel that reads hidden arguments (gridDim, blockDim,
+global offsets) would read memory the launch never filled; every compiled
+kernel's code object must therefore declare a kernarg segment of exactly the
+struct's size."""
 import ctypes as C
 import os
 import random
+import re
+import subprocess
+import tempfile
 
 import pytest
 
@@ -23,7 +32,11 @@ def rtc():
     return C.CDLL(HIPRTC)
 
 
-def compile_src(rtc, src):
+READELF = "/opt/rocm/llvm/bin/llvm-readelf"
+ARGS_BYTES = 1280   # sizeof(JitArgs) in csrc/nebo_jit.hip
+
+
+def compile_src(rtc, src, want_code=False):
     prog = C.c_void_p()
     assert rtc.hiprtcCreateProgram(C.byref(prog), src.encode(), b"nebo_fused.hip", 0, None, None) == 0
     opts = (C.c_char_p * len(OPTS))(*OPTS)
@@ -33,8 +46,34 @@ def compile_src(rtc, src):
     log = C.create_string_buffer(n.value + 1)
     if n.value:
         rtc.hiprtcGetProgramLog(prog, log)
+    code = None
+    if rc == 0 and want_code:
+        sz = C.c_size_t()
+        rtc.hiprtcGetCodeSize(prog, C.byref(sz))
+        buf = C.create_string_buffer(sz.value)
+        rtc.hiprtcGetCode(prog, buf)
+        code = buf.raw
     rtc.hiprtcDestroyProgram(C.byref(prog))
+    if want_code:
+        return rc, log.value.decode(errors="replace"), code
     return rc, log.value.decode(errors="replace")
+
+
+def kernarg_sizes(code):
+    """Kernarg segment size of every kernel in a code object, from its metadata note."""
+    with tempfile.NamedTemporaryFile(suffix=".co") as t:
+        t.write(code)
+        t.flush()
+        notes = subprocess.run([READELF, "--notes", t.name], capture_output=True, text=True, check=True).stdout
+    return [int(v) for v in re.findall(r"\.kernarg_segment_size:\s+(\d+)", notes)]
+
+
+def check_no_hidden_args(rtc, src):
+    rc, log, code = compile_src(rtc, src, want_code=True)
+    assert rc == 0, log + "\n" + src
+    if os.path.exists(READELF):
+        sizes = kernarg_sizes(code)
+        assert sizes == [ARGS_BYTES], (sizes, src)
 
 
 def host(L):
@@ -87,7 +126,28 @@ def test_every_opcode_compiles(rtc):
         except nh.NeboError as e:   # beyond the evaluator's limits: refused before any kernel
             assert e.code == _abi.NEBO_ERR_TOO_LARGE
             continue
-        rc, log = compile_src(rtc, src)
-        assert rc == 0, log + "\n" + src
+        check_no_hidden_args(rtc, src)
         compiled += 1
     assert compiled >= 10
+
+
+@pytest.mark.parametrize("op", [_abi.RED_MAX, _abi.RED_MIN, _abi.RED_SUM, _abi.RED_NORM])
+def test_reduction_kernels_compile(rtc, op):
+    n = (6, 5, 4)
+    L = nh.Layout.build(n, 1)
+    f, g = host(L), host(L)
+    db = nh.build_stencils(*n, 1, 1, 1)
+    m = nh.Mask(L, [(0, 0, 0)])
+    grad = db[("Gradient", "SVol", "SSurfX")]
+    trees = [nh.E(f), f * g + 1.0, nh.cond(m, f)(g), grad(f) * 2.0, nh.sqrt(nh.abs_(f - g))]
+    for t in trees:
+        src = nh.fused_source(t, reduce_op=op)
+        assert "nebo_reduce" in src
+        check_no_hidden_args(rtc, src)
+
+
+def test_unknown_reduction_refused():
+    L = nh.Layout.build((4, 4, 4), 1)
+    with pytest.raises(nh.NeboError) as ei:
+        nh.fused_source(nh.E(host(L)), reduce_op=9)
+    assert ei.value.code == _abi.NEBO_ERR_INVALID

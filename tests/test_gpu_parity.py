@@ -278,6 +278,64 @@ def test_compiled_kernels_equal_interpreter():
     assert ran >= 12 and f1 == f0 and c1 > c0
 
 
+def test_compiled_reductions_match_interpreter():
+    """Reductions of random trees on the compiled kernels and on the interpreter:
+    max and min equal; sum and norm within one rounding of each other's
+    double-double result (both fold cells into (hi, lo) pairs, in different
+    orders); every reduction kernel compiled."""
+    import random
+    from test_fused_source import random_tree
+    n = (17, 9, 11)
+    L = nh.Layout.build(n, 2)
+    rng = np.random.default_rng(6)
+    f = both(L, rng.uniform(0.2, 2.0, L.npts))
+    g = both(L, rng.standard_normal(L.npts))
+    db = nh.build_stencils(*n, 1.0, 1.0, 1.0)
+    m = nh.Mask(L, rng.integers(-1, 5, (40, 3))).add_consumer(0)
+    stencils = [db[("Interpolant", "SVol", "XSurfY")], db[("GradientY", "SVol", "SVol")],
+                nh.Stencil([(1, 0, 0)], [2.0], kind=_abi.STENCIL_EDGELESS)]
+    r = random.Random(8)
+    c0, _, f0 = nh.fused_jit_stats()
+    trees = [nh.E(g), f * g + 1.0, stencils[1](g) * f]
+    trees += [random_tree(r, [f, g], stencils, [m, nh.mask_shift(m, (0, 1, 0))], 3) for _ in range(14)]
+    ran = 0
+    for i, t in enumerate(trees):
+        print(f"compiled-vs-interpreter reductions, tree {i}", flush=True)
+        try:
+            nh.lower_stats(t, "gpu")
+        except nh.NeboError:
+            continue
+        for use_ghost in (True, False):
+            got = {}
+            for jit in (True, False):
+                nh.set_fused_jit(jit)
+                try:
+                    got[jit] = [fn(t) for fn in ((nh.field_max, nh.field_min, nh.field_sum, nh.field_norm)
+                                                 if use_ghost else
+                                                 (nh.field_max_interior, nh.field_min_interior,
+                                                  nh.field_sum_interior, nh.field_norm_interior))]
+                except nh.NeboError:
+                    got[jit] = None
+            nh.set_fused_jit(True)
+            assert (got[True] is None) == (got[False] is None)
+            if got[True] is None:
+                continue
+            (mx, mn, sm, nr), (mx0, mn0, sm0, nr0) = got[True], got[False]
+            same = lambda a, b: a == b or (math.isnan(a) and math.isnan(b))
+            assert same(mx, mx0) and same(mn, mn0), i
+            if math.isfinite(sm0):
+                assert abs(sm - sm0) <= 2.0 ** -51 * abs(sm0) + 2.0 ** -90 * (abs(mx0) + abs(mn0)) * L.npts, i
+            else:
+                assert same(sm, sm0), i
+            if math.isfinite(nr0):
+                assert abs(nr - nr0) <= 4 * 2.0 ** -53 * nr0, i
+            else:
+                assert same(nr, nr0), i
+            ran += 1
+    c1, _, f1 = nh.fused_jit_stats()
+    assert ran >= 12 and f1 == f0 and c1 > c0
+
+
 def _all_operators():
     db = nh.build_stencils(9, 7, 6, 1.0, 0.9, 1.3)
     return db, sorted(db.keys())
@@ -378,7 +436,7 @@ def test_transcendental_ulp_budgets(fn, budget):
 # --------------------------------------------------------------------------
 
 @pytest.mark.parametrize("n", [(1, 1, 1), (5, 7, 3), (64, 64, 64), (258, 1, 1)])
-def test_reductions(n):
+def test_reductions(n, fused_mode):
     L = nh.Layout.build(n, 1)
     rng = np.random.default_rng(sum(n))
     data = rng.standard_normal(L.npts) * 10.0
@@ -403,7 +461,7 @@ def test_reductions(n):
         assert abs(nr(f) - nr_ref) <= 4 * 2.0 ** -53 * nr_ref
 
 
-def test_reduce_expression_and_nan():
+def test_reduce_expression_and_nan(fused_mode):
     L = nh.Layout.build((16, 16, 16), 1)
     rng = np.random.default_rng(9)
     v = rng.standard_normal(L.npts)
@@ -446,3 +504,13 @@ def test_errors_on_gpu():
     d = a.desc("gpu")
     rc = nh.lib().nebo_assign(C.byref(d), s.ref, 1, None)
     assert rc == _abi.NEBO_ERR_DEVICE
+    # a tree that reads no field or mask: with ghost cells its region is
+    # unbounded (refused before any kernel); its interior is empty and gives the
+    # fold's initial value, as the reference's loop over [0, 0) does
+    c = nh.E(2.0) + 1.0
+    for fn in (nh.field_max, nh.field_min, nh.field_sum, nh.field_norm):
+        with pytest.raises(nh.NeboError) as ei:
+            fn(c)
+        assert ei.value.code == _abi.NEBO_ERR_INVALID and "unbounded" in str(ei.value)
+    assert nh.field_max_interior(c) == -math.inf and nh.field_min_interior(c) == math.inf
+    assert nh.field_sum_interior(c) == 0.0 and nh.field_norm_interior(c) == 0.0

@@ -344,6 +344,19 @@ def test_reductions():
     assert nrm == math.sqrt(acc2)
 
 
+def test_reduce_without_fields():
+    """A tree that reads no field and no mask has extent 0 and GHOST_MAX ghosts
+    (NeboRhs.h:52-62): its interior fold is empty and returns the initial value
+    (NeboReductions.h:58-90 loops over [0, 0)); with ghost cells its region is
+    [-GHOST_MAX, GHOST_MAX)^3 and the checker refuses it."""
+    c = nh.E(2.0) + 1.0
+    assert orc.reduce(c, nh._abi.RED_MAX, False) == -math.inf
+    assert orc.reduce(c, nh._abi.RED_MIN, False) == math.inf
+    assert orc.reduce(c, nh._abi.RED_SUM, False) == 0.0
+    with pytest.raises(RuntimeError, match="oracle status"):
+        orc.reduce(c, nh._abi.RED_MAX, True)
+
+
 def test_pow2_follows_host_pow():
     """NeboTest.cpp:516 compares pow(x,2) with std::pow(x, 2.0) exactly; the host
     pow is within one ulp of the correctly rounded x*x (not always equal to it)."""
