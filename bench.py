@@ -176,6 +176,86 @@ def adjoint_leg(n=128, steps=100, every=10):
             "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3)}
 
 
+def _time_calls(fn, stream, reps, warm=3):
+    """Mean ms per call over reps back-to-back calls, HIP events on stream."""
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def generic_leg(k, phi, ops, reps=20):
+    """Trees that have no hand-written kernel, on the same 256^3 fields: each
+    evaluated by the kernel compiled at run time for its shape and by the
+    stack-program interpreter (assignments must agree bit for bit), plus
+    reductions.  GB/s counts algorithmic bytes: 8 B per distinct field read per
+    cell and 8 B per written cell; the div(k grad phi) tree is forced through the
+    generic evaluator to compare with the hand-written kernel."""
+    import ctypes as C
+    import torch
+    import nebo_hip as nh
+    L = phi.layout
+    n3 = L.interior[0] * L.interior[1] * L.interior[2]
+    out = nh.Field.empty(L, "gpu")
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    lib = nh.lib()
+    div = sum((ops[f"Div{a}"](ops[f"InterpC2F{a}"](k) * ops[f"Grad{a}"](phi)) for a in "YZ"),
+              ops["DivX"](ops["InterpC2FX"](k) * ops["GradX"](phi)))
+    lap = sum((ops[f"Div{a}"](ops[f"Grad{a}"](phi)) for a in "YZ"), ops["DivX"](ops["GradX"](phi)))
+    trees = {"cond(phi > 0, phi*phi)(-phi)": (nh.cond(phi > 0.0, phi * phi)(-phi), 16),
+             "phi*k + sin(phi)": (phi * k + nh.sin(phi), 24),
+             "div(k grad phi), generic": (div, 24),
+             "laplacian, generic": (lap, 16)}
+    res = {"assign": {}, "reduce": {}}
+    for name, (t, bpc) in trees.items():
+        s, d = nh.serialize(t, "gpu"), out.desc("gpu")
+
+        def call():
+            rc = lib.nebo_assign_generic(C.byref(d), s.ref, 1, sp)
+            if rc:
+                raise RuntimeError(lib.nebo_last_error().decode())
+        e, outs = {}, []
+        for mode in ("compiled", "interpreter"):
+            nh.set_fused_jit(mode == "compiled")
+            ms = _time_calls(call, stream, reps)
+            e[mode + "_us"] = round(ms * 1e3, 2)
+            e[mode + "_GBps"] = round(bpc * n3 / (ms * 1e-3) / 1e9, 1)
+            outs.append(out.gpu.clone())
+        nh.set_fused_jit(True)
+        e["bit_identical"] = bool(torch.equal(outs[0], outs[1]))
+        res["assign"][name] = e
+    dev = torch.zeros(1, dtype=torch.float64, device="cuda")
+    reds = {"max(phi)": (nh.E(phi), nh._abi.RED_MAX, 1, 8),
+            "norm_interior(phi)": (nh.E(phi), nh._abi.RED_NORM, 0, 8),
+            "sum(phi*k)": (phi * k, nh._abi.RED_SUM, 1, 16)}
+    for name, (t, op, ug, bpc) in reds.items():
+        s = nh.serialize(t, "gpu")
+
+        def call():
+            rc = lib.nebo_reduce_async(s.ref, op, ug, C.c_void_p(dev.data_ptr()), sp)
+            if rc:
+                raise RuntimeError(lib.nebo_last_error().decode())
+        e, vals = {}, []
+        for mode in ("compiled", "interpreter"):
+            nh.set_fused_jit(mode == "compiled")
+            ms = _time_calls(call, stream, reps)
+            e[mode + "_us"] = round(ms * 1e3, 2)
+            e[mode + "_GBps"] = round(bpc * n3 / (ms * 1e-3) / 1e9, 1)
+            vals.append(float(dev.item()))
+        nh.set_fused_jit(True)
+        e["values"] = vals
+        res["reduce"][name] = e
+    return res
+
+
 def transport_leg(n=512, steps=20, every=5):
     """Scalar transport at 512^3: upwinded convective flux (cond on the sign of the
     face velocity) plus div(k grad phi), SSP-RK3, 20 steps.  Timed two ways: each
@@ -372,6 +452,7 @@ def main():
         extras["batched"] = batched_leg()
         extras["adjoint"] = adjoint_leg()
         extras["transport"] = transport_leg()
+        extras["generic"] = generic_leg(k, phi, ops)
 
     cpu = None
     if not a.no_cpu:

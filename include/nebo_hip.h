@@ -224,6 +224,18 @@ int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
  *   assignment kernel for reduce_op -1, else the NEBO_RED_* reduction kernel
  *   (*needed = bytes with the terminating 0; buf may be NULL). */
 int nebo_set_fused_jit(int enable);
+/* Shape of the compiled assignment kernels: threads per block (a multiple of 64
+ * in [64, 1024]); tile_x 0 lays a block over consecutive points of the
+ * flattened x-y plane, else over a tile_x by block_threads/tile_x tile (tile_x a
+ * multiple of 64 dividing block_threads); z cells per thread: the largest power
+ * of two up to max_cells whose distinct reads per thread (reads one plane apart
+ * along z shared between cells) stay within load_budget; xcd_remap 1 renumbers
+ * blocks so that neighbouring tiles run on one XCD (one L2).  0 keeps the
+ * default of block_threads, max_cells and load_budget, -1 that of tile_x and
+ * xcd_remap.  Defaults: 512 threads, flattened plane, 4 cells, 48 reads, XCD
+ * renumbering on.  Kernels compiled for another shape stay cached. */
+int nebo_fused_jit_tune(int32_t block_threads, int32_t tile_x, int32_t max_cells,
+                        int32_t load_budget, int32_t xcd_remap);
 int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);
 int nebo_fused_source(const nebo_expr_t* rhs, int32_t reduce_op, char* buf, int32_t size,
                       int32_t* needed);

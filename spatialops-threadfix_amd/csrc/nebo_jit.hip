@@ -59,6 +59,7 @@ namespace {
   int mX[6];                                                                \
   int mY[6];                                                                \
   int nblocks;                                                              \
+  int ntx;                                                                  \
   double c[48];
 
 struct JitArgs {
@@ -70,7 +71,28 @@ static_assert(kVmMaxLoads == 40 && kVmMaxMasks == 6 && kVmMaxConsts == 48 && kVm
 #define NEBO_JIT_STR2(...) #__VA_ARGS__
 #define NEBO_JIT_STR(...) NEBO_JIT_STR2(__VA_ARGS__)
 
-constexpr int kJitBlock = 256;
+constexpr int kReduceBlock = 256;   // threads per block of the reduction kernels
+
+// Launch shape of the assignment kernels and the cells-per-thread rule (see
+// choose_cells); 0 in nebo_fused_jit_tune restores these defaults.
+// Defaults from a sweep at 256^3 (pointwise, cond, Laplacian and div(k grad phi)
+// trees): 512-thread blocks over the flattened plane, XCD renumbering on, up to
+// 4 cells per thread within 48 distinct reads.  Larger blocks let a block's rows
+// share their y-neighbour rows in L1; 2-D tiles and 8 cells per thread measured
+// slower.
+constexpr int kDefaultBlock = 512, kDefaultTileX = 0, kDefaultMaxCells = 4, kDefaultLoadBudget = 48,
+              kDefaultXcd = 1;
+std::atomic<int> g_block{kDefaultBlock}, g_tile_x{kDefaultTileX}, g_max_cells{kDefaultMaxCells},
+    g_load_budget{kDefaultLoadBudget}, g_xcd{kDefaultXcd};
+constexpr int kXcds = 8;   // MI355X: 8 accelerator complex dies, each with its own L2
+
+// Launch shape of an assignment kernel: threads per block; tile_x == 0 lays the
+// block over consecutive points of the flattened x-y plane, else over a
+// tile_x by block/tile_x tile of it; xcd renumbers blocks so that consecutive
+// tiles (which share halo rows and planes) run on one XCD and share its L2.
+struct Shape {
+  int block, tile_x, xcd;
+};
 
 std::atomic<int> g_mode{1};
 std::atomic<int> g_compiled{0}, g_hits{0}, g_failures{0};
@@ -83,28 +105,92 @@ struct Entry {
 std::mutex g_mu;
 std::unordered_map<std::string, Entry> g_cache;   // key: device + source
 
-// Index set-up, reads and operations for V cells of one z column at (x, y,
-// z0 .. z{V-1}); the generated code expects x, y and the z variables in scope
-// and returns the name of each cell's value.
-std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, int V, const std::string& ind)
+// Which load slots a program reads, whether every read of a slot is
+// bounds-checked (VM_LOADG only), and which masks it reads.
+struct SlotUse {
+  bool used[kVmMaxLoads] = {}, guarded[kVmMaxLoads] = {}, mused[kVmMaxMasks] = {};
+  // z chains of unguarded slots: chain[c] lists slots whose bases lie one plane
+  // (cls_XY doubles) apart, in increasing z; cell v + 1's read of chain[c][j]
+  // is then cell v's read of chain[c][j + 1], the same address.
+  std::vector<std::vector<int>> chains;
+};
+
+SlotUse slot_use(const VmProgram& P)
 {
-  // which slots are read, and whether every read of a slot is bounds-checked
-  bool used[kVmMaxLoads] = {}, guarded[kVmMaxLoads], mused[kVmMaxMasks] = {};
-  for (int s = 0; s < kVmMaxLoads; ++s) guarded[s] = true;
+  SlotUse u;
+  for (int s = 0; s < kVmMaxLoads; ++s) u.guarded[s] = true;
   for (int pc = 0; pc < P.n_insns; ++pc) {
     const int op = P.insns[pc].code >> 4, imm = P.insns[pc].imm;
-    if (op == VM_LOAD) { used[imm] = true; guarded[imm] = false; }
-    if (op == VM_LOADG) used[imm] = true;
-    if (op == VM_MASK) mused[imm] = true;
+    if (op == VM_LOAD) { u.used[imm] = true; u.guarded[imm] = false; }
+    if (op == VM_LOADG) u.used[imm] = true;
+    if (op == VM_MASK) u.mused[imm] = true;
   }
+  int next[kVmMaxLoads], has_prev[kVmMaxLoads] = {};
+  for (int s = 0; s < kVmMaxLoads; ++s) next[s] = -1;
+  for (int s = 0; s < kVmMaxLoads; ++s) {
+    if (!u.used[s] || u.guarded[s]) continue;
+    for (int t = 0; t < kVmMaxLoads; ++t) {
+      if (t == s || !u.used[t] || u.guarded[t] || P.loads[t].cls != P.loads[s].cls) continue;
+      if (P.loads[t].base == P.loads[s].base + P.cls_XY[P.loads[s].cls] && next[s] < 0 && !has_prev[t]) {
+        next[s] = t;
+        has_prev[t] = 1;
+      }
+    }
+  }
+  for (int s = 0; s < kVmMaxLoads; ++s) {
+    if (!u.used[s] || u.guarded[s] || has_prev[s]) continue;
+    std::vector<int> c;
+    for (int t = s; t >= 0 && c.size() < (size_t)kVmMaxLoads; t = next[t]) c.push_back(t);
+    u.chains.push_back(c);
+  }
+  return u;
+}
+
+// Distinct reads per thread for V cells of a full z chunk.
+int reads_per_thread(const SlotUse& u, int V)
+{
+  int n = 0;
+  for (const auto& c : u.chains) n += (int)c.size() + V - 1;
+  for (int s = 0; s < kVmMaxLoads; ++s)
+    if (u.used[s] && u.guarded[s]) n += V;
+  for (int m = 0; m < kVmMaxMasks; ++m)
+    if (u.mused[m]) n += V;
+  return n;
+}
+
+// Index set-up, reads and operations for V cells of one z column at (x, y,
+// z0 .. z{V-1}); the generated code expects x, y and the z variables in scope
+// and returns the name of each cell's value.  With full, the V cells are
+// consecutive planes (z{v} = z0 + v) and reads along a z chain are issued once
+// per distinct plane; otherwise (a chunk cut by the box end, z clamped) every
+// cell reads for itself.
+std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int V, bool full,
+                                    const std::string& ind)
+{
+  const bool* used = u.used;
+  const bool* guarded = u.guarded;
+  const bool* mused = u.mused;
   for (int v = 0; v < V; ++v)
     for (int k = 0; k < P.n_classes; ++k)
       o << ind << "const int i" << v << "_" << k << " = x + A.clsX[" << k << "] * y + A.clsXY[" << k << "] * z"
         << v << ";\n";
   // phase 1: every read, for every cell
+  if (full) {
+    for (size_t c = 0; c < u.chains.size(); ++c) {
+      const std::vector<int>& ch = u.chains[c];
+      const int k = P.loads[ch[0]].cls;
+      const int planes = (int)ch.size() + V - 1;
+      for (int p = 0; p < planes; ++p)
+        o << ind << "const double Z" << c << "_" << p << " = A.base[" << ch[0] << "][i0_" << k << " + " << p
+          << " * A.clsXY[" << k << "]];\n";
+      for (int v = 0; v < V; ++v)
+        for (size_t j = 0; j < ch.size(); ++j)
+          o << ind << "const double L" << v << "_" << ch[j] << " = Z" << c << "_" << v + (int)j << ";\n";
+    }
+  }
   for (int v = 0; v < V; ++v) {
     for (int s = 0; s < kVmMaxLoads; ++s) {
-      if (!used[s]) continue;
+      if (!used[s] || (full && !guarded[s])) continue;
       const std::string idx = "i" + std::to_string(v) + "_" + std::to_string(P.loads[s].cls);
       if (guarded[s])
         o << ind << "double L" << v << "_" << s << " = __builtin_nan(\"\");\n" << ind << "if (" << idx
@@ -200,35 +286,83 @@ std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, i
   return results;
 }
 
-// Assignment kernel: one thread per (x, y) point of the box and V z cells.
-std::string jit_source(const VmProgram& P, int V)
+// One z chunk of V cells at zb with nv of them inside the box.  A full chunk
+// (nv == V, uniform over the block) reads all V cells at once with shared
+// z-chain reads; a chunk cut by the box end evaluates its cells one at a time,
+// each in its own scope under v < nv, so the tail never needs more registers
+// than one cell.  sink(indent, z variable, value) consumes each cell in z order.
+template <class Sink>
+void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int V, const std::string& ind,
+                Sink sink)
 {
+  const std::string in2 = V > 1 ? ind + "  " : ind;
+  if (V > 1) o << ind << "if (nv == " << V << ") {\n";
+  for (int v = 0; v < V; ++v) o << in2 << "const int z" << v << " = zb + " << v << ";\n";
+  std::vector<std::string> r = emit_cells(o, P, u, V, true, in2);
+  for (int v = 0; v < V; ++v) sink(in2, "z" + std::to_string(v), r[v]);
+  if (V > 1) {
+    o << ind << "} else {\n";
+    const std::string in3 = in2 + "  ";
+    for (int v = 0; v < V - 1; ++v) {   // nv < V: cell V - 1 is never inside
+      o << in2 << "if (" << v << " < nv) {\n";
+      o << in3 << "const int z0 = zb + " << v << ";\n";
+      r = emit_cells(o, P, u, 1, false, in3);
+      sink(in3, "z0", r[0]);
+      o << in2 << "}\n";
+    }
+    o << ind << "}\n";
+  }
+}
+
+// Assignment kernel: one thread per (x, y) point of the box and V z cells.
+// Blocks form a 1-D grid of A.ntx plane tiles times A.nzc z chunks (A.nblocks
+// in all), plane tiles fastest.
+std::string jit_source(const VmProgram& P, int V, const Shape& sh)
+{
+  const SlotUse u = slot_use(P);
   std::ostringstream o;
   o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
-  o << "extern \"C\" __global__ __launch_bounds__(" << kJitBlock << ") void nebo_fused(const JitArgs A)\n{\n";
-  o << "  const int t = (int)blockIdx.x * " << kJitBlock << " + (int)threadIdx.x;\n";
-  o << "  if (t >= A.nplane) return;\n";
-  o << "  const int yy = t / A.n0;\n  const int x = A.lo0 + (t - yy * A.n0);\n  const int y = A.lo1 + yy;\n";
-  o << "  const int zb = A.lo2 + (int)blockIdx.y * " << V << ";\n";
+  o << "extern \"C\" __global__ __launch_bounds__(" << sh.block << ") void nebo_fused(const JitArgs A)\n{\n";
+  if (sh.xcd) {
+    // hardware block L runs on XCD L % 8; give XCD d the contiguous range of
+    // logical blocks [start_d, start_d + count_d), count_d = per + (d < rem)
+    o << "  const int L = (int)blockIdx.x, d = L % " << kXcds << ";\n";
+    o << "  const int per = A.nblocks / " << kXcds << ", rem = A.nblocks % " << kXcds << ";\n";
+    o << "  const int b = d * per + (d < rem ? d : rem) + L / " << kXcds << ";\n";
+  } else {
+    o << "  const int b = (int)blockIdx.x;\n";
+  }
+  o << "  const int bz = b / A.ntx, bt = b - bz * A.ntx;\n";
+  if (sh.tile_x == 0) {
+    o << "  const int t = bt * " << sh.block << " + (int)threadIdx.x;\n";
+    o << "  if (t >= A.nplane) return;\n";
+    o << "  const int yy = t / A.n0;\n  const int x = A.lo0 + (t - yy * A.n0);\n  const int y = A.lo1 + yy;\n";
+  } else {
+    const int TX = sh.tile_x, TY = sh.block / sh.tile_x;
+    o << "  const int tpr = (A.n0 + " << TX - 1 << ") / " << TX << ";\n";
+    o << "  const int by = bt / tpr, bx = bt - by * tpr;\n";
+    o << "  const int xi = bx * " << TX << " + (int)threadIdx.x % " << TX << ";\n";
+    o << "  const int yi = by * " << TY << " + (int)threadIdx.x / " << TX << ";\n";
+    o << "  if (xi >= A.n0 || yi >= A.n1) return;\n";
+    o << "  const int x = A.lo0 + xi, y = A.lo1 + yi;\n";
+  }
+  o << "  const int zb = A.lo2 + bz * " << V << ";\n";
   o << "  const int left = A.lo2 + A.n2 - zb;\n  const int nv = left < " << V << " ? left : " << V << ";\n";
-  // cells past the box end repeat the last cell (in bounds); only stores are guarded
-  for (int v = 0; v < V; ++v)
-    o << "  const int z" << v << " = zb + (" << v << " < nv ? " << v << " : nv - 1);\n";
-  const std::vector<std::string> r = emit_cells(o, P, V, "  ");
-  for (int v = 0; v < V; ++v)
-    o << "  if (" << v << " < nv) __builtin_nontemporal_store(" << r[v] << ", &A.out[x + A.oX * y + A.oXY * z" << v
-      << "]);\n";
+  emit_chunk(o, P, u, V, "  ", [&](const std::string& ind, const std::string& z, const std::string& val) {
+    o << ind << "__builtin_nontemporal_store(" << val << ", &A.out[x + A.oX * y + A.oXY * " << z << "]);\n";
+  });
   o << "}\n";
   return o.str();
 }
 
 // Reduction kernel: a grid-stride loop over (x, y, z chunk) items, V z cells
-// per item, each thread folding its cells into a double-double (sum, norm) or
-// a select-form max/min exactly as the interpreter's vm_reduce_kernel does;
-// then a fixed-order block reduction writes one (hi, lo) partial per block to
-// A.out, which reduce_final_kernel folds.
+// per item, each thread folding its cells in z order into a double-double
+// (sum, norm) or a select-form max/min exactly as the interpreter's
+// vm_reduce_kernel does; then a fixed-order block reduction writes one
+// (hi, lo) partial per block to A.out, which reduce_final_kernel folds.
 std::string jit_reduce_source(const VmProgram& P, int V, int op)
 {
+  const SlotUse u = slot_use(P);
   std::ostringstream o;
   o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
   const char* init = op == NEBO_RED_MAX ? "-__builtin_inf()" : op == NEBO_RED_MIN ? "__builtin_inf()" : "0.0";
@@ -241,64 +375,49 @@ std::string jit_reduce_source(const VmProgram& P, int V, int op)
     combine = "{ const double s = hi + h2; const double bb = s - hi; "
               "const double e = ((hi - (s - bb)) + (h2 - bb)) + (lo + l2); "
               "const double h = s + e; lo = e - (h - s); hi = h; }";
-  o << "extern \"C\" __global__ __launch_bounds__(" << kJitBlock << ") void nebo_reduce(const JitArgs A)\n{\n";
+  const int B = kReduceBlock;
+  o << "extern \"C\" __global__ __launch_bounds__(" << B << ") void nebo_reduce(const JitArgs A)\n{\n";
   o << "  double hi = " << init << ", lo = 0.0;\n";
   o << "  const long items = (long)A.nplane * A.nzc;\n";
-  o << "  for (long w = (long)blockIdx.x * " << kJitBlock << " + threadIdx.x; w < items; w += (long)A.nblocks * "
-    << kJitBlock << ") {\n";
+  o << "  for (long w = (long)blockIdx.x * " << B << " + threadIdx.x; w < items; w += (long)A.nblocks * " << B
+    << ") {\n";
   o << "    const int zc = (int)(w / A.nplane);\n    const int t = (int)(w - (long)zc * A.nplane);\n";
   o << "    const int yy = t / A.n0;\n    const int x = A.lo0 + (t - yy * A.n0);\n    const int y = A.lo1 + yy;\n";
   o << "    const int zb = A.lo2 + zc * " << V << ";\n";
   o << "    const int left = A.lo2 + A.n2 - zb;\n    const int nv = left < " << V << " ? left : " << V << ";\n";
-  for (int v = 0; v < V; ++v)
-    o << "    const int z" << v << " = zb + (" << v << " < nv ? " << v << " : nv - 1);\n";
-  const std::vector<std::string> r = emit_cells(o, P, V, "    ");
-  for (int v = 0; v < V; ++v) {
-    o << "    if (" << v << " < nv) {\n      const double xv = " << r[v] << ";\n";
-    if (op == NEBO_RED_MAX) o << "      hi = (hi < xv) ? xv : hi;\n";
-    else if (op == NEBO_RED_MIN) o << "      hi = (hi > xv) ? xv : hi;\n";
+  emit_chunk(o, P, u, V, "    ", [&](const std::string& ind, const std::string&, const std::string& val) {
+    o << ind << "{\n";
+    o << ind << "  const double xv = " << val << ";\n";
+    if (op == NEBO_RED_MAX) o << ind << "  hi = (hi < xv) ? xv : hi;\n";
+    else if (op == NEBO_RED_MIN) o << ind << "  hi = (hi > xv) ? xv : hi;\n";
     else if (op == NEBO_RED_SUM)
-      o << "      const double s = hi + xv; const double bb = s - hi;\n"
-           "      lo = lo + ((hi - (s - bb)) + (xv - bb)); hi = s;\n";
+      o << ind << "  const double s = hi + xv; const double bb = s - hi;\n"
+        << ind << "  lo = lo + ((hi - (s - bb)) + (xv - bb)); hi = s;\n";
     else
       // x*x exactly as p + err (Dekker split, no FMA)
-      o << "      const double p = xv * xv; const double t2 = 134217729.0 * xv;\n"
-           "      const double xh = t2 - (t2 - xv), xl = xv - xh;\n"
-           "      const double err = ((xh * xh - p) + 2.0 * xh * xl) + xl * xl;\n"
-           "      const double s = hi + p; const double bb = s - hi;\n"
-           "      lo = lo + ((hi - (s - bb)) + (p - bb)); hi = s; lo = lo + err;\n";
-    o << "    }\n";
-  }
+      o << ind << "  const double p = xv * xv; const double t2 = 134217729.0 * xv;\n"
+        << ind << "  const double xh = t2 - (t2 - xv), xl = xv - xh;\n"
+        << ind << "  const double err = ((xh * xh - p) + 2.0 * xh * xl) + xl * xl;\n"
+        << ind << "  const double s = hi + p; const double bb = s - hi;\n"
+        << ind << "  lo = lo + ((hi - (s - bb)) + (p - bb)); hi = s; lo = lo + err;\n";
+    o << ind << "}\n";
+  });
   o << "  }\n";
   // fixed-order block reduction: xor-shuffle tree inside each wave, then the waves in order
   o << "  for (int off = 32; off >= 1; off >>= 1) {\n";
   o << "    const double h2 = __shfl_xor(hi, off, 64);\n";
   o << "    const double l2 = " << (dd ? "__shfl_xor(lo, off, 64)" : "0.0") << ";\n";
   o << "    " << combine << "\n  }\n";
-  o << "  __shared__ double sh[" << 2 * (kJitBlock / 64) << "];\n";
+  o << "  __shared__ double sh[" << 2 * (B / 64) << "];\n";
   o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
   o << "  if (lane == 0) { sh[2 * wave] = hi; sh[2 * wave + 1] = lo; }\n";
   o << "  __syncthreads();\n";
   o << "  if (threadIdx.x == 0) {\n    hi = sh[0]; lo = sh[1];\n";
-  o << "    for (int w = 1; w < " << kJitBlock / 64 << "; ++w) {\n";
+  o << "    for (int w = 1; w < " << B / 64 << "; ++w) {\n";
   o << "      const double h2 = sh[2 * w], l2 = sh[2 * w + 1];\n      " << combine << "\n    }\n";
   o << "    A.out[2 * blockIdx.x] = hi;\n    A.out[2 * blockIdx.x + 1] = " << (dd ? "lo" : "0.0") << ";\n  }\n";
   o << "}\n";
   return o.str();
-}
-
-int reads_of(const VmProgram& P)
-{
-  bool used[kVmMaxLoads + kVmMaxMasks] = {};
-  int n = 0;
-  for (int pc = 0; pc < P.n_insns; ++pc) {
-    const int op = P.insns[pc].code >> 4, imm = P.insns[pc].imm;
-    int slot = -1;
-    if (op == VM_LOAD || op == VM_LOADG) slot = imm;
-    if (op == VM_MASK) slot = kVmMaxLoads + imm;
-    if (slot >= 0 && !used[slot]) { used[slot] = true; ++n; }
-  }
-  return n;
 }
 
 int compile(const std::string& src, const char* name, int device, Entry* e)
@@ -368,10 +487,19 @@ int get_kernel(const std::string& src, const char* name, int V, Entry* out)
   return NEBO_OK;
 }
 
-int cells_per_thread(const VmProgram& P)
+// Cells per thread: the largest power of two up to the configured maximum
+// whose distinct reads per thread (z-chain reads shared) stay within the load
+// budget, so a thread's reads are all in flight without spilling.
+Shape current_shape() { return Shape{g_block.load(), g_tile_x.load(), g_xcd.load()}; }
+
+int choose_cells(const VmProgram& P)
 {
-  const int nr = reads_of(P);
-  return nr <= 6 ? 4 : nr <= 16 ? 2 : 1;
+  const SlotUse u = slot_use(P);
+  const int budget = g_load_budget.load();
+  int V = 1;
+  for (int c = 2; c <= g_max_cells.load(); c *= 2)
+    if (reads_per_thread(u, c) <= budget) V = c;
+  return V;
 }
 
 void fill_args(const VmProgram& P, const nebo_limits_t& lim, int V, JitArgs* A)
@@ -403,11 +531,11 @@ void fill_args(const VmProgram& P, const nebo_limits_t& lim, int V, JitArgs* A)
   for (int c = 0; c < kVmMaxConsts; ++c) A->c[c] = P.consts[c];
 }
 
-int launch(const Entry& e, JitArgs* A, unsigned gx, unsigned gy, void* stream)
+int launch(const Entry& e, JitArgs* A, unsigned gx, unsigned gy, unsigned block, void* stream)
 {
   size_t bytes = sizeof *A;
   void* cfg[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, A, HIP_LAUNCH_PARAM_BUFFER_SIZE, &bytes, HIP_LAUNCH_PARAM_END};
-  return check_hip(hipModuleLaunchKernel(e.fn, gx, gy, 1, kJitBlock, 1, 1, 0, (hipStream_t)stream, nullptr, cfg),
+  return check_hip(hipModuleLaunchKernel(e.fn, gx, gy, 1, block, 1, 1, 0, (hipStream_t)stream, nullptr, cfg),
                    "Nebo Fused Kernel");
 }
 
@@ -419,17 +547,29 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
 {
   *done = false;
   if (!jit_enabled()) return NEBO_OK;
-  const int V = cells_per_thread(P);
+  const int V = choose_cells(P);
+  const Shape sh = current_shape();
   Entry e;
-  int rc = get_kernel(jit_source(P, V), "nebo_fused", V, &e);
+  int rc = get_kernel(jit_source(P, V, sh), "nebo_fused", V, &e);
   if (rc || !e.fn) return rc;
   JitArgs A;
   fill_args(P, lim, V, &A);
   A.out = frame_origin(lhs);
   A.oX = lhs->glob[0];
   A.oXY = lhs->glob[0] * lhs->glob[1];
-  const unsigned gx = (unsigned)((A.nplane + kJitBlock - 1) / kJitBlock);
-  if ((rc = launch(e, &A, gx, (unsigned)A.nzc, stream))) return rc;
+  long ntx;
+  if (sh.tile_x == 0) {
+    ntx = ((long)A.nplane + sh.block - 1) / sh.block;
+  } else {
+    const long ty = sh.block / sh.tile_x;
+    ntx = (((long)A.n0 + sh.tile_x - 1) / sh.tile_x) * (((long)A.n1 + ty - 1) / ty);
+  }
+  const long total = ntx * A.nzc;
+  if (total >= (1L << 31))
+    return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
+  A.ntx = (int)ntx;
+  A.nblocks = (int)total;
+  if ((rc = launch(e, &A, (unsigned)total, 1, (unsigned)sh.block, stream))) return rc;
   *done = true;
   return NEBO_OK;
 }
@@ -439,7 +579,7 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
 {
   *done = false;
   if (!jit_enabled()) return NEBO_OK;
-  const int V = cells_per_thread(P);
+  const int V = choose_cells(P);
   Entry e;
   int rc = get_kernel(jit_reduce_source(P, V, op), "nebo_reduce", V, &e);
   if (rc || !e.fn) return rc;
@@ -447,11 +587,11 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
   fill_args(P, lim, V, &A);
   A.out = partial;
   const long items = (long)A.nplane * A.nzc;
-  long nb = (items + kJitBlock - 1) / kJitBlock;
+  long nb = (items + kReduceBlock - 1) / kReduceBlock;
   if (nb > max_blocks) nb = max_blocks;
   if (nb < 1) nb = 1;
   A.nblocks = (int)nb;
-  if ((rc = launch(e, &A, (unsigned)nb, 1, stream))) return rc;
+  if ((rc = launch(e, &A, (unsigned)nb, 1, kReduceBlock, stream))) return rc;
   *blocks = (int)nb;
   *done = true;
   return NEBO_OK;
@@ -459,8 +599,8 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
 
 std::string jit_source_for(const VmProgram& P, int reduce_op)
 {
-  const int V = cells_per_thread(P);
-  return reduce_op < 0 ? jit_source(P, V) : jit_reduce_source(P, V, reduce_op);
+  const int V = choose_cells(P);
+  return reduce_op < 0 ? jit_source(P, V, current_shape()) : jit_reduce_source(P, V, reduce_op);
 }
 
 }  // namespace nebo
@@ -469,6 +609,32 @@ extern "C" int nebo_set_fused_jit(int enable)
 {
   nebo::clear_error();
   nebo::g_mode.store(enable ? 1 : 0);
+  return NEBO_OK;
+}
+
+extern "C" int nebo_fused_jit_tune(int32_t block_threads, int32_t tile_x, int32_t max_cells,
+                                   int32_t load_budget, int32_t xcd_remap)
+{
+  nebo::clear_error();
+  static const char* ctx = "Nebo Fused Kernel";
+  const int b = block_threads ? block_threads : nebo::kDefaultBlock;
+  const int tx = tile_x >= 0 ? tile_x : nebo::kDefaultTileX;
+  const int m = max_cells ? max_cells : nebo::kDefaultMaxCells;
+  const int l = load_budget ? load_budget : nebo::kDefaultLoadBudget;
+  const int x = xcd_remap >= 0 ? xcd_remap : nebo::kDefaultXcd;
+  if (b < 64 || b > 1024 || b % 64)
+    return nebo::set_error(NEBO_ERR_INVALID, ctx, "block_threads: a multiple of 64 in [64, 1024]");
+  if (tx != 0 && (tx % 64 || b % tx))
+    return nebo::set_error(NEBO_ERR_INVALID, ctx, "tile_x: 0, or a multiple of 64 that divides block_threads");
+  if (m < 1 || m > 16 || (m & (m - 1)))
+    return nebo::set_error(NEBO_ERR_INVALID, ctx, "max_cells: a power of two in [1, 16]");
+  if (l < 1 || l > 256) return nebo::set_error(NEBO_ERR_INVALID, ctx, "load_budget: in [1, 256]");
+  if (x != 0 && x != 1) return nebo::set_error(NEBO_ERR_INVALID, ctx, "xcd_remap: 0 or 1");
+  nebo::g_block.store(b);
+  nebo::g_tile_x.store(tx);
+  nebo::g_max_cells.store(m);
+  nebo::g_load_budget.store(l);
+  nebo::g_xcd.store(x);
   return NEBO_OK;
 }
 

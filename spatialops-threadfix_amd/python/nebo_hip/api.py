@@ -120,6 +120,14 @@ def fused_jit_stats():
     return tuple(x.value for x in v)
 
 
+def fused_jit_tune(block_threads=0, tile_x=-1, max_cells=0, load_budget=0, xcd_remap=-1):
+    """Launch shape of the compiled assignment kernels; the defaults restore the
+    library's own (512 threads over the flattened plane, up to 4 z cells per
+    thread within 48 distinct reads, XCD renumbering on)."""
+    _abi.call("nebo_fused_jit_tune", int(block_threads), int(tile_x), int(max_cells), int(load_budget),
+              int(xcd_remap))
+
+
 def fused_source(rhs, where="host", reduce_op=None) -> str:
     """The kernel source the run-time compiler generates for rhs: the
     assignment kernel, or with reduce_op (RED_MAX, RED_MIN, RED_SUM, RED_NORM)

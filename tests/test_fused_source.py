@@ -151,3 +151,59 @@ def test_unknown_reduction_refused():
     with pytest.raises(nh.NeboError) as ei:
         nh.fused_source(nh.E(host(L)), reduce_op=9)
     assert ei.value.code == _abi.NEBO_ERR_INVALID
+
+
+def _divkgrad_tree(n=16):
+    L = nh.Layout.build((n, n, n), 1)
+    k, phi = host(L), host(L)
+    ops = nh.basic_op_types(nh.build_stencils(n, n, n, 1.0, 1.0, 1.0))
+    return sum((ops[f"Div{a}"](ops[f"InterpC2F{a}"](k) * ops[f"Grad{a}"](phi)) for a in "YZ"),
+               ops["DivX"](ops["InterpC2FX"](k) * ops["GradX"](phi)))
+
+
+def test_z_chain_reads_are_shared():
+    """div(k grad phi) reads phi and k at 7 points each: (0,0,-1), (0,0,0), (0,0,1)
+    form one z chain per field and the four x/y neighbours four chains of one.
+    In a full chunk of V cells a chain of length m needs m + V - 1 planes, so the
+    kernel issues 2 (3 + V - 1) + 8 V reads instead of 14 V; the tail path (a
+    chunk cut by the box end) reads each cell on its own."""
+    t = _divkgrad_tree()
+    try:
+        for (block, cells, budget), V in (((0, 0, 0), 4), ((256, 4, 32), 2), ((128, 8, 96), 8),
+                                          ((64, 1, 0), 1)):
+            nh.fused_jit_tune(block, max_cells=cells, load_budget=budget)
+            src = nh.fused_source(t)
+            full = src.split("} else {")[0]
+            assert full.count("const double Z") == 2 * (3 + V - 1) + 8 * V, (block, cells, budget)
+            assert f"__launch_bounds__({block or 512})" in src
+            if V > 1:
+                tail = src.split("} else {")[1]
+                assert tail.count("if (") >= V - 1 and tail.count("A.base[") == 14 * (V - 1)
+    finally:
+        nh.fused_jit_tune()
+
+
+def test_tune_bounds():
+    for bad in ((32, -1, 0, 0, -1), (1088, -1, 0, 0, -1), (0, -1, 3, 0, -1), (0, -1, 32, 0, -1),
+                (0, -1, 0, 300, -1), (256, 96, 0, 0, -1), (256, 512, 0, 0, -1), (0, -1, 0, 0, 2)):
+        with pytest.raises(nh.NeboError) as ei:
+            nh.fused_jit_tune(*bad)
+        assert ei.value.code == _abi.NEBO_ERR_INVALID
+    nh.fused_jit_tune()
+
+
+@pytest.mark.parametrize("shape", [(1024, 0, 8, 64, 0), (64, -1, 16, 256, 1), (1024, 64, 4, 48, 1),
+                                   (512, 128, 2, 0, 0)])
+def test_other_shapes_compile(rtc, shape):
+    try:
+        nh.fused_jit_tune(*shape)
+        n = (6, 5, 4)
+        L = nh.Layout.build(n, 1)
+        f, g = host(L), host(L)
+        m = nh.Mask(L, [(0, 0, 0)])
+        db = nh.build_stencils(*n, 1, 1, 1)
+        for t in (_divkgrad_tree(), f * g + 1.0, nh.cond(m, f)(g), db[("Gradient", "SVol", "SSurfX")](f)):
+            check_no_hidden_args(rtc, nh.fused_source(t))
+            check_no_hidden_args(rtc, nh.fused_source(t, reduce_op=_abi.RED_SUM))
+    finally:
+        nh.fused_jit_tune()
