@@ -333,20 +333,34 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  * ty/2 such waves in y in one workgroup (4 resp. 2 rows per lane), so that a
  * wave's halo rows are its neighbours' own rows, loaded on the same CU; variants
  * 7 and 8 are variant 5 with loads issued 3 resp. 4 planes ahead (the Laplacian
- * kernel runs variant 3 with the same rows per lane for 5-8).  Until a tile is
- * set or tuned both kernels run 128 x 8 x 64, variant 7.  kernel 0 = the
+ * kernel runs variant 3 with the same rows per lane for 5-8).  kernel 0 = the
  * var-coefficient diffusion kernels (nebo_divkgrad, nebo_diffusion_step), 1 =
- * Laplacian+Euler.  The tile changes only the schedule, never the results.  The
+ * Laplacian+Euler.  Which tile a launch uses:
+ *   1. the tile pinned by nebo_stencil_set_tile, for every box;
+ *   2. else the tile nebo_stencil_autotune found fastest for a box of the same
+ *      dimensions on the same device;
+ *   3. else 128 x 8 x 64, variant 7, when it gives at least half the device's
+ *      CUs a workgroup (256^3 and up), and otherwise the direct schedule with
+ *      64 x 8 x 16 tiles, or 64 x 4 x 16 when those give fewer workgroups than
+ *      CUs (small boxes).
+ * nebo_stencil_set_tile with tx = ty = zchunk = variant = 0 removes the pin.
+ * nebo_stencil_get_tile reports the tile last pinned or tuned (the default tile
+ * if neither).  The tile changes only the schedule, never the results.  The
  * reference has no counterpart (its CUDA launch shape is fixed at 16x16,
  * NeboLhs.h:413-427). */
 int nebo_stencil_tiles(int32_t* n, int32_t* tx, int32_t* ty, int32_t* zchunk,
                        int32_t* variant);                               /* arrays of n */
 int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t zchunk, int32_t variant);
 int nebo_stencil_get_tile(int kernel, int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant);
+/* The tile a launch of `kernel` over an nx x ny x nz box on `device` would use
+ * (rules 1-3 above). */
+int nebo_stencil_select_tile(int kernel, int device, int32_t nx, int32_t ny, int32_t nz, int32_t* tx,
+                             int32_t* ty, int32_t* zchunk, int32_t* variant);
 /* Time every instantiated tile of `kernel` on these operands (r is overwritten on
- * its interior; k is unused for kernel 1), keep the fastest for that kernel and
- * report it; times_ms (optional) receives one mean time per tile in
- * nebo_stencil_tiles order.  Synchronises stream. */
+ * its interior; k is unused for kernel 1), remember the fastest for boxes of
+ * these dimensions on this device (removing any pin of that kernel) and report
+ * it; times_ms (optional) receives one mean time per tile in nebo_stencil_tiles
+ * order.  Synchronises stream. */
 int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const nebo_field_t* k,
                           const nebo_field_t* phi, const nebo_axis_coefs_t* c, void* stream,
                           int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant,

@@ -13,7 +13,8 @@ from .api import (lower_stats, assign, interior_assign, masked_assign, assign_po
                   field_max, field_min, field_sum, field_norm, field_max_interior,
                   field_min_interior, field_sum_interior, field_norm_interior,
                   fill_ghosts_periodic, divkgrad, lapl_euler, device_count, stencil_tiles,
-                  stencil_set_tile, stencil_get_tile, stencil_autotune, KERNEL_DIVKGRAD, KERNEL_LAPL,
+                  stencil_set_tile, stencil_get_tile, stencil_reset_tile, stencil_select_tile,
+                  stencil_autotune, KERNEL_DIVKGRAD, KERNEL_LAPL,
                   set_fused_jit, fused_jit_stats, fused_jit_tune, fused_source)
 from .bc import MaskShift, BoundaryCondition, shift_points  # noqa: F401,E402
 from .batch import Batch, alloc_patches  # noqa: F401,E402

@@ -149,6 +149,8 @@ _SIGS = {
     "nebo_stencil_tiles": (C.c_int, [P(C.c_int32)] * 5),
     "nebo_stencil_set_tile": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "nebo_stencil_get_tile": (C.c_int, [C.c_int] + [P(C.c_int32)] * 4),
+    "nebo_stencil_select_tile": (C.c_int, [C.c_int, C.c_int, C.c_int32, C.c_int32, C.c_int32, P(C.c_int32),
+                                           P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
     "nebo_stencil_autotune": (C.c_int, [C.c_int, P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), _vp,
                                         P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_float)]),
     "nebo_batch_create": (C.c_int, [P(PatchT), C.c_int32, P(AxisCoefsT), C.c_double, P(C.c_int32), P(_vp)]),

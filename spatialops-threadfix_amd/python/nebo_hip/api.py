@@ -168,8 +168,23 @@ def stencil_set_tile(kernel, tx, ty, zchunk, variant=0):
 
 
 def stencil_get_tile(kernel=0):
+    """The tile last pinned or tuned for kernel (the default tile if neither)."""
     v = [C.c_int32() for _ in range(4)]
     _abi.call("nebo_stencil_get_tile", int(kernel), *[C.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def stencil_reset_tile(kernel=0):
+    """Remove a pinned tile: tuned tiles and the box-size rule apply again."""
+    _abi.call("nebo_stencil_set_tile", int(kernel), 0, 0, 0, 0)
+
+
+def stencil_select_tile(kernel, box, device=0):
+    """The tile a launch of kernel over a box of interior size box (nx, ny, nz)
+    on device would use."""
+    v = [C.c_int32() for _ in range(4)]
+    _abi.call("nebo_stencil_select_tile", int(kernel), int(device), *[int(b) for b in box],
+              *[C.byref(x) for x in v])
     return tuple(x.value for x in v)
 
 

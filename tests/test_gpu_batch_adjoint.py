@@ -150,7 +150,6 @@ def test_diffusion_step_every_tile_bit_exact():
     ro = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
     orc.diffusion_step(ro, nh.Field(L, host=phi), nh.Field(L, host=k), c, dt)
     gp, gkk = torch.from_numpy(phi).cuda(), torch.from_numpy(k).cuda()
-    saved = nh.stencil_get_tile(0)
     try:
         for tile in nh.stencil_tiles():
             nh.stencil_set_tile(0, *tile)
@@ -159,7 +158,7 @@ def test_diffusion_step_every_tile_bit_exact():
             torch.cuda.synchronize()
             assert np.array_equal(out.cpu().numpy(), ro.host), tile
     finally:
-        nh.stencil_set_tile(0, *saved)
+        nh.stencil_reset_tile(0)
 
 
 def _trajectory_case(n, steps, seed=5):

@@ -402,7 +402,6 @@ def test_every_stencil_tile_bit_exact(lengths):
     orc.divkgrad(ref1, k, phi, c, 1)
     lref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
     orc.lapl_euler(lref, phi, c, 0.01)
-    saved = nh.stencil_get_tile(0), nh.stencil_get_tile(1)
     try:
         for tile in nh.stencil_tiles():
             nh.stencil_set_tile(0, *tile)
@@ -422,8 +421,45 @@ def test_every_stencil_tile_bit_exact(lengths):
         best, times = nh.stencil_autotune(gpu_empty_like(L), None, phi, c, kernel=nh.KERNEL_LAPL)
         assert best in nh.stencil_tiles() and nh.stencil_get_tile(1) == best
     finally:
-        nh.stencil_set_tile(0, *saved[0])
-        nh.stencil_set_tile(1, *saved[1])
+        nh.stencil_reset_tile(0)
+        nh.stencil_reset_tile(1)
+
+
+def test_tile_selection_by_box_size():
+    """Without a pinned or tuned tile, boxes that the default tile fills (at
+    least half the CUs get a workgroup) run it, smaller ones the direct
+    schedule; a pin applies to every box; a tuned tile to its box size only."""
+    nh.stencil_reset_tile(0)
+    nh.stencil_reset_tile(1)
+    default = (128, 8, 64, 7)
+    try:
+        for kern in (0, 1):
+            assert nh.stencil_select_tile(kern, (256, 256, 256)) == default
+            assert nh.stencil_select_tile(kern, (512, 512, 512)) == default
+            assert nh.stencil_select_tile(kern, (128, 128, 128)) == (64, 8, 16, 0)
+            assert nh.stencil_select_tile(kern, (64, 64, 64)) == (64, 4, 16, 0)
+        nh.stencil_set_tile(0, 64, 4, 24, 3)
+        assert nh.stencil_select_tile(0, (64, 64, 64)) == (64, 4, 24, 3)
+        assert nh.stencil_select_tile(0, (256, 256, 256)) == (64, 4, 24, 3)
+        assert nh.stencil_select_tile(1, (256, 256, 256)) == default
+        n = 24
+        L = nh.Layout.build((n, n, n), 1)
+        rng = np.random.default_rng(2)
+        k = both(L, rng.uniform(0.5, 2.0, L.npts))
+        phi = both(L, rng.standard_normal(L.npts))
+        c = nh.axis_coefs(nh.build_stencils(n, n, n, 1.0, 1.0, 1.0))
+        best, _ = nh.stencil_autotune(gpu_empty_like(L), k, phi, c)
+        assert nh.stencil_select_tile(0, (n, n, n)) == best and nh.stencil_get_tile(0) == best
+        assert nh.stencil_select_tile(0, (128, 128, 128)) == (64, 8, 16, 0)   # the pin is gone
+        # results do not depend on the selection
+        ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+        orc.divkgrad(ref, k, phi, c, 0)
+        r = gpu_empty_like(L)
+        nh.divkgrad(r, k, phi, c, 0)
+        assert np.array_equal(fetch(r), ref.host)
+    finally:
+        nh.stencil_reset_tile(0)
+        nh.stencil_reset_tile(1)
 
 
 @pytest.mark.parametrize("fn,budget", [("SIN", 1), ("COS", 1), ("TAN", 2), ("EXP", 1), ("TANH", 2),
