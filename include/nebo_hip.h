@@ -339,10 +339,12 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  *   1. the tile pinned by nebo_stencil_set_tile, for every box;
  *   2. else the tile nebo_stencil_autotune found fastest for a box of the same
  *      dimensions on the same device;
- *   3. else 128 x 8 x 64, variant 7, when it gives at least half the device's
- *      CUs a workgroup (256^3 and up), and otherwise the direct schedule with
- *      64 x 8 x 16 tiles, or 64 x 4 x 16 when those give fewer workgroups than
- *      CUs (small boxes).
+ *   3. else 128 x 8 tiles, variant 7, with the longest z chunk of 512, 256 or
+ *      128 planes that gives every CU a workgroup (512^3: 512 planes);
+ *   4. else 128 x 8 x 64, variant 7, when it gives at least half the device's
+ *      CUs a workgroup (256^3);
+ *   5. else the direct schedule with 64 x 8 x 16 tiles, or 64 x 4 x 16 when those
+ *      give fewer workgroups than CUs (small boxes).
  * nebo_stencil_set_tile with tx = ty = zchunk = variant = 0 removes the pin.
  * nebo_stencil_get_tile reports the tile last pinned or tuned (the default tile
  * if neither).  The tile changes only the schedule, never the results.  The
@@ -353,7 +355,7 @@ int nebo_stencil_tiles(int32_t* n, int32_t* tx, int32_t* ty, int32_t* zchunk,
 int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t zchunk, int32_t variant);
 int nebo_stencil_get_tile(int kernel, int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant);
 /* The tile a launch of `kernel` over an nx x ny x nz box on `device` would use
- * (rules 1-3 above). */
+ * (rules 1-5 above). */
 int nebo_stencil_select_tile(int kernel, int device, int32_t nx, int32_t ny, int32_t nz, int32_t* tx,
                              int32_t* ty, int32_t* zchunk, int32_t* variant);
 /* Time every instantiated tile of `kernel` on these operands (r is overwritten on
@@ -454,6 +456,13 @@ int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk,
 /* scalar transport chain: upwinded convection + var-coefficient diffusion,   */
 /* SSP-RK3, periodic, and its adjoint                                         */
 /* ------------------------------------------------------------------------- */
+
+/* Planes each workgroup of the transport stage and adjoint-stage kernels
+ * marches (64 x 4 columns per workgroup); 0 restores the default, 16 (at 512^3
+ * on MI355X, 16 to 128 planes measure within 3 % of each other, 256 and 512
+ * slower).  Changes only the schedule, never the results.  No reference
+ * counterpart. */
+int nebo_transport_set_zchunk(int32_t planes);
 
 /* One SSP-RK3 stage of R(s) = div(interp(k)*grad(s)) - div(u*upwind(s)), where
  * upwind(s) on a face is cond(u > 0, s of the cell below)(s of the cell above):

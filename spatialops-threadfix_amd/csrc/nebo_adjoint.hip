@@ -96,8 +96,8 @@ __global__ __launch_bounds__(kTx * kTy) void adjoint_kernel(
   const int x = tx * kTx + (threadIdx.x & (kTx - 1));
   const int y = ty * kTy + (threadIdx.x / kTx);
   if (x >= B.n[0] || y >= B.n[1]) return;
-  const int z0 = tz * kZChunk;
-  const int z1 = min(z0 + kZChunk, B.n[2]);
+  const int z0 = tz * B.zc;
+  const int z1 = min(z0 + B.zc, B.n[2]);
   const double i0x = c.interp[0][0], i1x = c.interp[0][1], g0x = c.grad[0][0], g1x = c.grad[0][1];
   const double i0y = c.interp[1][0], i1y = c.interp[1][1], g0y = c.grad[1][0], g1y = c.grad[1][1];
   const double i0z = c.interp[2][0], i1z = c.interp[2][1], g0z = c.grad[2][0], g1z = c.grad[2][1];
@@ -155,6 +155,7 @@ int adjoint_filled(const nebo_field_t* lam_out, double* gk, const nebo_field_t* 
   B.nx = (B.n[0] + kTx - 1) / kTx;
   B.ny = (B.n[1] + kTy - 1) / kTy;
   B.nz = (B.n[2] + kZChunk - 1) / kZChunk;
+  B.zc = kZChunk;
   const int nb = B.nx * B.ny * B.nz;
   const int X = lam_out->glob[0], XY = lam_out->glob[0] * lam_out->glob[1];
   // gk has k's shape: the same frame offset as k

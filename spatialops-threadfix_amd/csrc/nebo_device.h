@@ -20,12 +20,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 struct Box3 {
   int n[3];
   int nx, ny, nz;  // tile counts
+  int zc;          // planes per tile
 };
 
 inline Box3 make_box(const int n[3], int tx, int ty, int zc)
 {
   Box3 B;
   for (int d = 0; d < 3; ++d) B.n[d] = n[d];
+  B.zc = zc;
   B.nx = (n[0] + tx - 1) / tx;
   B.ny = (n[1] + ty - 1) / ty;
   B.nz = (n[2] + zc - 1) / zc;

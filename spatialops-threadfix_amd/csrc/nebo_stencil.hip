@@ -693,6 +693,7 @@ constexpr TileShape kShapes[] = {
     {128, 16, 32, 5},
     {64, 8, 32, 6},   {64, 16, 32, 6},
     {128, 8, 32, 7},  {128, 8, 64, 7},  {64, 16, 64, 7},  {256, 4, 64, 7},  {128, 4, 128, 7},
+    {128, 8, 128, 7}, {128, 8, 256, 7}, {128, 8, 512, 7},
     {128, 8, 64, 8}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // The tile used until a caller sets or tunes one: 128 x 8 rows (2 x 2 waves)
@@ -778,10 +779,16 @@ LaplFn lapl_table(int s) { return lapl_pick(s, std::make_index_sequence<kNumShap
 //  1. a tile pinned with nebo_stencil_set_tile applies to every box;
 //  2. else the tile nebo_stencil_autotune found fastest for a box of the same
 //     dimensions on the same device;
-//  3. else the default tile when it gives at least half the CUs a workgroup;
-//     smaller boxes run the direct schedule, whose short tiles fill the GPU
-//     (measured on MI355X: 64^3 8.3 us, 128^3 14.5 us against 51 us for the
-//     default tile, which gives them 32 workgroups).
+//  3. else, on boxes large enough, the default tile's family (128 x 8, loads 3
+//     planes ahead) with the longest z chunk (512, 256 or 128 planes) that still
+//     gives every CU a workgroup: one workgroup per CU, all walking the same
+//     planes (512^3 on MI355X: 658 us against 775 us with 64-plane chunks, whose
+//     eight concurrent z slabs lie 135 MB apart);
+//  4. else the default tile when it gives at least half the CUs a workgroup
+//     (256^3: 64-plane chunks, one workgroup per CU);
+//  5. smaller boxes run the direct schedule, whose short tiles fill the GPU
+//     (64^3 8.3 us, 128^3 14.5 us against 51 us for the default tile, which
+//     gives them 32 workgroups).
 std::atomic<int> g_pinned[2] = {{-1}, {-1}};
 std::atomic<int> g_last[2] = {{kDefaultShape}, {kDefaultShape}};   // last pinned or tuned tile
 struct TunedTile {
@@ -791,7 +798,11 @@ std::mutex g_tuned_mu;
 std::vector<TunedTile> g_tuned;
 
 constexpr int kSmallWide = find_shape_c(64, 8, 16, 0), kSmallNarrow = find_shape_c(64, 4, 16, 0);
-static_assert(kSmallWide >= 0 && kSmallNarrow >= 0, "small-box tiles");
+constexpr int kLongChunks[] = {find_shape_c(128, 8, 512, 7), find_shape_c(128, 8, 256, 7),
+                               find_shape_c(128, 8, 128, 7)};
+static_assert(kSmallWide >= 0 && kSmallNarrow >= 0 && kLongChunks[0] >= 0 && kLongChunks[1] >= 0 &&
+                  kLongChunks[2] >= 0,
+              "box-size tiles");
 
 constexpr int kMaxDevices = 64;
 std::atomic<int> g_cus[kMaxDevices];
@@ -824,6 +835,8 @@ int select_shape(int kernel, int device, const nebo_limits_t& lim)
         return t.shape;
   }
   const int cus = device_cus(device);
+  for (int sh : kLongChunks)
+    if (tile_groups(sh, lim) >= cus) return sh;
   if (2 * tile_groups(kDefaultShape, lim) >= cus) return kDefaultShape;
   return tile_groups(kSmallWide, lim) >= cus ? kSmallWide : kSmallNarrow;
 }

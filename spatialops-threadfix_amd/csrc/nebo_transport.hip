@@ -26,6 +26,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 namespace nebo {
 namespace {
 
@@ -69,8 +71,8 @@ __global__ __launch_bounds__(kTx * kTy) void transport_stage_kernel(
   const int x = tx * kTx + (threadIdx.x & (kTx - 1));
   const int y = ty * kTy + (threadIdx.x / kTx);
   if (x >= B.n[0] || y >= B.n[1]) return;
-  const int z0 = tz * kZChunk;
-  const int z1 = min(z0 + kZChunk, B.n[2]);
+  const int z0 = tz * B.zc;
+  const int z1 = min(z0 + B.zc, B.n[2]);
   const AxisC cx = C.a[0], cy = C.a[1], cz = C.a[2];
 
   int i = x + X * y + XY * z0;
@@ -148,8 +150,8 @@ __global__ __launch_bounds__(kTx * kTy) void transport_adjoint_kernel(
   const int x = tx * kTx + (threadIdx.x & (kTx - 1));
   const int y = ty * kTy + (threadIdx.x / kTx);
   if (x >= B.n[0] || y >= B.n[1]) return;
-  const int z0 = tz * kZChunk;
-  const int z1 = min(z0 + kZChunk, B.n[2]);
+  const int z0 = tz * B.zc;
+  const int z1 = min(z0 + B.zc, B.n[2]);
   const AxisC cx = C.a[0], cy = C.a[1], cz = C.a[2];
 
   int i = x + X * y + XY * z0;
@@ -187,10 +189,12 @@ struct Operands {
   const nebo_field_t* k;
 };
 
+std::atomic<int> g_zchunk{0};   // planes per workgroup; 0: kZChunk
+
 Box3 interior_box(const nebo_field_t* f)
 {
   const int n[3] = {f->extent[0] - 2, f->extent[1] - 2, f->extent[2] - 2};
-  return make_box(n, kTx, kTy, kZChunk);
+  return make_box(n, kTx, kTy, g_zchunk.load() > 0 ? g_zchunk.load() : kZChunk);
 }
 
 int launch_stage(const nebo_field_t* out, const nebo_field_t* pn, const nebo_field_t* s,
@@ -262,6 +266,15 @@ int rk3_step(double* next, double* s1, double* s2, const double* phin, const neb
 }  // namespace nebo
 
 using namespace nebo;
+
+extern "C" int nebo_transport_set_zchunk(int32_t planes)
+{
+  clear_error();
+  if (planes < 0 || planes > 4096)
+    return set_error(NEBO_ERR_INVALID, kCtx, "z chunk: 0 (the default) or 1 to 4096 planes");
+  g_zchunk.store(planes);
+  return NEBO_OK;
+}
 
 extern "C" int nebo_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
                                     const nebo_field_t* s, const nebo_field_t* k,

@@ -30,6 +30,11 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def set_zchunk(planes=0):
+    """Planes each workgroup of the stage kernels marches (0: the default, 16)."""
+    _abi.call("nebo_transport_set_zchunk", int(planes))
+
+
 def stage(layout, out, phin, s, k, us, coefs, dt, stage, face_layouts=None, stream=None):
     """out <<= stage `stage` (1, 2, 3) of SSP-RK3, then out's periodic ghosts.
     Operands are Fields or flat tensors of `layout` (faces: of `face_layouts`)."""

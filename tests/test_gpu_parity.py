@@ -435,7 +435,8 @@ def test_tile_selection_by_box_size():
     try:
         for kern in (0, 1):
             assert nh.stencil_select_tile(kern, (256, 256, 256)) == default
-            assert nh.stencil_select_tile(kern, (512, 512, 512)) == default
+            assert nh.stencil_select_tile(kern, (512, 512, 512)) == (128, 8, 512, 7)
+            assert nh.stencil_select_tile(kern, (384, 384, 384)) == (128, 8, 256, 7)
             assert nh.stencil_select_tile(kern, (128, 128, 128)) == (64, 8, 16, 0)
             assert nh.stencil_select_tile(kern, (64, 64, 64)) == (64, 4, 16, 0)
         nh.stencil_set_tile(0, 64, 4, 24, 3)

@@ -28,8 +28,16 @@ def _periodic_random(L, rng):
     return inputs.periodic_fill(a, 1).reshape(-1)
 
 
+@pytest.fixture(params=[0, 5, 64], ids=["zchunk-auto", "zchunk-5", "zchunk-64"])
+def zchunk(request):
+    """The stage kernels with several z chunks (5 does not divide the boxes)."""
+    tr.set_zchunk(request.param)
+    yield request.param
+    tr.set_zchunk(0)
+
+
 @pytest.mark.parametrize("n", [20, 37])
-def test_stage_kernel_matches_generic_evaluator_and_oracle(n):
+def test_stage_kernel_matches_generic_evaluator_and_oracle(n, zchunk):
     L, Ls, ops, k, phi, u, dt, c = _problem(n)
     rng = np.random.default_rng(n)
     s = _periodic_random(L, rng)
@@ -57,7 +65,7 @@ def test_stage_kernel_matches_generic_evaluator_and_oracle(n):
 
 
 @pytest.mark.parametrize("n", [16, 23])
-def test_adjoint_stage_bit_exact(n):
+def test_adjoint_stage_bit_exact(n, zchunk):
     L, Ls, ops, k, phi, u, dt, c = _problem(n)
     rng = np.random.default_rng(100 + n)
     a, s, acc_in = _periodic_random(L, rng), _periodic_random(L, rng), rng.standard_normal(L.npts)
