@@ -387,7 +387,9 @@ typedef struct nebo_batch nebo_batch_t;   /* opaque */
 
 /* Build a device patch table and the tile schedule for `n` patches, all on the
  * current device.  tile[3] is the (x, y, z) extent of one work item (x*y <= 1024
- * threads); tile == NULL starts from (64, 4, 16).  Coefficients and s are shared
+ * threads), or with y = 0 a flattened item of x consecutive points of a patch's
+ * x-y plane (x a multiple of 64 up to 1024) by z planes, which leaves no lane
+ * idle on patches narrower than the tile; tile == NULL starts from (64, 4, 16).  Coefficients and s are shared
  * by all patches: out = in + s*((Lx + Ly) + Lz) as in nebo_lapl_euler. */
 int nebo_batch_create(const nebo_patch_t* patches, int32_t n,
                       const nebo_axis_coefs_t* c, double s,

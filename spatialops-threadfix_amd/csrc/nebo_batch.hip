@@ -3,7 +3,8 @@
 // Uintah hands SpatialOps one patch at a time; on a GPU that means one small
 // launch per patch, each far too small to fill 256 CUs.  Here the patches live
 // in a device table and the step is cut into work items (patch, x0, y0, z0) of a
-// fixed tile shape.  One persistent launch of ~4 workgroups per CU walks the item
+// fixed tile shape, or (patch, q0, z0) runs of consecutive points of a patch's
+// flattened x-y plane.  One persistent launch of ~4 workgroups per CU walks the item
 // list with a fixed stride, so every patch of every size is covered by one
 // launch, and a hipGraph holds several such steps.  The tile shape is chosen by
 // timing candidates on the device (nebo_batch_autotune).
@@ -49,6 +50,68 @@ __device__ __forceinline__ void patch_column(const double* __restrict__ in, doub
     out[i] = pc + s * ((Lx + Ly) + Lz);
     pc = pz;
     gz_lo = gz_hi;
+  }
+}
+
+// patch_column for a full chunk of TZ planes: every read of the chunk (TZ + 2
+// planes of the column, 4 x/y neighbours per plane) is issued before the first
+// cell is computed, so one memory latency covers TZ cells; the arithmetic is
+// patch_column's, in the same order.
+template <int TZ>
+__device__ __forceinline__ void patch_chunk(const double* __restrict__ in, double* __restrict__ out, int i,
+                                            int XY, int X, const Coef6& c, double s)
+{
+  double p[TZ + 2], xm[TZ], xp[TZ], ym[TZ], yp[TZ];
+#pragma unroll
+  for (int z = 0; z < TZ + 2; ++z) p[z] = in[i + (z - 1) * XY];
+#pragma unroll
+  for (int z = 0; z < TZ; ++z) {
+    const int j = i + z * XY;
+    xm[z] = in[j - 1];
+    xp[z] = in[j + 1];
+    ym[z] = in[j - X];
+    yp[z] = in[j + X];
+  }
+  double gz_lo = p[0] * c.gz0 + p[1] * c.gz1;
+#pragma unroll
+  for (int z = 0; z < TZ; ++z) {
+    const double pc = p[z + 1], pz = p[z + 2];
+    const double gz_hi = pc * c.gz0 + pz * c.gz1;
+    const double Lx = (xm[z] * c.gx0 + pc * c.gx1) * c.dx0 + (pc * c.gx0 + xp[z] * c.gx1) * c.dx1;
+    const double Ly = (ym[z] * c.gy0 + pc * c.gy1) * c.dy0 + (pc * c.gy0 + yp[z] * c.gy1) * c.dy1;
+    const double Lz = gz_lo * c.dz0 + gz_hi * c.dz1;
+    out[i + z * XY] = pc + s * ((Lx + Ly) + Lz);
+    gz_lo = gz_hi;
+  }
+}
+
+// Flattened-plane items (tile y == 0): w = (patch, q0, -, z0) covers the tile-x
+// consecutive points q0 .. q0 + tile_x - 1 of the patch's x-y plane (q = x + nx*y),
+// so no lane idles on a patch narrower than the tile, and tz planes.  TZ > 0 is
+// tz known at compile time: full chunks take patch_chunk, a chunk cut by the
+// patch's end (uniform over the workgroup) the loop.
+template <int TZ>
+__global__ __launch_bounds__(1024) void batch_flat_kernel(const DevPatch* __restrict__ P,
+                                                          const int4* __restrict__ items, int n_items,
+                                                          int tz, Coef6 c, double s, int swap)
+{
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int4 w = items[it];
+    const DevPatch p = P[w.x];
+    const int q = w.y + (int)threadIdx.x;
+    if (q >= p.nx * p.ny) continue;
+    const int y = q / p.nx, x = q - y * p.nx;
+    const double* in = swap ? p.b : p.a;
+    double* out = swap ? p.a : p.b;
+    const int zn = min(tz, p.nz - w.w);
+    const int i = p.origin + x + p.X * y + p.XY * w.w;
+    if constexpr (TZ > 0) {
+      if (zn == TZ) {
+        patch_chunk<TZ>(in, out, i, p.XY, p.X, c, s);
+        continue;
+      }
+    }
+    patch_column(in, out, i, p.XY, p.X, zn, c, s);
   }
 }
 
@@ -110,9 +173,14 @@ int build_items(nebo_batch* b)
   for (size_t k = 0; k < b->patches.size(); ++k) {
     b->patch_first[k] = (int)items.size();
     const nebo_patch_t& p = b->patches[k];
-    for (int z = 0; z < p.n[2]; z += tz)
-      for (int y = 0; y < p.n[1]; y += ty)
-        for (int x = 0; x < p.n[0]; x += tx) items.push_back(make_int4((int)k, x, y, z));
+    for (int z = 0; z < p.n[2]; z += tz) {
+      if (ty == 0) {   // flattened plane
+        for (int q = 0; q < p.n[0] * p.n[1]; q += tx) items.push_back(make_int4((int)k, q, 0, z));
+      } else {
+        for (int y = 0; y < p.n[1]; y += ty)
+          for (int x = 0; x < p.n[0]; x += tx) items.push_back(make_int4((int)k, x, y, z));
+      }
+    }
   }
   b->patch_first[b->patches.size()] = (int)items.size();
   if (b->d_items) {
@@ -128,7 +196,7 @@ int build_items(nebo_batch* b)
   rc = check_hip(hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice), ctx);
   if (rc) return rc;
   // persistent grid: a few workgroups per CU, never more than there are items
-  const int threads = tx * ty;
+  const int threads = ty == 0 ? tx : tx * ty;
   const int per_cu = threads <= 256 ? 8 : threads <= 512 ? 4 : 2;
   b->blocks = std::max(1, std::min(b->n_items, b->cus * per_cu));
   return NEBO_OK;
@@ -144,6 +212,12 @@ void drop_graph(nebo_batch* b)
 
 int check_tile(const int32_t* t)
 {
+  if (t[1] == 0) {
+    if (t[0] < 64 || t[0] > 1024 || t[0] % 64 || t[2] < 1)
+      return set_error(NEBO_ERR_INVALID, "Nebo Batch",
+                       "a flattened tile (y = 0) takes a multiple of 64 threads up to 1024 and z >= 1");
+    return NEBO_OK;
+  }
   if (t[0] < 1 || t[1] < 1 || t[2] < 1 || t[0] * t[1] > 1024)
     return set_error(NEBO_ERR_INVALID, "Nebo Batch", "tile must be positive with x*y <= 1024 threads");
   return NEBO_OK;
@@ -152,9 +226,17 @@ int check_tile(const int32_t* t)
 int launch_items(nebo_batch* b, int first, int count, int grid, int swap, hipStream_t s)
 {
   if (count <= 0) return NEBO_OK;
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(b->tile[0] * b->tile[1]), 0, s, b->d_patches,
-                     b->d_items + first, count, b->tile[0], b->tile[1], b->tile[2], coef6(b->coefs), b->s,
-                     swap);
+  if (b->tile[1] == 0) {
+    const int tz = b->tile[2];
+    auto fn = tz == 4 ? batch_flat_kernel<4> : tz == 8 ? batch_flat_kernel<8> : tz == 16 ? batch_flat_kernel<16>
+                                                                                         : batch_flat_kernel<0>;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(b->tile[0]), 0, s, b->d_patches, b->d_items + first, count, tz,
+                       coef6(b->coefs), b->s, swap);
+  } else {
+    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(b->tile[0] * b->tile[1]), 0, s, b->d_patches,
+                       b->d_items + first, count, b->tile[0], b->tile[1], b->tile[2], coef6(b->coefs), b->s,
+                       swap);
+  }
   return check_hip(hipGetLastError(), "Nebo Batch");
 }
 
@@ -278,7 +360,10 @@ extern "C" int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* str
   static const int cand[][3] = {
       {64, 4, 16}, {64, 4, 32}, {64, 4, 8}, {64, 2, 16}, {64, 8, 16}, {64, 8, 8}, {64, 16, 8},
       {32, 8, 16}, {32, 8, 8}, {32, 4, 16}, {32, 16, 8}, {16, 16, 16}, {16, 16, 8}, {16, 8, 16},
-      {64, 1, 64}, {128, 2, 16}, {128, 4, 8}, {256, 1, 16}};
+      {64, 1, 64}, {128, 2, 16}, {128, 4, 8}, {256, 1, 16},
+      // flattened plane (y = 0)
+      {64, 0, 16}, {128, 0, 8}, {128, 0, 16}, {256, 0, 4}, {256, 0, 8}, {256, 0, 16}, {512, 0, 4},
+      {512, 0, 8}, {1024, 0, 4}, {1024, 0, 8}};
   hipEvent_t e0, e1;
   int rc = check_hip(hipEventCreate(&e0), ctx);
   if (!rc) rc = check_hip(hipEventCreate(&e1), ctx);
