@@ -100,7 +100,6 @@ std::atomic<int> g_compiled{0}, g_hits{0}, g_failures{0};
 struct Entry {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
-  int V = 1;
 };
 std::mutex g_mu;
 std::unordered_map<std::string, Entry> g_cache;   // key: device + source
@@ -459,7 +458,7 @@ int compile(const std::string& src, const char* name, int device, Entry* e)
 // Compiled kernel for src, from the cache or compiled now; fn stays null when
 // compilation failed (remembered, so the interpreter runs such trees without
 // another attempt).
-int get_kernel(const std::string& src, const char* name, int V, Entry* out)
+int get_kernel(const std::string& src, const char* name, Entry* out)
 {
   static const char* ctx = "Nebo Fused Kernel";
   int device = 0;
@@ -474,7 +473,6 @@ int get_kernel(const std::string& src, const char* name, int V, Entry* out)
     return NEBO_OK;
   }
   Entry fresh;
-  fresh.V = V;
   if (compile(src, name, device, &fresh)) {
     ++g_failures;
     clear_error();
@@ -550,7 +548,7 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
   const int V = choose_cells(P);
   const Shape sh = current_shape();
   Entry e;
-  int rc = get_kernel(jit_source(P, V, sh), "nebo_fused", V, &e);
+  int rc = get_kernel(jit_source(P, V, sh), "nebo_fused", &e);
   if (rc || !e.fn) return rc;
   JitArgs A;
   fill_args(P, lim, V, &A);
@@ -581,7 +579,7 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
   if (!jit_enabled()) return NEBO_OK;
   const int V = choose_cells(P);
   Entry e;
-  int rc = get_kernel(jit_reduce_source(P, V, op), "nebo_reduce", V, &e);
+  int rc = get_kernel(jit_reduce_source(P, V, op), "nebo_reduce", &e);
   if (rc || !e.fn) return rc;
   JitArgs A;
   fill_args(P, lim, V, &A);

@@ -696,10 +696,10 @@ constexpr TileShape kShapes[] = {
     {128, 8, 128, 7}, {128, 8, 256, 7}, {128, 8, 512, 7},
     {128, 8, 64, 8}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
-// The tile used until a caller sets or tunes one: 128 x 8 rows (2 x 2 waves)
+// The default tile (rule 4 of select_shape): 128 x 8 rows (2 x 2 waves)
 // marching 64 planes with loads 3 planes ahead, the fastest at 256^3 on MI355X
-// (about one workgroup per CU there).  Planes too large for the column
-// schedule's 32-bit offsets run the direct schedule (kFallbackShape).
+// (one workgroup per CU there).  Planes too large for the column schedule's
+// 32-bit offsets run the direct schedule (kFallbackShape).
 constexpr int find_shape_c(int tx, int ty, int zc, int variant, int s = 0)
 {
   return s >= kNumShapes ? -1
