@@ -459,12 +459,20 @@ int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk,
 /* SSP-RK3, periodic, and its adjoint                                         */
 /* ------------------------------------------------------------------------- */
 
-/* Planes each workgroup of the transport stage and adjoint-stage kernels
- * marches (64 x 4 columns per workgroup); 0 restores the default, 16 (at 512^3
- * on MI355X, 16 to 128 planes measure within 3 % of each other, 256 and 512
- * slower).  Changes only the schedule, never the results.  No reference
+/* Schedule of the transport stage kernel: 0 chosen by box size (the default),
+ * 1 direct (64 x 4 columns of cells per workgroup, every neighbour loaded),
+ * 2 column tiles of 128 x 8 cells (each lane a column of 4 rows; x neighbours by
+ * lane exchange, y neighbours from the lane's rows) with loads 2 planes ahead,
+ * 3 the same with loads 1 plane ahead.  By box size: column tiles with loads 1
+ * plane ahead and the longest z chunk of 512, 256, 128 or 64 planes that gives
+ * every CU a workgroup, else the direct schedule.  zchunk: planes per workgroup, 0 for the schedule's
+ * default (16 for the direct one).  The adjoint stage always runs the direct
+ * schedule (a zchunk set with schedule 1 applies to it).  Changes only the
+ * schedule, never the results.  nebo_transport_schedule reports the schedule
+ * and chunk a stage over `field`'s interior would use.  No reference
  * counterpart. */
-int nebo_transport_set_zchunk(int32_t planes);
+int nebo_transport_tune(int32_t schedule, int32_t zchunk);
+int nebo_transport_schedule(const nebo_field_t* field, int32_t* schedule, int32_t* zchunk);
 
 /* One SSP-RK3 stage of R(s) = div(interp(k)*grad(s)) - div(u*upwind(s)), where
  * upwind(s) on a face is cond(u > 0, s of the cell below)(s of the cell above):

@@ -3,7 +3,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <utility>
+
+#include "../../include/nebo_hip.h"
+
 namespace nebo {
+
+constexpr int kLanes = 64;   // wavefront width of gfx9 (wave64)
 
 // Hardware dispatches workgroup b to XCD b % 8.  Map b to a logical tile so that
 // each XCD receives one contiguous run of logical tiles (neighbouring tiles share
@@ -41,6 +48,118 @@ __device__ __forceinline__ double face_flux(double klo, double khi, double plo, 
   const double ki = klo * i0 + khi * i1;
   const double gr = plo * g0 + phi * g1;
   return ki * gr;
+}
+
+// ---- column-schedule helpers (shared by the stencil and transport kernels) ----
+
+struct Tiles {
+  int lo[3], hi[3];
+  int nx, ny, nz;  // tile counts
+};
+
+inline Tiles make_tiles(const nebo_limits_t& L, int tx, int ty, int zc)
+{
+  Tiles t;
+  for (int d = 0; d < 3; ++d) { t.lo[d] = L.lo[d]; t.hi[d] = L.hi[d]; }
+  t.nx = (L.hi[0] - L.lo[0] + tx - 1) / tx;
+  t.ny = (L.hi[1] - L.lo[1] + ty - 1) / ty;
+  t.nz = (L.hi[2] - L.lo[2] + zc - 1) / zc;
+  return t;
+}
+
+
+// Lane l receives lane l-1's value (DPP wave_shr:1); lane 0 receives 0 (every
+// caller replaces lane 0's value).
+__device__ __forceinline__ double from_lower_lane(double v)
+{
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
+}
+
+// Lane l receives lane l+1's value (DPP wave_shl:1); lane 63 receives 0.
+__device__ __forceinline__ double from_upper_lane(double v)
+{
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
+}
+
+
+// Lane j's value of v (v_readlane; j is uniform), as seen by every lane.
+__device__ __forceinline__ double lane_value(double v, int j)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
+
+
+// Loads of one column-schedule lane within a plane, clamped into [lo-1, hi] on
+// every axis: rows y0-1 .. y0+RY of its column, and one edge load: lanes 0..RY-1
+// take the left neighbours (x = xw0-1) of rows 0..RY-1, lanes 32..32+RY-1 the
+// right neighbours (x = xw0+64), other lanes re-read their row 0.  Offsets are in
+// bytes from the plane's base (the frame origin of the plane, minus X + 1), so
+// they are non-negative 32-bit values and every load is base (uniform) + offset;
+// for an active lane the offset of row j+1 is also the store offset of row j.
+template <int RY>
+struct ColumnGeom {
+  uint32_t row[RY + 2];
+  uint32_t edge;
+  uint32_t inner;   // lanes 0..RY-1: (xw0, row); lanes 32..32+RY-1: (xw0+63, row); else row 0
+};
+
+template <int RY>
+__device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int xw0, int y0, int lane)
+{
+  ColumnGeom<RY> g;
+  auto off = [X](int x, int y) { return (uint32_t)((x + 1) + X * (y + 1)) * 8u; };
+  const int xc = min(xw0 + lane, T.hi[0]);
+#pragma unroll
+  for (int r = 0; r < RY + 2; ++r) g.row[r] = off(xc, min(y0 - 1 + r, T.hi[1]));
+  // (a wave wholly beyond hi, in a partial last tile, has xw0 - 1 > hi: clamp too)
+  if (lane < RY) g.edge = off(min(xw0 - 1, T.hi[0]), min(y0 + lane, T.hi[1]));
+  else if (lane >= 32 && lane < 32 + RY) g.edge = off(min(xw0 + kLanes, T.hi[0]), min(y0 + lane - 32, T.hi[1]));
+  else g.edge = g.row[1];
+  if (lane < RY) g.inner = off(min(xw0, T.hi[0]), min(y0 + lane, T.hi[1]));
+  else if (lane >= 32 && lane < 32 + RY) g.inner = off(min(xw0 + kLanes - 1, T.hi[0]), min(y0 + lane - 32, T.hi[1]));
+  else g.inner = g.row[1];
+  return g;
+}
+
+// Plane z's base: frame origin + XY*z - X - 1 (uniform across the wave).
+template <typename T>
+__device__ __forceinline__ T* plane_base(T* origin, int X, int XY, int z)
+{
+  return origin + ((long)XY * z - X - 1);
+}
+
+// Raw buffer resource over one plane (base from plane_base): loads and stores
+// take the lane's 32-bit byte offset, so no per-lane address arithmetic is left
+// in the loop.  The range, (XY + 2X + 2) doubles, covers every offset the column
+// geometry forms (the hardware returns 0 for a load beyond it instead of
+// faulting; the launcher keeps it below 2^31 bytes).  Word 3 is the gfx9 raw
+// buffer format.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+using U2 = __attribute__((__vector_size__(2 * sizeof(unsigned int)))) unsigned int;
+
+__device__ __forceinline__ Rsrc plane_rsrc(const double* base, int X, int XY)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, (XY + 2 * X + 2) * 8, 0x00020000);
+}
+
+__device__ __forceinline__ double load_at(Rsrc rs, uint32_t off)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+}
+
+// nontemporal store (cache-policy bit 1: nt)
+__device__ __forceinline__ void store_nt_at(Rsrc rs, uint32_t off, double v)
+{
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, 2);
+}
+
+// Run f(integral_constant<int, I>) for I = 0, 1, ... in order (compile-time steps).
+template <typename F, int... I>
+__device__ __forceinline__ void unroll_steps(F&& f, std::integer_sequence<int, I...>)
+{
+  (f(std::integral_constant<int, I>{}), ...);
 }
 
 }  // namespace nebo

@@ -99,6 +99,9 @@ struct VmProgram {
 // Lower a tree to a stack program.  Also records the max stack depth.
 int lower_program(const nebo_expr_t* e, VmProgram* prog, int* max_depth);
 
+// Compute units of a GPU (cached per device; 256 if the query fails).
+int device_cu_count(int device);
+
 // Fused kernels compiled at run time (nebo_jit.hip): runs the program as one
 // generated kernel and sets *done, or leaves *done false when run-time
 // compilation is off or failed for this program (the interpreter then runs it).

@@ -32,6 +32,7 @@
 // tiles, which share halo rows, land on the same XCD and therefore the same L2.
 
 #include "nebo_internal.h"
+#include "nebo_device.h"
 
 #include <hip/hip_runtime.h>
 
@@ -45,52 +46,6 @@ namespace nebo {
 namespace {
 
 constexpr int kTx = 64;
-
-// Hardware dispatches workgroup b to XCD b % 8.  Map b to a logical tile so that
-// each XCD receives one contiguous run of logical tiles.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, k = b >> 3;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
-struct Tiles {
-  int lo[3], hi[3];
-  int nx, ny, nz;  // tile counts
-};
-
-Tiles make_tiles(const nebo_limits_t& L, int tx, int ty, int zc)
-{
-  Tiles t;
-  for (int d = 0; d < 3; ++d) { t.lo[d] = L.lo[d]; t.hi[d] = L.hi[d]; }
-  t.nx = (L.hi[0] - L.lo[0] + tx - 1) / tx;
-  t.ny = (L.hi[1] - L.lo[1] + ty - 1) / ty;
-  t.nz = (L.hi[2] - L.lo[2] + zc - 1) / zc;
-  return t;
-}
-
-// interp(k)*grad(phi) on one face: (k_lo*i0 + k_hi*i1) * (p_lo*g0 + p_hi*g1)
-__device__ __forceinline__ double face(double klo, double khi, double plo, double phi,
-                                       double i0, double i1, double g0, double g1)
-{
-  const double ki = klo * i0 + khi * i1;
-  const double gr = plo * g0 + phi * g1;
-  return ki * gr;
-}
-
-// Lane l receives lane l-1's value (DPP wave_shr:1); lane 0 receives 0 (every
-// caller replaces lane 0's value).
-__device__ __forceinline__ double from_lower_lane(double v)
-{
-  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
-}
-
-// Lane l receives lane l+1's value (DPP wave_shl:1); lane 63 receives 0.
-__device__ __forceinline__ double from_upper_lane(double v)
-{
-  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
-}
 
 // Where one thread of a staged tile loads within a plane.  Every coordinate is
 // clamped into [lo-1, hi] on its axis, the box the one-cell stencil reads, so a
@@ -160,16 +115,16 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_kernel(
 
   int i = x + X * y + XY * z0;
   double pc = p[i], kc = k[i];
-  double fz_lo = face(k[i - XY], kc, p[i - XY], pc, c.iz0, c.iz1, c.gz0, c.gz1);
+  double fz_lo = face_flux(k[i - XY], kc, p[i - XY], pc, c.iz0, c.iz1, c.gz0, c.gz1);
   for (int z = z0; z < z1; ++z, i += XY) {
     const double pz = p[i + XY], kz = k[i + XY];
     const double pxm = p[i - 1], pxp = p[i + 1], kxm = k[i - 1], kxp = k[i + 1];
     const double pym = p[i - X], pyp = p[i + X], kym = k[i - X], kyp = k[i + X];
-    const double fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
-    const double fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
-    const double fy_lo = face(kym, kc, pym, pc, c.iy0, c.iy1, c.gy0, c.gy1);
-    const double fy_hi = face(kc, kyp, pc, pyp, c.iy0, c.iy1, c.gy0, c.gy1);
-    const double fz_hi = face(kc, kz, pc, pz, c.iz0, c.iz1, c.gz0, c.gz1);
+    const double fx_lo = face_flux(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+    const double fx_hi = face_flux(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+    const double fy_lo = face_flux(kym, kc, pym, pc, c.iy0, c.iy1, c.gy0, c.gy1);
+    const double fy_hi = face_flux(kc, kyp, pc, pyp, c.iy0, c.iy1, c.gy0, c.gy1);
+    const double fz_hi = face_flux(kc, kz, pc, pz, c.iz0, c.iz1, c.gz0, c.gz1);
     const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
     const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
     const double Dz = fz_lo * c.dz0 + fz_hi * c.dz1;
@@ -234,7 +189,7 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_staged_kernel(
   stage(0, S[0]);
   __syncthreads();
 
-  double fz_lo = face(kzm, S[0].k, pzm, S[0].p, c.iz0, c.iz1, c.gz0, c.gz1);
+  double fz_lo = face_flux(kzm, S[0].k, pzm, S[0].p, c.iz0, c.iz1, c.gz0, c.gz1);
   int buf = 0;
   const int periods = (z1 - z0 + P - 1) / P;
   for (int q = 0; q < periods; ++q) {
@@ -249,11 +204,11 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_staged_kernel(
       const double kxm = lane == 0 ? ke : kl, kxp = lane == kTx - 1 ? ke : ku;
       const double pym = sp[buf][wy][lane], pyp = sp[buf][wy + 2][lane];
       const double kym = sk[buf][wy][lane], kyp = sk[buf][wy + 2][lane];
-      const double fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
-      const double fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
-      const double fy_lo = face(kym, kc, pym, pc, c.iy0, c.iy1, c.gy0, c.gy1);
-      const double fy_hi = face(kc, kyp, pc, pyp, c.iy0, c.iy1, c.gy0, c.gy1);
-      const double fz_hi = face(kc, B1.k, pc, B1.p, c.iz0, c.iz1, c.gz0, c.gz1);
+      const double fx_lo = face_flux(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+      const double fx_hi = face_flux(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+      const double fy_lo = face_flux(kym, kc, pym, pc, c.iy0, c.iy1, c.gy0, c.gy1);
+      const double fy_hi = face_flux(kc, kyp, pc, pyp, c.iy0, c.iy1, c.gy0, c.gy1);
+      const double fz_hi = face_flux(kc, B1.k, pc, B1.p, c.iz0, c.iz1, c.gz0, c.gz1);
       const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
       const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
       const double Dz = fz_lo * c.dz0 + fz_hi * c.dz1;
@@ -267,78 +222,6 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_staged_kernel(
       fz_lo = fz_hi;
     }
   }
-}
-
-// Lane j's value of v (v_readlane; j is uniform), as seen by every lane.
-__device__ __forceinline__ double lane_value(double v, int j)
-{
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
-  return __hiloint2double(hi, lo);
-}
-
-// Loads of one column-schedule lane within a plane, clamped into [lo-1, hi] on
-// every axis: rows y0-1 .. y0+RY of its column, and one edge load: lanes 0..RY-1
-// take the left neighbours (x = xw0-1) of rows 0..RY-1, lanes 32..32+RY-1 the
-// right neighbours (x = xw0+64), other lanes re-read their row 0.  Offsets are in
-// bytes from the plane's base (the frame origin of the plane, minus X + 1), so
-// they are non-negative 32-bit values and every load is base (uniform) + offset;
-// for an active lane the offset of row j+1 is also the store offset of row j.
-template <int RY>
-struct ColumnGeom {
-  uint32_t row[RY + 2];
-  uint32_t edge;
-  uint32_t inner;   // lanes 0..RY-1: (xw0, row); lanes 32..32+RY-1: (xw0+63, row); else row 0
-};
-
-template <int RY>
-__device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int xw0, int y0, int lane)
-{
-  ColumnGeom<RY> g;
-  auto off = [X](int x, int y) { return (uint32_t)((x + 1) + X * (y + 1)) * 8u; };
-  const int xc = min(xw0 + lane, T.hi[0]);
-#pragma unroll
-  for (int r = 0; r < RY + 2; ++r) g.row[r] = off(xc, min(y0 - 1 + r, T.hi[1]));
-  // (a wave wholly beyond hi, in a partial last tile, has xw0 - 1 > hi: clamp too)
-  if (lane < RY) g.edge = off(min(xw0 - 1, T.hi[0]), min(y0 + lane, T.hi[1]));
-  else if (lane >= 32 && lane < 32 + RY) g.edge = off(min(xw0 + kTx, T.hi[0]), min(y0 + lane - 32, T.hi[1]));
-  else g.edge = g.row[1];
-  if (lane < RY) g.inner = off(min(xw0, T.hi[0]), min(y0 + lane, T.hi[1]));
-  else if (lane >= 32 && lane < 32 + RY) g.inner = off(min(xw0 + kTx - 1, T.hi[0]), min(y0 + lane - 32, T.hi[1]));
-  else g.inner = g.row[1];
-  return g;
-}
-
-// Plane z's base: frame origin + XY*z - X - 1 (uniform across the wave).
-template <typename T>
-__device__ __forceinline__ T* plane_base(T* origin, int X, int XY, int z)
-{
-  return origin + ((long)XY * z - X - 1);
-}
-
-// Raw buffer resource over one plane (base from plane_base): loads and stores
-// take the lane's 32-bit byte offset, so no per-lane address arithmetic is left
-// in the loop.  The range, (XY + 2X + 2) doubles, covers every offset the column
-// geometry forms (the hardware returns 0 for a load beyond it instead of
-// faulting; the launcher keeps it below 2^31 bytes).  Word 3 is the gfx9 raw
-// buffer format.
-using Rsrc = __amdgpu_buffer_rsrc_t;
-using U2 = __attribute__((__vector_size__(2 * sizeof(unsigned int)))) unsigned int;
-
-__device__ __forceinline__ Rsrc plane_rsrc(const double* base, int X, int XY)
-{
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, (XY + 2 * X + 2) * 8, 0x00020000);
-}
-
-__device__ __forceinline__ double load_at(Rsrc rs, uint32_t off)
-{
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
-}
-
-// nontemporal store (cache-policy bit 1: nt)
-__device__ __forceinline__ void store_nt_at(Rsrc rs, uint32_t off, double v)
-{
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, 2);
 }
 
 // Column schedule (variant 3): W waves side by side in x (a 64*W-wide strip, so
@@ -366,12 +249,6 @@ __device__ __forceinline__ void store_nt_at(Rsrc rs, uint32_t off, double v)
 // that only another CU loaded comes back through L2 at about the cost of an HBM
 // row.  (Measured at 256^3 with 4 rows per lane: the halo rows cost ~13 us of an
 // ~84 us launch with one wave per workgroup, ~3 us with four stacked.)
-// Run f(integral_constant<int, I>) for I = 0, 1, ... in order (compile-time steps).
-template <typename F, int... I>
-__device__ __forceinline__ void unroll_steps(F&& f, std::integer_sequence<int, I...>)
-{
-  (f(std::integral_constant<int, I>{}), ...);
-}
 
 template <int FORM, int W, int NW, int RY, int ZC, bool XR, int PF = 2>
 __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
@@ -424,7 +301,7 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
     const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
 #pragma unroll
     for (int j = 0; j < RY; ++j)
-      fz[j] = face(load_at(kb, g.row[j + 1]), S[0].k[j + 1], load_at(pb, g.row[j + 1]), S[0].p[j + 1], c.iz0,
+      fz[j] = face_flux(load_at(kb, g.row[j + 1]), S[0].k[j + 1], load_at(pb, g.row[j + 1]), S[0].p[j + 1], c.iz0,
                    c.iz1, c.gz0, c.gz1);
   }
   // one step: plane z, whose slot is jj (a compile-time index)
@@ -434,12 +311,12 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
       const Col& C0 = S[jj];              // plane z
       const Col& C1 = S[(jj + 1) % NS];   // plane z+1
       // the y face between rows j-1 and j is computed once, as row j-1's high face
-      double fy_lo = face(C0.k[0], C0.k[1], C0.p[0], C0.p[1], c.iy0, c.iy1, c.gy0, c.gy1);
+      double fy_lo = face_flux(C0.k[0], C0.k[1], C0.p[0], C0.p[1], c.iy0, c.iy1, c.gy0, c.gy1);
       // XR: the wave's end faces of every row (lane j: left face of row j, lane 32+j: right face)
       double fend = 0.0;
       if constexpr (XR) {
         const bool left = lane < 32;
-        fend = face(left ? C0.ke : C0.ki, left ? C0.ki : C0.ke, left ? C0.pe : C0.pi, left ? C0.pi : C0.pe,
+        fend = face_flux(left ? C0.ke : C0.ki, left ? C0.ki : C0.ke, left ? C0.pe : C0.pi, left ? C0.pi : C0.pe,
                     c.ix0, c.ix1, c.gx0, c.gx1);
       }
 #pragma unroll
@@ -448,7 +325,7 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
         double fx_lo, fx_hi;
         if constexpr (XR) {
           const double pu = from_upper_lane(pc), ku = from_upper_lane(kc);
-          const double own = face(kc, ku, pc, pu, c.ix0, c.ix1, c.gx0, c.gx1);
+          const double own = face_flux(kc, ku, pc, pu, c.ix0, c.ix1, c.gx0, c.gx1);
           fx_hi = lane == kTx - 1 ? lane_value(fend, 32 + j) : own;
           const double below = from_lower_lane(fx_hi);
           fx_lo = lane == 0 ? lane_value(fend, j) : below;
@@ -459,11 +336,11 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
           const double kel = lane_value(C0.ke, j), ker = lane_value(C0.ke, 32 + j);
           const double pxm = lane == 0 ? pel : pl, pxp = lane == kTx - 1 ? per : pu;
           const double kxm = lane == 0 ? kel : kl, kxp = lane == kTx - 1 ? ker : ku;
-          fx_lo = face(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
-          fx_hi = face(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
+          fx_lo = face_flux(kxm, kc, pxm, pc, c.ix0, c.ix1, c.gx0, c.gx1);
+          fx_hi = face_flux(kc, kxp, pc, pxp, c.ix0, c.ix1, c.gx0, c.gx1);
         }
-        const double fy_hi = face(kc, C0.k[j + 2], pc, C0.p[j + 2], c.iy0, c.iy1, c.gy0, c.gy1);
-        const double fz_hi = face(kc, C1.k[j + 1], pc, C1.p[j + 1], c.iz0, c.iz1, c.gz0, c.gz1);
+        const double fy_hi = face_flux(kc, C0.k[j + 2], pc, C0.p[j + 2], c.iy0, c.iy1, c.gy0, c.gy1);
+        const double fz_hi = face_flux(kc, C1.k[j + 1], pc, C1.p[j + 1], c.iz0, c.iz1, c.gz0, c.gz1);
         const double Dx = fx_lo * c.dx0 + fx_hi * c.dx1;
         const double Dy = fy_lo * c.dy0 + fy_hi * c.dy1;
         const double Dz = fz[j] * c.dz0 + fz_hi * c.dz1;
@@ -804,17 +681,6 @@ static_assert(kSmallWide >= 0 && kSmallNarrow >= 0 && kLongChunks[0] >= 0 && kLo
                   kLongChunks[2] >= 0,
               "box-size tiles");
 
-constexpr int kMaxDevices = 64;
-std::atomic<int> g_cus[kMaxDevices];
-
-int device_cus(int device)
-{
-  if (device >= 0 && device < kMaxDevices && g_cus[device].load() > 0) return g_cus[device].load();
-  int n = 0;
-  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
-  if (device >= 0 && device < kMaxDevices) g_cus[device].store(n);
-  return n;
-}
 
 long tile_groups(int shape, const nebo_limits_t& lim)
 {
@@ -834,7 +700,7 @@ int select_shape(int kernel, int device, const nebo_limits_t& lim)
       if (t.kernel == kernel && t.device == device && t.n[0] == n[0] && t.n[1] == n[1] && t.n[2] == n[2])
         return t.shape;
   }
-  const int cus = device_cus(device);
+  const int cus = device_cu_count(device);
   for (int sh : kLongChunks)
     if (tile_groups(sh, lim) >= cus) return sh;
   if (2 * tile_groups(kDefaultShape, lim) >= cus) return kDefaultShape;
@@ -1004,6 +870,21 @@ void stencil_range(const nebo_field_t* f, nebo_limits_t* L, int* empty)
 }
 
 }  // namespace
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<int> g_cus[kMaxDevices];
+}  // namespace
+
+int device_cu_count(int device)
+{
+  if (device >= 0 && device < kMaxDevices && g_cus[device].load() > 0) return g_cus[device].load();
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+  if (device >= 0 && device < kMaxDevices) g_cus[device].store(n);
+  return n;
+}
+
 }  // namespace nebo
 
 using namespace nebo;

@@ -111,6 +111,145 @@ __global__ __launch_bounds__(kTx * kTy) void transport_stage_kernel(
   }
 }
 
+// Column schedule of the stage (the layout of divkgrad_column_kernel): W waves
+// side by side in x and NW stacked in y per workgroup, each lane one x column of
+// RY rows marching zc planes.  Per plane a lane holds s and k on its rows plus
+// one halo row on each side, u_x on its rows (the high x face is the next lane's
+// low one), u_y on its rows and the row above, u_z on its rows (plane z+1's
+// values are the high z faces) and phi_n; x neighbours come from the adjacent
+// lanes by DPP and, at the wave ends, from the edge loads (readlane), y
+// neighbours from the lane's own rows.  Loads run PF planes ahead in PF+1
+// rotated register slots.  Every cell is evaluated in transport_stage_kernel's
+// order, so the two are bit-identical.  Offsets are 32-bit buffer offsets from
+// each plane's base (launcher: plane sizes below 2^31 bytes).  A plane slot is
+// 32 doubles, so the kernel takes the register file of one wave per SIMD (two
+// with loads one plane ahead): one 4-wave workgroup per CU, no spills.
+template <int STAGE, int W, int NW, int RY, int PF>
+__global__ __launch_bounds__(kLanes * W * NW) __attribute__((amdgpu_waves_per_eu(1, PF >= 2 ? 1 : 2)))
+void transport_column_kernel(
+    double* __restrict__ out, const double* __restrict__ pn, const double* __restrict__ s,
+    const double* __restrict__ k, const double* __restrict__ ux, const double* __restrict__ uy,
+    const double* __restrict__ uz, Tiles T, int zc, int X, int XY, Coefs3 C, double dt)
+{
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = threadIdx.x / kLanes;
+  const int xw0 = T.lo[0] + tx * (kLanes * W) + (wave % W) * kLanes;
+  const int x = xw0 + lane;
+  const int y0 = T.lo[1] + ty * (RY * NW) + (wave / W) * RY;
+  const int z0 = T.lo[2] + tz * zc;
+  const int z1 = min(z0 + zc, T.hi[2]);
+  const bool xin = x < T.hi[0];
+  const ColumnGeom<RY> g = column_geom<RY>(T, X, xw0, y0, lane);
+  const AxisC cx = C.a[0], cy = C.a[1], cz = C.a[2];
+
+  struct Col {
+    double s[RY + 2], k[RY + 2], se, ke, ux[RY], uxe, uy[RY + 1], uz[RY], pn[RY];
+  };
+  auto load = [&](int z) __attribute__((always_inline)) {
+    Col v;
+    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z), X, XY);
+    const Rsrc xb = plane_rsrc(plane_base(ux, X, XY, z), X, XY);
+    const Rsrc yb = plane_rsrc(plane_base(uy, X, XY, z), X, XY);
+    const Rsrc zb = plane_rsrc(plane_base(uz, X, XY, z), X, XY);
+#pragma unroll
+    for (int q = 0; q < RY + 2; ++q) {
+      v.s[q] = load_at(sb, g.row[q]);
+      v.k[q] = load_at(kb, g.row[q]);
+    }
+    v.se = load_at(sb, g.edge);
+    v.ke = load_at(kb, g.edge);
+    v.uxe = load_at(xb, g.edge);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      v.ux[j] = load_at(xb, g.row[j + 1]);
+      v.uz[j] = load_at(zb, g.row[j + 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < RY + 1; ++j) v.uy[j] = load_at(yb, g.row[j + 1]);
+    if constexpr (STAGE != 1) {
+      const Rsrc nb = plane_rsrc(plane_base(pn, X, XY, z), X, XY);
+#pragma unroll
+      for (int j = 0; j < RY; ++j) v.pn[j] = load_at(nb, g.row[j + 1]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < RY; ++j) v.pn[j] = 0.0;   // stage 1 reads phi_n through s
+    }
+    return v;
+  };
+
+  constexpr int NS = PF + 1;   // register slots: planes z .. z+PF
+  Col S[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
+  // low z faces of plane z0
+  double fz[RY], vz[RY];
+  {
+    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      const double szm = load_at(sb, g.row[j + 1]), kzm = load_at(kb, g.row[j + 1]);
+      const double sc = S[0].s[j + 1], kc = S[0].k[j + 1];
+      fz[j] = face_flux(kzm, kc, szm, sc, cz.i0, cz.i1, cz.g0, cz.g1);
+      vz[j] = conv_face(S[0].uz[j], szm, sc);
+    }
+  }
+  // one step: plane z, whose slot is jj (a compile-time index)
+  auto step = [&](auto JJ, const int z) __attribute__((always_inline)) {
+    constexpr int jj = decltype(JJ)::value;
+    {
+      const Col& C0 = S[jj];              // plane z
+      const Col& C1 = S[(jj + 1) % NS];   // plane z+1
+      const Rsrc ob = plane_rsrc(plane_base(out, X, XY, z), X, XY);
+#pragma unroll
+      for (int j = 0; j < RY; ++j) {
+        const double sc = C0.s[j + 1], kc = C0.k[j + 1];
+        const double sl = from_lower_lane(sc), su = from_upper_lane(sc);
+        const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
+        const double uu = from_upper_lane(C0.ux[j]);
+        const double sxm = lane == 0 ? lane_value(C0.se, j) : sl;
+        const double sxp = lane == kLanes - 1 ? lane_value(C0.se, 32 + j) : su;
+        const double kxm = lane == 0 ? lane_value(C0.ke, j) : kl;
+        const double kxp = lane == kLanes - 1 ? lane_value(C0.ke, 32 + j) : ku;
+        const double uxh = lane == kLanes - 1 ? lane_value(C0.uxe, 32 + j) : uu;
+        const double sym = C0.s[j], syp = C0.s[j + 2], kym = C0.k[j], kyp = C0.k[j + 2];
+        const double szp = C1.s[j + 1], kzp = C1.k[j + 1];
+        const double fx = face_flux(kxm, kc, sxm, sc, cx.i0, cx.i1, cx.g0, cx.g1) * cx.d0 +
+                          face_flux(kc, kxp, sc, sxp, cx.i0, cx.i1, cx.g0, cx.g1) * cx.d1;
+        const double fy = face_flux(kym, kc, sym, sc, cy.i0, cy.i1, cy.g0, cy.g1) * cy.d0 +
+                          face_flux(kc, kyp, sc, syp, cy.i0, cy.i1, cy.g0, cy.g1) * cy.d1;
+        const double fz_hi = face_flux(kc, kzp, sc, szp, cz.i0, cz.i1, cz.g0, cz.g1);
+        const double fzz = fz[j] * cz.d0 + fz_hi * cz.d1;
+        const double vx = conv_face(C0.ux[j], sxm, sc) * cx.d0 + conv_face(uxh, sc, sxp) * cx.d1;
+        const double vy = conv_face(C0.uy[j], sym, sc) * cy.d0 + conv_face(C0.uy[j + 1], sc, syp) * cy.d1;
+        const double vz_hi = conv_face(C1.uz[j], sc, szp);
+        const double vzz = vz[j] * cz.d0 + vz_hi * cz.d1;
+        const double R = ((fx + fy) + fzz) - ((vx + vy) + vzz);
+        double v;
+        if constexpr (STAGE == 1) v = sc + dt * R;   // phi_n is s
+        else if constexpr (STAGE == 2) v = 0.75 * C0.pn[j] + 0.25 * (sc + dt * R);
+        else v = (1.0 / 3.0) * C0.pn[j] + (2.0 / 3.0) * (sc + dt * R);
+        fz[j] = fz_hi;
+        vz[j] = vz_hi;
+        if (xin && y0 + j < T.hi[1]) store_nt_at(ob, g.row[j + 1], v);
+      }
+      S[jj] = load(min(z + NS, z1));   // plane z is consumed: its slot takes z+NS
+    }
+  };
+  const int full = (z1 - z0) / NS, rem = (z1 - z0) - NS * full;
+  int z = z0;
+  for (int q = 0; q < full; ++q, z += NS)
+    unroll_steps([&](auto J) { step(J, z + decltype(J)::value); }, std::make_integer_sequence<int, NS>{});
+  unroll_steps([&](auto J) { if (rem > decltype(J)::value) step(J, z + decltype(J)::value); },
+               std::make_integer_sequence<int, NS - 1>{});
+}
+
 // Adjoint quantities of one face: kf*w, w*(G s), u*w and u itself.
 struct AdjFace {
   double kw, wg, uw, u;
@@ -189,26 +328,92 @@ struct Operands {
   const nebo_field_t* k;
 };
 
-std::atomic<int> g_zchunk{0};   // planes per workgroup; 0: kZChunk
+// Stage schedule: 0 chosen by box size; 1 direct (64 x 4 columns); 2 column
+// tiles with loads 2 planes ahead; 3 column tiles with loads 1 plane ahead.
+// g_zchunk: planes per workgroup, 0 for the schedule's default.
+std::atomic<int> g_schedule{0};
+std::atomic<int> g_zchunk{0};
 
+constexpr int kColW = 2, kColNW = 2, kColRY = 4;   // column tiles of 128 x 8 cells
+
+// The adjoint stage always runs the direct schedule; a z chunk set together
+// with schedule 1 applies to it too.
 Box3 interior_box(const nebo_field_t* f)
 {
   const int n[3] = {f->extent[0] - 2, f->extent[1] - 2, f->extent[2] - 2};
-  return make_box(n, kTx, kTy, g_zchunk.load() > 0 ? g_zchunk.load() : kZChunk);
+  const int zc = g_schedule.load() == 1 && g_zchunk.load() > 0 ? g_zchunk.load() : kZChunk;
+  return make_box(n, kTx, kTy, zc);
+}
+
+// Schedule and z chunk of a stage over an n[0] x n[1] x n[2] interior.  By box
+// size: column tiles (loads 1 plane ahead) with the longest chunk (512 .. 64
+// planes) that gives every CU a workgroup, as the stencil kernels do, else the
+// direct schedule.
+void stage_schedule(const nebo_field_t* f, int* sched, int* zc)
+{
+  const int n[3] = {f->extent[0] - 2, f->extent[1] - 2, f->extent[2] - 2};
+  *sched = g_schedule.load();
+  int auto_zc = 64;
+  if (*sched == 0) {
+    *sched = 1;
+    const long cols = (long)((n[0] + kLanes * kColW - 1) / (kLanes * kColW)) *
+                      ((n[1] + kColRY * kColNW - 1) / (kColRY * kColNW));
+    const int cus = device_cu_count(f->device);
+    for (int c : {512, 256, 128, 64})
+      if (cols * ((n[2] + c - 1) / c) >= cus) {
+        *sched = 3;   // loads 1 plane ahead: 1388 us per stage at 512^3 on MI355X, 2 planes 1413, direct 1797
+        auto_zc = c;
+        break;
+      }
+  }
+  // the column schedule's 32-bit plane offsets
+  const long X = f->glob[0], XY = (long)f->glob[0] * f->glob[1];
+  if (*sched >= 2 && (XY + 2 * X + 2) * 8 >= (1L << 31)) *sched = 1;
+  *zc = g_zchunk.load() > 0 ? g_zchunk.load() : *sched == 1 ? kZChunk : auto_zc;
+}
+
+using ColumnFn = void (*)(double*, const double*, const double*, const double*, const double*, const double*,
+                          const double*, Tiles, int, int, int, Coefs3, double);
+
+ColumnFn column_fn(int stage, int pf)
+{
+  static const ColumnFn t[2][3] = {
+      {transport_column_kernel<1, kColW, kColNW, kColRY, 2>, transport_column_kernel<2, kColW, kColNW, kColRY, 2>,
+       transport_column_kernel<3, kColW, kColNW, kColRY, 2>},
+      {transport_column_kernel<1, kColW, kColNW, kColRY, 1>, transport_column_kernel<2, kColW, kColNW, kColRY, 1>,
+       transport_column_kernel<3, kColW, kColNW, kColRY, 1>}};
+  return t[pf == 2 ? 0 : 1][stage - 1];
 }
 
 int launch_stage(const nebo_field_t* out, const nebo_field_t* pn, const nebo_field_t* s,
                  const Operands& o, const nebo_axis_coefs_t& c, double dt, int stage, void* stream)
 {
-  const Box3 B = interior_box(out);
-  const int nb = B.nx * B.ny * B.nz;
+  int sched = 1, zc = kZChunk;
+  stage_schedule(out, &sched, &zc);
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
-  auto fn = stage == 1 ? transport_stage_kernel<1> : stage == 2 ? transport_stage_kernel<2>
-                                                                : transport_stage_kernel<3>;
-  hipLaunchKernelGGL(fn, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream, frame_origin(out),
-                     frame_origin(pn), frame_origin(stage == 1 ? pn : s), frame_origin(o.k),
-                     frame_origin(&o.u[0]), frame_origin(&o.u[1]), frame_origin(&o.u[2]), B, X, XY,
-                     coefs3(c), dt);
+  const double* sp = frame_origin(stage == 1 ? pn : s);
+  if (sched == 1) {
+    const int n[3] = {out->extent[0] - 2, out->extent[1] - 2, out->extent[2] - 2};
+    const Box3 B = make_box(n, kTx, kTy, zc);
+    const int nb = B.nx * B.ny * B.nz;
+    auto fn = stage == 1 ? transport_stage_kernel<1> : stage == 2 ? transport_stage_kernel<2>
+                                                                  : transport_stage_kernel<3>;
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream, frame_origin(out),
+                       frame_origin(pn), sp, frame_origin(o.k), frame_origin(&o.u[0]), frame_origin(&o.u[1]),
+                       frame_origin(&o.u[2]), B, X, XY, coefs3(c), dt);
+  } else {
+    nebo_limits_t L;
+    for (int d = 0; d < 3; ++d) {
+      L.lo[d] = 0;
+      L.hi[d] = out->extent[d] - 2;
+    }
+    const Tiles T = make_tiles(L, kLanes * kColW, kColRY * kColNW, zc);
+    const long nb = (long)T.nx * T.ny * T.nz;
+    hipLaunchKernelGGL(column_fn(stage, sched == 2 ? 2 : 1), dim3((unsigned)nb), dim3(kLanes * kColW * kColNW), 0,
+                       (hipStream_t)stream, frame_origin(out), frame_origin(pn), sp, frame_origin(o.k),
+                       frame_origin(&o.u[0]), frame_origin(&o.u[1]), frame_origin(&o.u[2]), T, zc, X, XY,
+                       coefs3(c), dt);
+  }
   int rc = check_hip(hipGetLastError(), kCtx);
   if (rc) return rc;
   return fill_ghosts_periodic(out, stream);
@@ -267,12 +472,25 @@ int rk3_step(double* next, double* s1, double* s2, const double* phin, const neb
 
 using namespace nebo;
 
-extern "C" int nebo_transport_set_zchunk(int32_t planes)
+extern "C" int nebo_transport_tune(int32_t schedule, int32_t zchunk)
 {
   clear_error();
-  if (planes < 0 || planes > 4096)
-    return set_error(NEBO_ERR_INVALID, kCtx, "z chunk: 0 (the default) or 1 to 4096 planes");
-  g_zchunk.store(planes);
+  if (schedule < 0 || schedule > 3) return set_error(NEBO_ERR_INVALID, kCtx, "schedule: 0 to 3");
+  if (zchunk < 0 || zchunk > 4096)
+    return set_error(NEBO_ERR_INVALID, kCtx, "z chunk: 0 (the schedule's default) or 1 to 4096 planes");
+  g_schedule.store(schedule);
+  g_zchunk.store(zchunk);
+  return NEBO_OK;
+}
+
+extern "C" int nebo_transport_schedule(const nebo_field_t* field, int32_t* schedule, int32_t* zchunk)
+{
+  clear_error();
+  if (!field) return set_error(NEBO_ERR_INVALID, kCtx, "null argument");
+  int sched = 1, zc = kZChunk;
+  stage_schedule(field, &sched, &zc);
+  if (schedule) *schedule = sched;
+  if (zchunk) *zchunk = zc;
   return NEBO_OK;
 }
 

@@ -168,7 +168,8 @@ _SIGS = {
                                          C.c_int32, C.c_int32, _vp, _vp]),
     "nebo_diffusion_backward": (C.c_int, [P(FieldT), _vp, P(FieldT), P(FieldT), P(AxisCoefsT),
                                           C.c_double, C.c_int32, C.c_int32, _vp, _vp, _vp]),
-    "nebo_transport_set_zchunk": (C.c_int, [C.c_int32]),
+    "nebo_transport_tune": (C.c_int, [C.c_int32, C.c_int32]),
+    "nebo_transport_schedule": (C.c_int, [P(FieldT), P(C.c_int32), P(C.c_int32)]),
     "nebo_transport_stage": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT),
                                        C.c_double, C.c_int, _vp]),
     "nebo_transport_adjoint_stage": (C.c_int, [P(FieldT), _vp, _vp, P(FieldT), _vp, P(FieldT), P(FieldT),

@@ -30,9 +30,22 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
-def set_zchunk(planes=0):
-    """Planes each workgroup of the stage kernels marches (0: the default, 16)."""
-    _abi.call("nebo_transport_set_zchunk", int(planes))
+SCHED_AUTO, SCHED_DIRECT, SCHED_COLUMN, SCHED_COLUMN_PF1 = 0, 1, 2, 3
+
+
+def tune(schedule=SCHED_AUTO, zchunk=0):
+    """Schedule of the stage kernel (chosen by box size, direct, column tiles
+    with loads 2 or 1 planes ahead) and planes per workgroup (0: its default)."""
+    _abi.call("nebo_transport_tune", int(schedule), int(zchunk))
+
+
+def schedule(layout, device=0):
+    """(schedule, z chunk) a stage over a field of this layout on device would
+    use (the query reads only the descriptor's sizes)."""
+    d = layout.desc(None, device)
+    sc, zc = C.c_int32(), C.c_int32()
+    _abi.call("nebo_transport_schedule", C.byref(d), C.byref(sc), C.byref(zc))
+    return sc.value, zc.value
 
 
 def stage(layout, out, phin, s, k, us, coefs, dt, stage, face_layouts=None, stream=None):

@@ -28,12 +28,29 @@ def _periodic_random(L, rng):
     return inputs.periodic_fill(a, 1).reshape(-1)
 
 
-@pytest.fixture(params=[0, 5, 64], ids=["zchunk-auto", "zchunk-5", "zchunk-64"])
+@pytest.fixture(params=[(0, 0), (1, 5), (1, 64), (2, 5), (2, 64), (3, 7), (3, 16)],
+                ids=["auto", "direct-z5", "direct-z64", "column-z5", "column-z64", "column-pf1-z7",
+                     "column-pf1-z16"])
 def zchunk(request):
-    """The stage kernels with several z chunks (5 does not divide the boxes)."""
-    tr.set_zchunk(request.param)
+    """The stage kernels under each schedule and several z chunks (5 and 7 divide
+    none of the boxes; 64 exceeds them)."""
+    tr.tune(*request.param)
     yield request.param
-    tr.set_zchunk(0)
+    tr.tune()
+
+
+def test_schedule_by_box_size():
+    """Column tiles with the longest z chunk that fills every CU on large boxes,
+    the direct schedule on small ones; a tuned schedule applies as set."""
+    tr.tune()
+    assert tr.schedule(nh.Layout.build((512, 512, 512), 1)) == (tr.SCHED_COLUMN_PF1, 512)
+    assert tr.schedule(nh.Layout.build((256, 256, 256), 1)) == (tr.SCHED_COLUMN_PF1, 64)
+    assert tr.schedule(nh.Layout.build((64, 64, 64), 1)) == (tr.SCHED_DIRECT, 16)
+    try:
+        tr.tune(tr.SCHED_COLUMN, 9)
+        assert tr.schedule(nh.Layout.build((64, 64, 64), 1)) == (tr.SCHED_COLUMN, 9)
+    finally:
+        tr.tune()
 
 
 @pytest.mark.parametrize("n", [20, 37])
