@@ -323,6 +323,144 @@ __global__ __launch_bounds__(kTx * kTy) void transport_adjoint_kernel(
   }
 }
 
+// The adjoint stage on column tiles (the layout of transport_column_kernel): a,
+// s and k on the lane's rows plus one halo row each side and the wave-end edges,
+// the face velocities as in the stage kernel, acc_in and g_k on the lane's rows.
+// Every cell is evaluated in transport_adjoint_kernel's order (bit-identical);
+// g_k is read before its cell is written and read by no other cell.
+template <int STAGE, int W, int NW, int RY, int PF>
+__global__ __launch_bounds__(kLanes * W * NW) __attribute__((amdgpu_waves_per_eu(1, PF >= 2 ? 1 : 2)))
+void transport_adjoint_column_kernel(
+    double* __restrict__ out, double* __restrict__ acc_out, double* __restrict__ gk,
+    const double* __restrict__ a, const double* __restrict__ acc_in, const double* __restrict__ s,
+    const double* __restrict__ k, const double* __restrict__ ux, const double* __restrict__ uy,
+    const double* __restrict__ uz, Tiles T, int zc, int X, int XY, Coefs3 C, double dt, double cs, double cdt)
+{
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = threadIdx.x / kLanes;
+  const int xw0 = T.lo[0] + tx * (kLanes * W) + (wave % W) * kLanes;
+  const int x = xw0 + lane;
+  const int y0 = T.lo[1] + ty * (RY * NW) + (wave / W) * RY;
+  const int z0 = T.lo[2] + tz * zc;
+  const int z1 = min(z0 + zc, T.hi[2]);
+  const bool xin = x < T.hi[0];
+  const ColumnGeom<RY> g = column_geom<RY>(T, X, xw0, y0, lane);
+  const AxisC cx = C.a[0], cy = C.a[1], cz = C.a[2];
+
+  struct Col {
+    double a[RY + 2], s[RY + 2], k[RY + 2], ae, se, ke, ux[RY], uxe, uy[RY + 1], uz[RY], ain[RY], gk[RY];
+  };
+  auto load = [&](int z) __attribute__((always_inline)) {
+    Col v;
+    const Rsrc ab = plane_rsrc(plane_base(a, X, XY, z), X, XY);
+    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z), X, XY);
+    const Rsrc xb = plane_rsrc(plane_base(ux, X, XY, z), X, XY);
+    const Rsrc yb = plane_rsrc(plane_base(uy, X, XY, z), X, XY);
+    const Rsrc zb = plane_rsrc(plane_base(uz, X, XY, z), X, XY);
+    const Rsrc gb = plane_rsrc(plane_base(static_cast<const double*>(gk), X, XY, z), X, XY);
+#pragma unroll
+    for (int q = 0; q < RY + 2; ++q) {
+      v.a[q] = load_at(ab, g.row[q]);
+      v.s[q] = load_at(sb, g.row[q]);
+      v.k[q] = load_at(kb, g.row[q]);
+    }
+    v.ae = load_at(ab, g.edge);
+    v.se = load_at(sb, g.edge);
+    v.ke = load_at(kb, g.edge);
+    v.uxe = load_at(xb, g.edge);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      v.ux[j] = load_at(xb, g.row[j + 1]);
+      v.uz[j] = load_at(zb, g.row[j + 1]);
+      v.gk[j] = load_at(gb, g.row[j + 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < RY + 1; ++j) v.uy[j] = load_at(yb, g.row[j + 1]);
+    if constexpr (STAGE != 3) {
+      const Rsrc ib = plane_rsrc(plane_base(acc_in, X, XY, z), X, XY);
+#pragma unroll
+      for (int j = 0; j < RY; ++j) v.ain[j] = load_at(ib, g.row[j + 1]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < RY; ++j) v.ain[j] = 0.0;   // stage 3 reads no acc_in
+    }
+    return v;
+  };
+
+  constexpr int NS = PF + 1;
+  Col S[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
+  AdjFace zlo[RY];
+  {
+    const Rsrc ab = plane_rsrc(plane_base(a, X, XY, z0 - 1), X, XY);
+    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
+#pragma unroll
+    for (int j = 0; j < RY; ++j)
+      zlo[j] = adj_face(load_at(ab, g.row[j + 1]), S[0].a[j + 1], load_at(sb, g.row[j + 1]), S[0].s[j + 1],
+                        load_at(kb, g.row[j + 1]), S[0].k[j + 1], S[0].uz[j], cz);
+  }
+  auto step = [&](auto JJ, const int z) __attribute__((always_inline)) {
+    constexpr int jj = decltype(JJ)::value;
+    {
+      const Col& C0 = S[jj];
+      const Col& C1 = S[(jj + 1) % NS];
+      const Rsrc ob = plane_rsrc(plane_base(out, X, XY, z), X, XY);
+      const Rsrc gb = plane_rsrc(plane_base(gk, X, XY, z), X, XY);
+#pragma unroll
+      for (int j = 0; j < RY; ++j) {
+        const double ac = C0.a[j + 1], sc = C0.s[j + 1], kc = C0.k[j + 1];
+        const double al = from_lower_lane(ac), au = from_upper_lane(ac);
+        const double sl = from_lower_lane(sc), su = from_upper_lane(sc);
+        const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
+        const double uu = from_upper_lane(C0.ux[j]);
+        const bool first = lane == 0, last = lane == kLanes - 1;
+        const double axm = first ? lane_value(C0.ae, j) : al, axp = last ? lane_value(C0.ae, 32 + j) : au;
+        const double sxm = first ? lane_value(C0.se, j) : sl, sxp = last ? lane_value(C0.se, 32 + j) : su;
+        const double kxm = first ? lane_value(C0.ke, j) : kl, kxp = last ? lane_value(C0.ke, 32 + j) : ku;
+        const double uxh = last ? lane_value(C0.uxe, 32 + j) : uu;
+        const AdjFace xlo = adj_face(axm, ac, sxm, sc, kxm, kc, C0.ux[j], cx);
+        const AdjFace xhi = adj_face(ac, axp, sc, sxp, kc, kxp, uxh, cx);
+        const AdjFace ylo = adj_face(C0.a[j], ac, C0.s[j], sc, C0.k[j], kc, C0.uy[j], cy);
+        const AdjFace yhi = adj_face(ac, C0.a[j + 2], sc, C0.s[j + 2], kc, C0.k[j + 2], C0.uy[j + 1], cy);
+        const AdjFace zhi = adj_face(ac, C1.a[j + 1], sc, C1.s[j + 1], kc, C1.k[j + 1], C1.uz[j], cz);
+        const AdjCell ex = adj_cell(xlo, xhi, cx), ey = adj_cell(ylo, yhi, cy), ez = adj_cell(zlo[j], zhi, cz);
+        const double rt = ((ex.A + ey.A) + ez.A) - ((ex.V + ey.V) + ez.V);
+        const double Xv = ac + dt * rt;
+        const double gnew = C0.gk[j] + cdt * ((ex.B + ey.B) + ez.B);
+        zlo[j] = zhi;
+        if (xin && y0 + j < T.hi[1]) {
+          const uint32_t o = g.row[j + 1];
+          if constexpr (STAGE == 3) {
+            store_nt_at(ob, o, cs * Xv);
+            store_nt_at(plane_rsrc(plane_base(acc_out, X, XY, z), X, XY), o, (1.0 / 3.0) * ac);
+          } else if constexpr (STAGE == 2) {
+            store_nt_at(ob, o, cs * Xv);
+            store_nt_at(plane_rsrc(plane_base(acc_out, X, XY, z), X, XY), o, C0.ain[j] + 0.75 * ac);
+          } else {
+            store_nt_at(ob, o, C0.ain[j] + Xv);
+          }
+          store_nt_at(gb, o, gnew);
+        }
+      }
+      S[jj] = load(min(z + NS, z1));
+    }
+  };
+  const int full = (z1 - z0) / NS, rem = (z1 - z0) - NS * full;
+  int z = z0;
+  for (int q = 0; q < full; ++q, z += NS)
+    unroll_steps([&](auto J) { step(J, z + decltype(J)::value); }, std::make_integer_sequence<int, NS>{});
+  unroll_steps([&](auto J) { if (rem > decltype(J)::value) step(J, z + decltype(J)::value); },
+               std::make_integer_sequence<int, NS - 1>{});
+}
+
 struct Operands {
   const nebo_field_t* u;   // three face fields
   const nebo_field_t* k;
@@ -335,15 +473,6 @@ std::atomic<int> g_schedule{0};
 std::atomic<int> g_zchunk{0};
 
 constexpr int kColW = 2, kColNW = 2, kColRY = 4;   // column tiles of 128 x 8 cells
-
-// The adjoint stage always runs the direct schedule; a z chunk set together
-// with schedule 1 applies to it too.
-Box3 interior_box(const nebo_field_t* f)
-{
-  const int n[3] = {f->extent[0] - 2, f->extent[1] - 2, f->extent[2] - 2};
-  const int zc = g_schedule.load() == 1 && g_zchunk.load() > 0 ? g_zchunk.load() : kZChunk;
-  return make_box(n, kTx, kTy, zc);
-}
 
 // Schedule and z chunk of a stage over an n[0] x n[1] x n[2] interior.  By box
 // size: column tiles (loads 1 plane ahead) with the longest chunk (512 .. 64
@@ -421,24 +550,57 @@ int launch_stage(const nebo_field_t* out, const nebo_field_t* pn, const nebo_fie
 
 double stage_weight(int stage) { return stage == 3 ? 2.0 / 3.0 : stage == 2 ? 0.25 : 1.0; }
 
+using AdjColumnFn = void (*)(double*, double*, double*, const double*, const double*, const double*,
+                             const double*, const double*, const double*, const double*, Tiles, int, int, int,
+                             Coefs3, double, double, double);
+
+AdjColumnFn adjoint_column_fn(int stage, int pf)
+{
+  static const AdjColumnFn t[2][3] = {
+      {transport_adjoint_column_kernel<1, kColW, kColNW, kColRY, 2>,
+       transport_adjoint_column_kernel<2, kColW, kColNW, kColRY, 2>,
+       transport_adjoint_column_kernel<3, kColW, kColNW, kColRY, 2>},
+      {transport_adjoint_column_kernel<1, kColW, kColNW, kColRY, 1>,
+       transport_adjoint_column_kernel<2, kColW, kColNW, kColRY, 1>,
+       transport_adjoint_column_kernel<3, kColW, kColNW, kColRY, 1>}};
+  return t[pf == 2 ? 0 : 1][stage - 1];
+}
+
 int launch_adjoint(const nebo_field_t* out, double* acc_out, double* gk, const nebo_field_t* a,
                    const double* acc_in, const nebo_field_t* s, const Operands& o,
                    const nebo_axis_coefs_t& c, double dt, int stage, void* stream)
 {
-  const Box3 B = interior_box(out);
-  const int nb = B.nx * B.ny * B.nz;
+  int sched = 1, zc = kZChunk;
+  stage_schedule(out, &sched, &zc);
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
   // acc and gk are plain blocks with the field's shape: offset them like the field
   const std::ptrdiff_t off = frame_origin(out) - static_cast<double*>(out->values);
   const double cs = stage_weight(stage);
   const double cdt = cs * dt;
-  auto fn = stage == 1 ? transport_adjoint_kernel<1> : stage == 2 ? transport_adjoint_kernel<2>
-                                                                  : transport_adjoint_kernel<3>;
-  hipLaunchKernelGGL(fn, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream, frame_origin(out),
-                     acc_out ? acc_out + off : nullptr, gk + off, frame_origin(a),
-                     acc_in ? acc_in + off : nullptr, frame_origin(s), frame_origin(o.k),
-                     frame_origin(&o.u[0]), frame_origin(&o.u[1]), frame_origin(&o.u[2]), B, X, XY,
-                     coefs3(c), dt, cs, cdt);
+  double* acc_o = acc_out ? acc_out + off : nullptr;
+  const double* acc_i = acc_in ? acc_in + off : nullptr;
+  if (sched == 1) {
+    const int n[3] = {out->extent[0] - 2, out->extent[1] - 2, out->extent[2] - 2};
+    const Box3 B = make_box(n, kTx, kTy, zc);
+    const int nb = B.nx * B.ny * B.nz;
+    auto fn = stage == 1 ? transport_adjoint_kernel<1> : stage == 2 ? transport_adjoint_kernel<2>
+                                                                    : transport_adjoint_kernel<3>;
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(kTx * kTy), 0, (hipStream_t)stream, frame_origin(out), acc_o, gk + off,
+                       frame_origin(a), acc_i, frame_origin(s), frame_origin(o.k), frame_origin(&o.u[0]),
+                       frame_origin(&o.u[1]), frame_origin(&o.u[2]), B, X, XY, coefs3(c), dt, cs, cdt);
+  } else {
+    nebo_limits_t L;
+    for (int d = 0; d < 3; ++d) {
+      L.lo[d] = 0;
+      L.hi[d] = out->extent[d] - 2;
+    }
+    const Tiles T = make_tiles(L, kLanes * kColW, kColRY * kColNW, zc);
+    const long nb = (long)T.nx * T.ny * T.nz;
+    hipLaunchKernelGGL(adjoint_column_fn(stage, sched == 2 ? 2 : 1), dim3((unsigned)nb),
+                       dim3(kLanes * kColW * kColNW), 0, (hipStream_t)stream, frame_origin(out), acc_o, gk + off,
+                       frame_origin(a), acc_i, frame_origin(s), frame_origin(o.k), frame_origin(&o.u[0]),
+                       frame_origin(&o.u[1]), frame_origin(&o.u[2]), T, zc, X, XY, coefs3(c), dt, cs, cdt);
+  }
   int rc = check_hip(hipGetLastError(), kCtx);
   if (rc) return rc;
   return fill_ghosts_periodic(out, stream);

@@ -109,7 +109,7 @@ def test_adjoint_stage_bit_exact(n, zchunk):
             assert np.array_equal(got, racc.reshape(L.glob[::-1])[1:-1, 1:-1, 1:-1]), f"stage {stage} acc"
 
 
-def test_checkpointed_trajectory_bit_exact_vs_oracle():
+def test_checkpointed_trajectory_bit_exact_vs_oracle(zchunk):
     n, steps, every = 12, 6, 3
     L, Ls, ops, k, phi, u, dt, c = _problem(n, k_scale=50.0)
     rng = np.random.default_rng(9)
