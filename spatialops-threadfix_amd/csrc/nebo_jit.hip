@@ -485,6 +485,71 @@ int get_kernel(const std::string& src, const char* name, Entry* out)
   return NEBO_OK;
 }
 
+// Everything the source generators read from a program (opcodes with their
+// immediates, the class and bounds check of each read slot, the z chains, the
+// masks read) with the cells per thread, the launch shape and the kernel kind
+// (-1 assignment, else the reduction op), as a compact key: equal keys give
+// equal sources, so a repeated tree finds its kernel without the source text
+// being generated again (tens of microseconds of host time per call).
+std::string structural_key(const VmProgram& P, int V, const Shape& sh, int kind)
+{
+  const SlotUse u = slot_use(P);
+  std::string k;
+  k.reserve(64 + 4 * P.n_insns);
+  auto put = [&k](int v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
+  put(kind);
+  put(V);
+  put(sh.block);
+  put(sh.tile_x);
+  put(sh.xcd);
+  put(P.n_classes);
+  put(P.n_insns);
+  for (int pc = 0; pc < P.n_insns; ++pc) put((int)P.insns[pc].code << 16 | (uint16_t)P.insns[pc].imm);
+  for (int s = 0; s < kVmMaxLoads; ++s)
+    if (u.used[s]) {
+      put(s);
+      put(P.loads[s].cls);
+      put(u.guarded[s] ? 1 : 0);
+    }
+  put(-1);
+  for (int m = 0; m < kVmMaxMasks; ++m) put(u.mused[m] ? 1 : 0);
+  for (const auto& c : u.chains) {
+    put(-(int)c.size());
+    for (int s2 : c) put(s2);
+  }
+  return k;
+}
+
+std::unordered_map<std::string, Entry> g_by_key;   // key: device + structural key
+
+// Kernel for a program: by structural key, else generated (gen()), found in or
+// added to the source cache, and remembered under the key.
+template <class Gen>
+int get_kernel_keyed(const std::string& skey, Gen gen, const char* name, Entry* out)
+{
+  int device = 0;
+  int rc = check_hip(hipGetDevice(&device), "Nebo Fused Kernel");
+  if (rc) return rc;
+  const std::string key = std::to_string(device) + "|" + skey;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_by_key.find(key);
+    if (it != g_by_key.end()) {
+      *out = it->second;
+      ++g_hits;
+      return NEBO_OK;
+    }
+  }
+  Entry e;
+  if ((rc = get_kernel(gen(), name, &e))) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_by_key[key] = e;
+  }
+  *out = e;
+  return NEBO_OK;
+}
+
 // Cells per thread: the largest power of two up to the configured maximum
 // whose distinct reads per thread (z-chain reads shared) stay within the load
 // budget, so a thread's reads are all in flight without spilling.
@@ -548,7 +613,7 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
   const int V = choose_cells(P);
   const Shape sh = current_shape();
   Entry e;
-  int rc = get_kernel(jit_source(P, V, sh), "nebo_fused", &e);
+  int rc = get_kernel_keyed(structural_key(P, V, sh, -1), [&] { return jit_source(P, V, sh); }, "nebo_fused", &e);
   if (rc || !e.fn) return rc;
   JitArgs A;
   fill_args(P, lim, V, &A);
@@ -579,7 +644,8 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
   if (!jit_enabled()) return NEBO_OK;
   const int V = choose_cells(P);
   Entry e;
-  int rc = get_kernel(jit_reduce_source(P, V, op), "nebo_reduce", &e);
+  int rc = get_kernel_keyed(structural_key(P, V, Shape{0, 0, 0}, op), [&] { return jit_reduce_source(P, V, op); },
+                            "nebo_reduce", &e);
   if (rc || !e.fn) return rc;
   JitArgs A;
   fill_args(P, lim, V, &A);
