@@ -333,8 +333,8 @@ def particles_leg(n=128, count=4_000_000, reps=5):
     reference's particle operators) on a 128^3 cell field, 4M particles with
     seeded uniform positions and cube edges of 0.2-1.5 cells: each C2P gathers
     the volume-weighted average over the cells a particle overlaps, each P2C
-    spreads a particle quantity over them (count, scan, fill, radix sort of
-    (cell, entry) keys, ordered run sums: bit-identical to the serial loop)."""
+    spreads a particle quantity over them (count, scan, fill, stable radix sort
+    of (cell, value) pairs, ordered run sums: bit-identical to the serial loop)."""
     import torch
     import nebo_hip as nh
     from nebo_hip import particles as P

@@ -535,8 +535,8 @@ typedef struct nebo_particle_geom {
  * adds src[p] * (its volume fraction in the cell) to every cell of dest it
  * overlaps.  px / py / pz may be NULL for an axis the particles do not resolve.
  * Every cell receives its contributions in particle order, so the sums are those
- * of the reference's serial loop bit for bit: the contributions are sorted by
- * (cell, particle) and summed per cell.  A particle whose box begins a cell or
+ * of the reference's serial loop bit for bit: the contributions, written in the
+ * serial loop's order, are stably sorted by cell and summed per cell.  A particle whose box begins a cell or
  * more below the window on some axis contributes nothing (the reference's
  * unsigned index conversion wraps there); `skipped` (device int, may be NULL)
  * receives how many did.  All particle arrays hold np doubles on dest's GPU. */

@@ -7,11 +7,12 @@
 // loop bit for bit (ParticleOperators.h:184-270), so atomics are out.  Instead:
 //   1. count the cells each particle overlaps, exclusive scan -> entry offsets;
 //   2. each particle writes its entries in its own loop order: value
-//      src[p] * fraction at entry e, key (cell << E) | e;
-//   3. radix sort the keys: entries of one cell become contiguous and, since e
-//      grows with the particle index, stay in particle order;
-//   4. the first entry of each cell sums the cell's run onto the cell's value.
-// Keys are unique, so the result does not depend on the sort being stable.
+//      src[p] * fraction and key cell at entry e, so entries stand in the serial
+//      loop's order;
+//   3. stable radix sort of (cell, value) pairs on the cell's bits alone:
+//      entries of one cell become contiguous and keep their serial order;
+//   4. the first entry of each cell sums the cell's run onto the cell's value,
+//      reading the sorted values in sequence.
 //
 // Point fields (pointfield/PointOperators.h:40-160) are an index gather and an
 // index scatter; repeated scatter indices keep the last point, resolved on the
@@ -171,9 +172,11 @@ __global__ void count_kernel(long* __restrict__ counts, CellGrid g, Coords pc, c
   counts[p] = box_cells(b);
 }
 
-__global__ void fill_kernel(uint64_t* __restrict__ keys, double* __restrict__ vals,
+// Entry e (the serial loop's e-th contribution: particles in order, each one's
+// cells in k, j, i order) holds its cell and its value.
+__global__ void fill_kernel(uint32_t* __restrict__ keys, double* __restrict__ vals,
                             const long* __restrict__ offsets, const double* __restrict__ src, CellGrid g,
-                            Coords pc, const double* __restrict__ psize, long np, int ebits)
+                            Coords pc, const double* __restrict__ psize, long np)
 {
   const long p = (long)blockIdx.x * kBlock + threadIdx.x;
   if (p >= np) return;
@@ -183,22 +186,22 @@ __global__ void fill_kernel(uint64_t* __restrict__ keys, double* __restrict__ va
   const double s = src[p];
   for_cells(g, b, [&](long cell, double c) {
     vals[e] = s * c;
-    keys[e] = ((uint64_t)cell << ebits) | (uint64_t)e;
+    keys[e] = (uint32_t)cell;
     ++e;
   });
 }
 
-// The first entry of each cell's run adds the run, in order, onto the cell.
-__global__ void run_sum_kernel(double* __restrict__ dest, const uint64_t* __restrict__ keys,
-                               const double* __restrict__ vals, long total, int ebits)
+// After a stable sort by cell, each cell's entries are contiguous and in serial
+// order; the first entry of each run adds the run, in order, onto the cell.
+__global__ void run_sum_kernel(double* __restrict__ dest, const uint32_t* __restrict__ keys,
+                               const double* __restrict__ vals, long total)
 {
   const long s = (long)blockIdx.x * kBlock + threadIdx.x;
   if (s >= total) return;
-  const uint64_t cell = keys[s] >> ebits;
-  if (s > 0 && (keys[s - 1] >> ebits) == cell) return;
-  const uint64_t mask = (ebits >= 64) ? ~0ull : ((1ull << ebits) - 1);
+  const uint32_t cell = keys[s];
+  if (s > 0 && keys[s - 1] == cell) return;
   double acc = dest[cell];
-  for (long t = s; t < total && (keys[t] >> ebits) == cell; ++t) acc += vals[keys[t] & mask];
+  for (long t = s; t < total && keys[t] == cell; ++t) acc += vals[t];
   dest[cell] = acc;
 }
 
@@ -306,34 +309,34 @@ extern "C" int nebo_particle_to_cell(const nebo_field_t* dest, const double* src
     (void)hipFreeAsync(counts, s);
     return rc;
   }
+  // cells fit 32-bit keys (the field has fewer than 2^31 points, checked above)
   const long npts = (long)dest->glob[0] * dest->glob[1] * dest->glob[2];
-  const int ebits = bits_for((uint64_t)total), cbits = bits_for((uint64_t)npts);
-  if (ebits + cbits > 64) {
-    (void)hipFreeAsync(counts, s);
-    return set_error(NEBO_ERR_TOO_LARGE, ctx, "too many particle-cell overlaps for one call");
-  }
-  uint64_t* keys = nullptr;
+  const int cbits = bits_for((uint64_t)npts);
+  uint32_t* keys = nullptr;
   double* vals = nullptr;
-  rc = check_hip(hipMallocAsync((void**)&keys, sizeof(uint64_t) * 2 * total, s), ctx);
-  if (!rc) rc = check_hip(hipMallocAsync((void**)&vals, sizeof(double) * total, s), ctx);
+  rc = check_hip(hipMallocAsync((void**)&keys, sizeof(uint32_t) * 2 * total, s), ctx);
+  if (!rc) rc = check_hip(hipMallocAsync((void**)&vals, sizeof(double) * 2 * total, s), ctx);
   if (!rc) {
     hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(kBlock), 0, s, keys, vals, offsets, src, g, pc, psize,
-                       (long)np, ebits);
+                       (long)np);
     rc = check_hip(hipGetLastError(), ctx);
   }
-  uint64_t* sorted = keys ? keys + total : nullptr;
+  // stable LSD radix sort on the cell alone: within a cell the entries keep
+  // their serial order, so the run sums add in the serial loop's order
+  uint32_t* skeys = keys ? keys + total : nullptr;
+  double* svals = vals ? vals + total : nullptr;
   if (!rc)
-    rc = check_hip(rocprim::radix_sort_keys(nullptr, tmp_bytes, keys, sorted, (size_t)total, 0,
-                                            (unsigned)(ebits + cbits), s),
+    rc = check_hip(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, skeys, vals, svals, (size_t)total, 0,
+                                             (unsigned)cbits, s),
                    ctx);
   if (!rc) rc = check_hip(hipMallocAsync(&tmp, tmp_bytes, s), ctx);
   if (!rc)
-    rc = check_hip(rocprim::radix_sort_keys(tmp, tmp_bytes, keys, sorted, (size_t)total, 0,
-                                            (unsigned)(ebits + cbits), s),
+    rc = check_hip(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, skeys, vals, svals, (size_t)total, 0,
+                                             (unsigned)cbits, s),
                    ctx);
   if (!rc) {
     hipLaunchKernelGGL(run_sum_kernel, dim3((unsigned)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       static_cast<double*>(dest->values), sorted, vals, total, ebits);
+                       static_cast<double*>(dest->values), skeys, svals, total);
     rc = check_hip(hipGetLastError(), ctx);
   }
   if (tmp) (void)hipFreeAsync(tmp, s);
