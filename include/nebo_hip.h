@@ -236,6 +236,17 @@ int nebo_set_fused_jit(int enable);
  * renumbering on.  Kernels compiled for another shape stay cached. */
 int nebo_fused_jit_tune(int32_t block_threads, int32_t tile_x, int32_t max_cells,
                         int32_t load_budget, int32_t xcd_remap);
+/* Column schedule of the compiled assignment kernels.  It applies to a tree
+ * without masks whose unguarded reads of each field lie within a 3 x 3 x 3
+ * neighbourhood and that reads some y or z neighbour.  A wave covers 64
+ * consecutive x points; each lane holds 1, 2 or 4 rows (as many as fit 80
+ * register doubles), walks zchunk planes and loads each plane of each field
+ * once, one plane ahead of its use.  x neighbours come from the adjacent lanes.
+ * Other trees keep the shape of nebo_fused_jit_tune.  mode 1 uses it where it
+ * applies (the default), 0 never; waves per block in [1, 16], zchunk in
+ * [1, 4096]; 0 keeps the default of waves (1) and zchunk (8).  Results are the
+ * same bits either way. */
+int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk);
 int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);
 int nebo_fused_source(const nebo_expr_t* rhs, int32_t reduce_op, char* buf, int32_t size,
                       int32_t* needed);

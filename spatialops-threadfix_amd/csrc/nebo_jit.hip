@@ -313,6 +313,178 @@ void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int
   }
 }
 
+// Column schedule for trees whose unguarded reads are stencils: the reads of
+// one array (one index class) form a family whose bases differ by
+// dx + dy*X + dz*XY with each offset in [-2, 2] and a span of at most 2 per
+// axis.  A wave covers 64 consecutive x, each lane RY rows, and walks a z chunk;
+// a lane keeps, per family, its own column's values of the rows and planes the
+// next z step needs in registers (one new plane loaded per step), x neighbours
+// come from the adjacent lanes (DPP), and lanes 0 and 63 load the value from
+// past the wave's edge.  Every index is clamped to its slot's [glo, ghi) so that
+// lanes and rows past the box read inside the allocation; their results are
+// not stored.
+struct ColumnPlan {
+  int nf = 0, RY = 4;
+  int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
+  int fam[kVmMaxLoads], dx[kVmMaxLoads], dy[kVmMaxLoads], dz[kVmMaxLoads];
+};
+constexpr int kColumnRegs = 80;   // ring doubles per lane, over all families
+// Defaults from a sweep at 256^3 (div(k grad phi) and Laplacian trees): one wave
+// per block and 8 planes per block measured as fast as 2-4 waves or 16-32
+// planes, and give small boxes the most blocks.
+constexpr int kDefaultColumnWaves = 1, kDefaultColumnZc = 8;
+std::atomic<int> g_column{1}, g_column_waves{kDefaultColumnWaves}, g_column_zc{kDefaultColumnZc};
+
+long round_div(long a, long b) { return a >= 0 ? (a + b / 2) / b : -((-a + b / 2) / b); }
+
+bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
+{
+  for (int m = 0; m < kVmMaxMasks; ++m)
+    if (u.mused[m]) return false;   // mask bits past the box are not addressable
+  int off[kVmMaxLoads][3];
+  cp->nf = 0;
+  for (int s = 0; s < kVmMaxLoads; ++s) {
+    cp->fam[s] = -1;
+    if (!u.used[s] || u.guarded[s]) continue;
+    const int k = P.loads[s].cls;
+    const long X = P.cls_X[k], XY = P.cls_XY[k];
+    if (X < 1 || XY < X) return false;
+    for (int f = 0; f < cp->nf && cp->fam[s] < 0; ++f) {
+      const int r = cp->root[f];
+      // one family per allocation: the clamp range is the root's allocation
+      if (P.loads[r].cls != k || P.loads[r].base + P.loads[r].glo != P.loads[s].base + P.loads[s].glo) continue;
+      const long diff = (long)(((intptr_t)P.loads[s].base - (intptr_t)P.loads[r].base) / (intptr_t)sizeof(double));
+      const long dz = round_div(diff, XY), rem = diff - dz * XY, dy = round_div(rem, X), dx = rem - dy * X;
+      if (dx < -2 || dx > 2 || dy < -2 || dy > 2 || dz < -2 || dz > 2) continue;
+      cp->fam[s] = f;
+      off[s][0] = (int)dx, off[s][1] = (int)dy, off[s][2] = (int)dz;
+    }
+    if (cp->fam[s] < 0) {
+      cp->root[cp->nf] = s;
+      cp->fam[s] = cp->nf++;
+      off[s][0] = off[s][1] = off[s][2] = 0;
+    }
+  }
+  if (cp->nf == 0) return false;
+  bool stencil = false;
+  int regs1 = 0, regs0 = 0;   // ring doubles: per row of RY, and fixed
+  for (int f = 0; f < cp->nf; ++f) {
+    int lo[3] = {2, 2, 2}, hi[3] = {-2, -2, -2};
+    for (int s = 0; s < kVmMaxLoads; ++s)
+      if (cp->fam[s] == f)
+        for (int d = 0; d < 3; ++d) lo[d] = off[s][d] < lo[d] ? off[s][d] : lo[d], hi[d] = off[s][d] > hi[d] ? off[s][d] : hi[d];
+    for (int d = 0; d < 3; ++d)
+      if (hi[d] - lo[d] > 2) return false;
+    cp->cx[f] = lo[0] + (hi[0] - lo[0] == 2 ? 1 : 0);
+    cp->ylo[f] = lo[1], cp->yhi[f] = hi[1], cp->zlo[f] = lo[2], cp->zhi[f] = hi[2];
+    stencil = stencil || hi[1] > lo[1] || hi[2] > lo[2];
+    regs1 += hi[2] - lo[2] + 2;   // planes in use and the one loading
+    regs0 += (hi[1] - lo[1]) * (hi[2] - lo[2] + 2);
+    for (int s = 0; s < kVmMaxLoads; ++s)
+      if (cp->fam[s] == f) cp->dx[s] = off[s][0] - cp->cx[f], cp->dy[s] = off[s][1], cp->dz[s] = off[s][2];
+  }
+  if (!stencil) return false;
+  cp->RY = 0;
+  for (int ry = 4; ry >= 1 && !cp->RY; ry /= 2)
+    if (regs1 * ry + regs0 <= kColumnRegs) cp->RY = ry;
+  return cp->RY > 0;
+}
+
+// Launch shape of the column schedule: waves per block (stacked in y) and z
+// planes per block.
+struct ColumnShape {
+  int waves, zc, xcd;
+};
+
+std::string jit_column_source(const VmProgram& P, const SlotUse& u, const ColumnPlan& cp, const ColumnShape& cs)
+{
+  const int RY = cp.RY, NW = cs.waves;
+  SlotUse direct = u;   // guarded reads stay per cell; no z chains
+  direct.chains.clear();
+  std::ostringstream o;
+  o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
+  o << "extern \"C\" __global__ __launch_bounds__(" << 64 * NW << ") void nebo_fused(const JitArgs A)\n{\n";
+  if (cs.xcd) {
+    o << "  const int L = (int)blockIdx.x, d = L % " << kXcds << ";\n";
+    o << "  const int per = A.nblocks / " << kXcds << ", rem = A.nblocks % " << kXcds << ";\n";
+    o << "  const int b = d * per + (d < rem ? d : rem) + L / " << kXcds << ";\n";
+  } else {
+    o << "  const int b = (int)blockIdx.x;\n";
+  }
+  o << "  const int lane_ = (int)threadIdx.x & 63, w_ = (int)threadIdx.x >> 6;\n";
+  o << "  const int ncx = (A.n0 + 63) >> 6, ncy = (A.n1 + " << RY * NW - 1 << ") / " << RY * NW << ";\n";
+  o << "  const int bz = b / (ncx * ncy), bxy = b - bz * (ncx * ncy), by = bxy / ncx, bx = bxy - by * ncx;\n";
+  o << "  const int x = A.lo0 + bx * 64 + lane_;\n";
+  o << "  const int y0 = A.lo1 + (by * " << NW << " + w_) * " << RY << ";\n";
+  o << "  const int zs = A.lo2 + bz * " << cs.zc << ";\n";
+  o << "  const int zend = zs + " << cs.zc << " < A.lo2 + A.n2 ? zs + " << cs.zc << " : A.lo2 + A.n2;\n";
+  o << "  const bool xin_ = x < A.lo0 + A.n0;\n  const int ye_ = A.lo1 + A.n1;\n";
+  auto ring = [](int f, int j, int k) {
+    return "R" + std::to_string(f) + "_" + std::to_string(j) + "_" + std::to_string(k);
+  };
+  // ring load of family f, row j (0-based from ylo), plane slot k at plane z + p
+  auto load = [&](const std::string& ind, int f, int j, int k, const std::string& zexpr) {
+    const int r = cp.root[f], cls = P.loads[r].cls;
+    o << ind << "{ const int i_ = x + " << cp.cx[f] << " + A.clsX[" << cls << "] * (y0 + " << cp.ylo[f] + j
+      << ") + A.clsXY[" << cls << "] * (" << zexpr << "); " << ring(f, j, k) << " = A.base[" << r
+      << "][i_ < A.glo[" << r << "] ? A.glo[" << r << "] : i_ >= A.ghi[" << r << "] ? A.ghi[" << r << "] - 1 : i_]; }\n";
+  };
+  // slots 0 .. np-1 hold planes z + zlo .. z + zhi; slot np receives plane
+  // z + zhi + 1 while step z computes (past the chunk end it is clamped, unused)
+  for (int f = 0; f < cp.nf; ++f) {
+    const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
+    for (int j = 0; j < rows; ++j)
+      for (int k = 0; k <= np; ++k) o << "  double " << ring(f, j, k) << " = 0.0;\n";
+    for (int k = 0; k < np; ++k)
+      for (int j = 0; j < rows; ++j) load("  ", f, j, k, "zs + " + std::to_string(cp.zlo[f] + k));
+  }
+  o << "  for (int z = zs; z < zend; ++z) {\n";
+  // the wave-edge reads of this step first: loads complete in issue order, so
+  // waiting for them does not wait for the next plane
+  for (int r = 0; r < RY; ++r)
+    for (int s = 0; s < kVmMaxLoads; ++s) {
+      if (cp.fam[s] < 0 || cp.dx[s] == 0) continue;
+      const int cls = P.loads[s].cls;
+      const std::string E = "E" + std::to_string(r) + "_" + std::to_string(s);
+      o << "    double " << E << " = 0.0;\n";
+      o << "    if (lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << ") { const int i_ = x + A.clsX[" << cls << "] * (y0 + "
+        << r << ") + A.clsXY[" << cls << "] * z; " << E << " = A.base[" << s << "][i_ < A.glo[" << s << "] ? A.glo["
+        << s << "] : i_ >= A.ghi[" << s << "] ? A.ghi[" << s << "] - 1 : i_]; }\n";
+    }
+  for (int f = 0; f < cp.nf; ++f) {
+    const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
+    for (int j = 0; j < rows; ++j) load("    ", f, j, np, "z + " + std::to_string(cp.zhi[f] + 1));
+  }
+  for (int r = 0; r < RY; ++r) {
+    const std::string ind = "      ";
+    o << "    {\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = z;\n";
+    for (int s = 0; s < kVmMaxLoads; ++s) {
+      const int f = cp.fam[s];
+      if (f < 0) continue;
+      const std::string R = ring(f, r + cp.dy[s] - cp.ylo[f], cp.dz[s] - cp.zlo[f]);
+      if (cp.dx[s] == 0) {
+        o << ind << "const double L0_" << s << " = " << R << ";\n";
+        continue;
+      }
+      // the exchange is taken by every lane, outside any branch
+      o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << R << ", "
+        << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
+      o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? E" << r << "_" << s
+        << " : D0_" << s << ";\n";
+    }
+    const std::vector<std::string> res = emit_cells(o, P, direct, 1, true, ind);
+    o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res[0]
+      << ", &A.out[x + A.oX * y + A.oXY * z]);\n    }\n";
+  }
+  for (int f = 0; f < cp.nf; ++f) {
+    const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
+    for (int k = 0; k < np; ++k)
+      for (int j = 0; j < rows; ++j) o << "    " << ring(f, j, k) << " = " << ring(f, j, k + 1) << ";\n";
+  }
+  o << "  }\n}\n";
+  return o.str();
+}
+
 // Assignment kernel: one thread per (x, y) point of the box and V z cells.
 // Blocks form a 1-D grid of A.ntx plane tiles times A.nzc z chunks (A.nblocks
 // in all), plane tiles fastest.
@@ -520,6 +692,43 @@ std::string structural_key(const VmProgram& P, int V, const Shape& sh, int kind)
   return k;
 }
 
+// Whether an assignment runs on the column schedule (on by default for every
+// tree column_plan accepts) and its shape.
+bool column_choice(const VmProgram& P, ColumnPlan* cp, ColumnShape* cs)
+{
+  if (!g_column.load() || !column_plan(P, slot_use(P), cp)) return false;
+  cs->waves = g_column_waves.load();
+  cs->zc = g_column_zc.load();
+  cs->xcd = g_xcd.load();
+  return true;
+}
+
+std::string column_key(const VmProgram& P, const ColumnPlan& cp, const ColumnShape& cs)
+{
+  std::string k = structural_key(P, 1, Shape{64 * cs.waves, 0, cs.xcd}, -2);
+  auto put = [&k](int v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
+  put(cp.RY);
+  put(cs.zc);
+  put(cp.nf);
+  for (int f = 0; f < cp.nf; ++f) {
+    put(cp.root[f]);
+    put(cp.cx[f]);
+    put(cp.ylo[f]);
+    put(cp.yhi[f]);
+    put(cp.zlo[f]);
+    put(cp.zhi[f]);
+  }
+  for (int s = 0; s < kVmMaxLoads; ++s)
+    if (cp.fam[s] >= 0) {
+      put(s);
+      put(cp.fam[s]);
+      put(cp.dx[s]);
+      put(cp.dy[s]);
+      put(cp.dz[s]);
+    }
+  return k;
+}
+
 std::unordered_map<std::string, Entry> g_by_key;   // key: device + structural key
 
 // Kernel for a program: by structural key, else generated (gen()), found in or
@@ -613,6 +822,26 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
   const int V = choose_cells(P);
   const Shape sh = current_shape();
   Entry e;
+  ColumnPlan cp;
+  ColumnShape cs;
+  if (column_choice(P, &cp, &cs)) {
+    int rc = get_kernel_keyed(column_key(P, cp, cs), [&] { return jit_column_source(P, slot_use(P), cp, cs); },
+                              "nebo_fused", &e);
+    if (rc || !e.fn) return rc;
+    JitArgs A;
+    fill_args(P, lim, 1, &A);
+    A.out = frame_origin(lhs);
+    A.oX = lhs->glob[0];
+    A.oXY = lhs->glob[0] * lhs->glob[1];
+    const long total = (((long)A.n0 + 63) / 64) * (((long)A.n1 + cp.RY * cs.waves - 1) / (cp.RY * cs.waves)) *
+                       (((long)A.n2 + cs.zc - 1) / cs.zc);
+    if (total >= (1L << 31))
+      return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
+    A.nblocks = (int)total;
+    if ((rc = launch(e, &A, (unsigned)total, 1, (unsigned)(64 * cs.waves), stream))) return rc;
+    *done = true;
+    return NEBO_OK;
+  }
   int rc = get_kernel_keyed(structural_key(P, V, sh, -1), [&] { return jit_source(P, V, sh); }, "nebo_fused", &e);
   if (rc || !e.fn) return rc;
   JitArgs A;
@@ -664,6 +893,9 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
 std::string jit_source_for(const VmProgram& P, int reduce_op)
 {
   const int V = choose_cells(P);
+  ColumnPlan cp;
+  ColumnShape cs;
+  if (reduce_op < 0 && column_choice(P, &cp, &cs)) return jit_column_source(P, slot_use(P), cp, cs);
   return reduce_op < 0 ? jit_source(P, V, current_shape()) : jit_reduce_source(P, V, reduce_op);
 }
 
@@ -673,6 +905,21 @@ extern "C" int nebo_set_fused_jit(int enable)
 {
   nebo::clear_error();
   nebo::g_mode.store(enable ? 1 : 0);
+  return NEBO_OK;
+}
+
+extern "C" int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk)
+{
+  nebo::clear_error();
+  static const char* ctx = "Nebo Fused Kernel";
+  const int w = waves ? waves : nebo::kDefaultColumnWaves;
+  const int z = zchunk ? zchunk : nebo::kDefaultColumnZc;
+  if (mode != 0 && mode != 1) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column mode must be 0 or 1");
+  if (w < 1 || w > 16) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column waves must be in [1, 16]");
+  if (z < 1 || z > 4096) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column z chunk must be in [1, 4096]");
+  nebo::g_column.store(mode);
+  nebo::g_column_waves.store(w);
+  nebo::g_column_zc.store(z);
   return NEBO_OK;
 }
 

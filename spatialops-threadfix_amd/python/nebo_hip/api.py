@@ -120,6 +120,15 @@ def fused_jit_stats():
     return tuple(x.value for x in v)
 
 
+def fused_jit_column(enable=True, waves=0, zchunk=0):
+    """Column schedule of the compiled assignment kernels (on by default): for
+    trees without masks whose reads of each field stay within one point in x, y
+    and z, a wave covers 64 x points, each lane 1-4 rows and `zchunk` planes, and
+    each plane of each field is loaded once.  0 keeps the defaults (1 wave per
+    block, 8 planes)."""
+    _abi.call("nebo_fused_jit_column", int(bool(enable)), int(waves), int(zchunk))
+
+
 def fused_jit_tune(block_threads=0, tile_x=-1, max_cells=0, load_budget=0, xcd_remap=-1):
     """Launch shape of the compiled assignment kernels; the defaults restore the
     library's own (512 threads over the flattened plane, up to 4 z cells per

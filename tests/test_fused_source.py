@@ -168,6 +168,7 @@ def test_z_chain_reads_are_shared():
     kernel issues 2 (3 + V - 1) + 8 V reads instead of 14 V; the tail path (a
     chunk cut by the box end) reads each cell on its own."""
     t = _divkgrad_tree()
+    nh.fused_jit_column(False)
     try:
         for (block, cells, budget), V in (((0, 0, 0), 4), ((256, 4, 32), 2), ((128, 8, 96), 8),
                                           ((64, 1, 0), 1)):
@@ -181,6 +182,52 @@ def test_z_chain_reads_are_shared():
                 assert tail.count("if (") >= V - 1 and tail.count("A.base[") == 14 * (V - 1)
     finally:
         nh.fused_jit_tune()
+        nh.fused_jit_column()
+
+
+def test_column_schedule_source():
+    """div(k grad phi) runs on the column schedule by default: k and phi are one
+    family each (reads within one point in x, y and z), each lane holds 4 rows,
+    so a family keeps rows -1 .. 4 of 3 planes plus the plane loading: 2 x 6 x 3
+    loads before the z loop and 2 x 6 per step.  The four x-neighbour reads per
+    cell come from the adjacent lanes (4 x 4 lane exchanges per step), with one
+    predicated load each for the lane at the wave's edge.  Trees with masks, and
+    pointwise trees, keep the flattened-plane schedule, as does every tree with
+    the column schedule switched off."""
+    t = _divkgrad_tree()
+    src = nh.fused_source(t)
+    assert "__launch_bounds__(64)" in src and "for (int z = zs; z < zend; ++z)" in src
+    loop = src.split("for (int z = zs; z < zend; ++z)")
+    pro, body = loop[0], loop[1]
+    ring = lambda text: len(re.findall(r"R\d+_\d+_\d+ = A\.base\[", text))
+    assert ring(pro) == 2 * 6 * 3 and ring(body) == 2 * 6
+    assert body.count("__builtin_amdgcn_mov_dpp") == 16
+    assert len(re.findall(r"E\d+_\d+ = A\.base\[", body)) == 16
+    # every read is clamped to its slot's allocation (lanes and rows past the box)
+    assert src.count("A.base[") == src.count("i_ < A.glo[") == 48 + 16
+    n = (6, 5, 4)
+    L = nh.Layout.build(n, 1)
+    f, g = host(L), host(L)
+    m = nh.Mask(L, [(0, 0, 0)])
+    db = nh.build_stencils(*n, 1, 1, 1)
+    for other in (f * g + 1.0, nh.cond(m, db[("Gradient", "SVol", "SSurfX")](f))(g)):
+        assert "for (int z = zs; z < zend; ++z)" not in nh.fused_source(other)
+    try:
+        nh.fused_jit_column(True, 4, 16)
+        src = nh.fused_source(t)
+        assert "__launch_bounds__(256)" in src and "bz * 16" in src
+        nh.fused_jit_column(False)
+        assert "for (int z = zs; z < zend; ++z)" not in nh.fused_source(t)
+    finally:
+        nh.fused_jit_column()
+
+
+def test_column_bounds():
+    for bad in ((2, 0, 0), (1, 17, 0), (1, -1, 0), (1, 0, 5000), (1, 0, -3)):
+        with pytest.raises(nh.NeboError) as ei:
+            nh._abi.call("nebo_fused_jit_column", *bad)
+        assert ei.value.code == _abi.NEBO_ERR_INVALID
+    nh.fused_jit_column()
 
 
 def test_tune_bounds():
@@ -192,11 +239,13 @@ def test_tune_bounds():
     nh.fused_jit_tune()
 
 
-@pytest.mark.parametrize("shape", [(1024, 0, 8, 64, 0), (64, -1, 16, 256, 1), (1024, 64, 4, 48, 1),
-                                   (512, 128, 2, 0, 0)])
-def test_other_shapes_compile(rtc, shape):
+@pytest.mark.parametrize("shape,column", [((1024, 0, 8, 64, 0), (0, 0, 0)), ((64, -1, 16, 256, 1), (0, 0, 0)),
+                                          ((1024, 64, 4, 48, 1), (0, 0, 0)), ((512, 128, 2, 0, 0), (0, 0, 0)),
+                                          ((0, -1, 0, 0, 0), (1, 4, 1)), ((0, -1, 0, 0, 1), (1, 16, 4096))])
+def test_other_shapes_compile(rtc, shape, column):
     try:
         nh.fused_jit_tune(*shape)
+        nh.fused_jit_column(*column)
         n = (6, 5, 4)
         L = nh.Layout.build(n, 1)
         f, g = host(L), host(L)
@@ -207,3 +256,4 @@ def test_other_shapes_compile(rtc, shape):
             check_no_hidden_args(rtc, nh.fused_source(t, reduce_op=_abi.RED_SUM))
     finally:
         nh.fused_jit_tune()
+        nh.fused_jit_column()

@@ -155,7 +155,10 @@ def test_mask_known_answer_on_gpu():
 
 
 def test_heat_equation_mask_step():
-    """examples/heat_equation.cpp pattern on a 2-D (nz = 1) domain with a +BC."""
+    """examples/heat_equation.cpp pattern on a 2-D (nz = 1) domain with a +BC.
+    Its fields are one plane each and may be allocated back to back, so a read
+    of one field can lie one plane from a read of another: the column schedule
+    must still keep each field's reads inside that field's allocation."""
     n = (6, 6, 1)
     L = nh.Layout.build(n, 1, "SVol", (True, True, True))
     left = [(-1, i, 0) for i in range(-1, n[1] + 1)]
@@ -235,15 +238,24 @@ def test_random_trees_bit_exact(seed, fused_mode):
             assert np.array_equal(got, ref.host)
 
 
-@pytest.fixture(params=[(0, -1, 0, 0, -1), (1024, 0, 8, 64, 1), (64, -1, 16, 256, 0), (512, 64, 4, 48, 1)],
-                ids=["default", "b1024-v8-xcd", "b64-v16", "tile64x8-xcd"])
+@pytest.fixture(params=[((0, -1, 0, 0, -1), (1, 0, 0)), ((0, -1, 0, 0, -1), (0, 0, 0)),
+                        ((1024, 0, 8, 64, 1), (0, 0, 0)), ((64, -1, 16, 256, 0), (0, 0, 0)),
+                        ((512, 64, 4, 48, 1), (0, 0, 0)), ((0, -1, 0, 0, -1), (1, 2, 7)),
+                        ((0, -1, 0, 0, 0), (1, 4, 1))],
+                ids=["default", "flat", "b1024-v8-xcd", "b64-v16", "tile64x8-xcd", "column-w2-z7",
+                     "column-w4-z1"])
 def jit_shape(request):
-    """Compiled kernels of several launch shapes: flattened-plane and 2-D tiles,
-    with and without the XCD renumbering, cells per thread from 1 to 16 (full
-    chunks with shared z-chain reads, and tails cut by the box end)."""
-    nh.fused_jit_tune(*request.param)
+    """Compiled kernels of several launch shapes: the column schedule (rows and
+    planes past the box end inside a block, 1-4 waves, 1-8 planes per block) and,
+    with it off, flattened-plane and 2-D tiles, with and without the XCD
+    renumbering, cells per thread from 1 to 16 (full chunks with shared z-chain
+    reads, and tails cut by the box end)."""
+    tune, column = request.param
+    nh.fused_jit_tune(*tune)
+    nh.fused_jit_column(*column)
     yield request.param
     nh.fused_jit_tune()
+    nh.fused_jit_column()
 
 
 def test_compiled_kernels_equal_interpreter(jit_shape):
