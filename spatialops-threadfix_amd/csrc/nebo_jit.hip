@@ -330,10 +330,10 @@ struct ColumnPlan {
 };
 constexpr int kColumnRegs = 80;   // ring doubles per lane, over all families
 // Defaults from a sweep at 256^3 (div(k grad phi) and Laplacian trees): one wave
-// per block and 8 planes per block measured as fast as 2-4 waves or 16-32
-// planes, and give small boxes the most blocks.
+// per block measured as fast as 2-4 waves and gives small boxes the most
+// blocks; g_column_zc 0 picks the planes per block from the box size.
 constexpr int kDefaultColumnWaves = 1, kDefaultColumnZc = 8;
-std::atomic<int> g_column{1}, g_column_waves{kDefaultColumnWaves}, g_column_zc{kDefaultColumnZc};
+std::atomic<int> g_column{1}, g_column_waves{kDefaultColumnWaves}, g_column_zc{0};
 
 long round_div(long a, long b) { return a >= 0 ? (a + b / 2) / b : -((-a + b / 2) / b); }
 
@@ -438,44 +438,71 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     for (int k = 0; k < np; ++k)
       for (int j = 0; j < rows; ++j) load("  ", f, j, k, "zs + " + std::to_string(cp.zlo[f] + k));
   }
-  o << "  for (int z = zs; z < zend; ++z) {\n";
-  // the wave-edge reads of this step first: loads complete in issue order, so
-  // waiting for them does not wait for the next plane
-  for (int r = 0; r < RY; ++r)
-    for (int s = 0; s < kVmMaxLoads; ++s) {
-      if (cp.fam[s] < 0 || cp.dx[s] == 0) continue;
-      const int cls = P.loads[s].cls;
-      const std::string E = "E" + std::to_string(r) + "_" + std::to_string(s);
-      o << "    double " << E << " = 0.0;\n";
-      o << "    if (lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << ") { const int i_ = x + A.clsX[" << cls << "] * (y0 + "
-        << r << ") + A.clsXY[" << cls << "] * z; " << E << " = A.base[" << s << "][i_ < A.glo[" << s << "] ? A.glo["
-        << s << "] : i_ >= A.ghi[" << s << "] ? A.ghi[" << s << "] - 1 : i_]; }\n";
-    }
-  for (int f = 0; f < cp.nf; ++f) {
-    const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
-    for (int j = 0; j < rows; ++j) load("    ", f, j, np, "z + " + std::to_string(cp.zhi[f] + 1));
-  }
-  for (int r = 0; r < RY; ++r) {
-    const std::string ind = "      ";
-    o << "    {\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = z;\n";
-    for (int s = 0; s < kVmMaxLoads; ++s) {
-      const int f = cp.fam[s];
-      if (f < 0) continue;
-      const std::string R = ring(f, r + cp.dy[s] - cp.ylo[f], cp.dz[s] - cp.zlo[f]);
-      if (cp.dx[s] == 0) {
-        o << ind << "const double L0_" << s << " = " << R << ";\n";
-        continue;
+  // One z step at plane zq: the wave-edge reads, the next plane's loads, then
+  // RY cells.  Ring slot k of family f (plane z + zlo + k; k = np is the plane
+  // loading) lives in register (k + u) mod (np + 1), so the unrolled steps
+  // u = 0 .. U-1 rename registers instead of copying them; the remainder loop
+  // (u = 0) copies.
+  auto step = [&](const std::string& in, int u) {
+    auto slot = [&](int f, int k) { return (k + u) % (cp.zhi[f] - cp.zlo[f] + 2); };
+    // edge reads first: loads complete in issue order, so waiting for them does
+    // not wait for the next plane
+    for (int r = 0; r < RY; ++r)
+      for (int s = 0; s < kVmMaxLoads; ++s) {
+        if (cp.fam[s] < 0 || cp.dx[s] == 0) continue;
+        const int cls = P.loads[s].cls;
+        const std::string E = "E" + std::to_string(r) + "_" + std::to_string(s);
+        o << in << "double " << E << " = 0.0;\n";
+        o << in << "if (lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << ") { const int i_ = x + A.clsX[" << cls
+          << "] * (y0 + " << r << ") + A.clsXY[" << cls << "] * zq; " << E << " = A.base[" << s << "][i_ < A.glo["
+          << s << "] ? A.glo[" << s << "] : i_ >= A.ghi[" << s << "] ? A.ghi[" << s << "] - 1 : i_]; }\n";
       }
-      // the exchange is taken by every lane, outside any branch
-      o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << R << ", "
-        << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
-      o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? E" << r << "_" << s
-        << " : D0_" << s << ";\n";
+    for (int f = 0; f < cp.nf; ++f) {
+      const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
+      for (int j = 0; j < rows; ++j) load(in, f, j, slot(f, np), "zq + " + std::to_string(cp.zhi[f] + 1));
     }
-    const std::vector<std::string> res = emit_cells(o, P, direct, 1, true, ind);
-    o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res[0]
-      << ", &A.out[x + A.oX * y + A.oXY * z]);\n    }\n";
+    for (int r = 0; r < RY; ++r) {
+      const std::string ind = in + "  ";
+      o << in << "{\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = zq;\n";
+      for (int s = 0; s < kVmMaxLoads; ++s) {
+        const int f = cp.fam[s];
+        if (f < 0) continue;
+        const std::string R = ring(f, r + cp.dy[s] - cp.ylo[f], slot(f, cp.dz[s] - cp.zlo[f]));
+        if (cp.dx[s] == 0) {
+          o << ind << "const double L0_" << s << " = " << R << ";\n";
+          continue;
+        }
+        // the exchange is taken by every lane, outside any branch
+        o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << R << ", "
+          << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
+        o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? E" << r << "_" << s
+          << " : D0_" << s << ";\n";
+      }
+      const std::vector<std::string> res = emit_cells(o, P, direct, 1, true, ind);
+      o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res[0]
+        << ", &A.out[x + A.oX * y + A.oXY * zq]);\n" << in << "}\n";
+    }
+  };
+  int U = 1;
+  for (int f = 0; f < cp.nf; ++f) {
+    const int n = cp.zhi[f] - cp.zlo[f] + 2;
+    int a = U, b = n;
+    while (b) { const int t = a % b; a = b; b = t; }
+    U = U / a * n;
   }
+  if (U > 6) U = 1;   // families of different depths: copies rather than a long unroll
+  o << "  int z = zs;   // column schedule\n";
+  if (U > 1) {
+    o << "  for (; z + " << U - 1 << " < zend; z += " << U << ") {\n";
+    for (int u = 0; u < U; ++u) {
+      o << "    {\n      const int zq = z + " << u << ";\n";
+      step("      ", u);
+      o << "    }\n";
+    }
+    o << "  }\n";
+  }
+  o << "  for (; z < zend; ++z) {\n    const int zq = z;\n";
+  step("    ", 0);
   for (int f = 0; f < cp.nf; ++f) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
     for (int k = 0; k < np; ++k)
@@ -693,13 +720,27 @@ std::string structural_key(const VmProgram& P, int V, const Shape& sh, int kind)
 }
 
 // Whether an assignment runs on the column schedule (on by default for every
-// tree column_plan accepts) and its shape.
-bool column_choice(const VmProgram& P, ColumnPlan* cp, ColumnShape* cs)
+// tree column_plan accepts) and its shape.  Unless set, the planes per block
+// are the most of 32, 16, 8 that still give 8 waves per compute unit (longer
+// chunks load fewer halo planes; a sweep at 256^3 measured 32 fastest), else 8.
+bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp, ColumnShape* cs)
 {
   if (!g_column.load() || !column_plan(P, slot_use(P), cp)) return false;
   cs->waves = g_column_waves.load();
   cs->zc = g_column_zc.load();
   cs->xcd = g_xcd.load();
+  if (cs->zc == 0) {
+    cs->zc = kDefaultColumnZc;
+    int device = 0;
+    if (lim && hipGetDevice(&device) == hipSuccess) {
+      const long cols = (long)((lim->hi[0] - lim->lo[0] + 63) / 64) *
+                        ((lim->hi[1] - lim->lo[1] + cp->RY * cs->waves - 1) / (cp->RY * cs->waves)) * cs->waves;
+      const long want = 8L * device_cu_count(device);
+      const int nz = lim->hi[2] - lim->lo[2];
+      for (int zc : {32, 16})
+        if (cs->zc == kDefaultColumnZc && cols * ((nz + zc - 1) / zc) >= want) cs->zc = zc;
+    }
+  }
   return true;
 }
 
@@ -824,7 +865,7 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
   Entry e;
   ColumnPlan cp;
   ColumnShape cs;
-  if (column_choice(P, &cp, &cs)) {
+  if (column_choice(P, &lim, &cp, &cs)) {
     int rc = get_kernel_keyed(column_key(P, cp, cs), [&] { return jit_column_source(P, slot_use(P), cp, cs); },
                               "nebo_fused", &e);
     if (rc || !e.fn) return rc;
@@ -895,7 +936,7 @@ std::string jit_source_for(const VmProgram& P, int reduce_op)
   const int V = choose_cells(P);
   ColumnPlan cp;
   ColumnShape cs;
-  if (reduce_op < 0 && column_choice(P, &cp, &cs)) return jit_column_source(P, slot_use(P), cp, cs);
+  if (reduce_op < 0 && column_choice(P, nullptr, &cp, &cs)) return jit_column_source(P, slot_use(P), cp, cs);
   return reduce_op < 0 ? jit_source(P, V, current_shape()) : jit_reduce_source(P, V, reduce_op);
 }
 
@@ -913,10 +954,10 @@ extern "C" int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk
   nebo::clear_error();
   static const char* ctx = "Nebo Fused Kernel";
   const int w = waves ? waves : nebo::kDefaultColumnWaves;
-  const int z = zchunk ? zchunk : nebo::kDefaultColumnZc;
+  const int z = zchunk;   // 0: from the box size
   if (mode != 0 && mode != 1) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column mode must be 0 or 1");
   if (w < 1 || w > 16) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column waves must be in [1, 16]");
-  if (z < 1 || z > 4096) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column z chunk must be in [1, 4096]");
+  if (z < 0 || z > 4096) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column z chunk must be in [1, 4096] or 0");
   nebo::g_column.store(mode);
   nebo::g_column_waves.store(w);
   nebo::g_column_zc.store(z);

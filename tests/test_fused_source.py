@@ -191,33 +191,39 @@ def test_column_schedule_source():
     so a family keeps rows -1 .. 4 of 3 planes plus the plane loading: 2 x 6 x 3
     loads before the z loop and 2 x 6 per step.  The four x-neighbour reads per
     cell come from the adjacent lanes (4 x 4 lane exchanges per step), with one
-    predicated load each for the lane at the wave's edge.  Trees with masks, and
-    pointwise trees, keep the flattened-plane schedule, as does every tree with
-    the column schedule switched off."""
+    predicated load each for the lane at the wave's edge.  The ring of 4 planes
+    makes the main loop 4 steps long (registers renamed, not copied) and a
+    remainder loop 1 step.  Trees with masks, and pointwise trees, keep the
+    flattened-plane schedule, as does every tree with the column schedule
+    switched off."""
     t = _divkgrad_tree()
     src = nh.fused_source(t)
-    assert "__launch_bounds__(64)" in src and "for (int z = zs; z < zend; ++z)" in src
-    loop = src.split("for (int z = zs; z < zend; ++z)")
-    pro, body = loop[0], loop[1]
+    assert "__launch_bounds__(64)" in src and "int z = zs;   // column schedule" in src
+    pro, loops = src.split("int z = zs;   // column schedule")
+    main, rest = loops.split("for (; z < zend; ++z)")
+    assert "for (; z + 3 < zend; z += 4)" in main
     ring = lambda text: len(re.findall(r"R\d+_\d+_\d+ = A\.base\[", text))
-    assert ring(pro) == 2 * 6 * 3 and ring(body) == 2 * 6
-    assert body.count("__builtin_amdgcn_mov_dpp") == 16
-    assert len(re.findall(r"E\d+_\d+ = A\.base\[", body)) == 16
+    edge = lambda text: len(re.findall(r"E\d+_\d+ = A\.base\[", text))
+    assert ring(pro) == 2 * 6 * 3 and ring(main) == 4 * 2 * 6 and ring(rest) == 2 * 6
+    assert main.count("__builtin_amdgcn_mov_dpp") == 4 * 16 and rest.count("__builtin_amdgcn_mov_dpp") == 16
+    assert edge(main) == 4 * 16 and edge(rest) == 16
+    # the main loop renames (no ring copies); the remainder loop shifts the ring
+    assert not re.search(r"R\d+_\d+_\d+ = R", main) and len(re.findall(r"R\d+_\d+_\d+ = R", rest)) == 2 * 6 * 3
     # every read is clamped to its slot's allocation (lanes and rows past the box)
-    assert src.count("A.base[") == src.count("i_ < A.glo[") == 48 + 16
+    assert src.count("A.base[") == src.count("i_ < A.glo[") == 36 + 5 * (12 + 16)
     n = (6, 5, 4)
     L = nh.Layout.build(n, 1)
     f, g = host(L), host(L)
     m = nh.Mask(L, [(0, 0, 0)])
     db = nh.build_stencils(*n, 1, 1, 1)
     for other in (f * g + 1.0, nh.cond(m, db[("Gradient", "SVol", "SSurfX")](f))(g)):
-        assert "for (int z = zs; z < zend; ++z)" not in nh.fused_source(other)
+        assert "// column schedule" not in nh.fused_source(other)
     try:
         nh.fused_jit_column(True, 4, 16)
         src = nh.fused_source(t)
         assert "__launch_bounds__(256)" in src and "bz * 16" in src
         nh.fused_jit_column(False)
-        assert "for (int z = zs; z < zend; ++z)" not in nh.fused_source(t)
+        assert "// column schedule" not in nh.fused_source(t)
     finally:
         nh.fused_jit_column()
 
