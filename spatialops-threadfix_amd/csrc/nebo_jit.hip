@@ -322,7 +322,7 @@ void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int
 // come from the adjacent lanes (DPP), and lanes 0 and 63 load the value from
 // past the wave's edge.  Every index is clamped to its slot's [glo, ghi) so that
 // lanes and rows past the box read inside the allocation; their results are
-// not stored.
+// not stored (column_choice keeps the indices within 32 bits).
 struct ColumnPlan {
   int nf = 0, RY = 4;
   int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
@@ -729,6 +729,16 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
   cs->waves = g_column_waves.load();
   cs->zc = g_column_zc.load();
   cs->xcd = g_xcd.load();
+  // Indices are 32-bit: lanes, rows and planes past the box reach at most
+  // 66 + X (RY waves + 2) + 3 XY past a slot's last valid index before the
+  // clamp, which must not overflow (fields hold fewer than 2^31 points, so only
+  // fields within that margin of the limit keep the flattened schedule).
+  for (int s = 0; s < kVmMaxLoads; ++s) {
+    if (cp->fam[s] < 0) continue;
+    const int k = P.loads[s].cls;
+    const long reach = (long)P.loads[s].ghi + 66 + (long)P.cls_X[k] * (cp->RY * cs->waves + 2) + 3L * P.cls_XY[k];
+    if (reach >= 2147483647L) return false;
+  }
   if (cs->zc == 0) {
     cs->zc = kDefaultColumnZc;
     int device = 0;
