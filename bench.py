@@ -194,7 +194,8 @@ def _time_calls(fn, stream, reps, warm=3):
 
 def generic_leg(k, phi, ops, reps=20):
     """Trees that have no hand-written kernel, on the same 256^3 fields: each
-    evaluated by the kernel compiled at run time for its shape and by the
+    evaluated by the kernel compiled at run time for its shape (stencil trees on
+    the column schedule, and again with it off: `flattened_us`) and by the
     stack-program interpreter (assignments must agree bit for bit), plus
     reductions.  GB/s counts algorithmic bytes: 8 B per distinct field read per
     cell and 8 B per written cell; the div(k grad phi) tree is forced through the
@@ -224,14 +225,16 @@ def generic_leg(k, phi, ops, reps=20):
             if rc:
                 raise RuntimeError(lib.nebo_last_error().decode())
         e, outs = {}, []
-        for mode in ("compiled", "interpreter"):
-            nh.set_fused_jit(mode == "compiled")
+        for mode in ("compiled", "flattened", "interpreter"):
+            nh.set_fused_jit(mode != "interpreter")
+            nh.fused_jit_column(mode == "compiled")
             ms = _time_calls(call, stream, reps)
             e[mode + "_us"] = round(ms * 1e3, 2)
             e[mode + "_GBps"] = round(bpc * n3 / (ms * 1e-3) / 1e9, 1)
             outs.append(out.gpu.clone())
         nh.set_fused_jit(True)
-        e["bit_identical"] = bool(torch.equal(outs[0], outs[1]))
+        nh.fused_jit_column()
+        e["bit_identical"] = bool(torch.equal(outs[0], outs[2]) and torch.equal(outs[1], outs[2]))
         res["assign"][name] = e
     dev = torch.zeros(1, dtype=torch.float64, device="cuda")
     reds = {"max(phi)": (nh.E(phi), nh._abi.RED_MAX, 1, 8),
