@@ -993,16 +993,14 @@ static double transport_rhs_cell(const nebo_field_t* s, const nebo_field_t* k,
   return ((f[0] + f[1]) + f[2]) - ((v[0] + v[1]) + v[2]);
 }
 
-void oracle_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
-                            const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
-                            const nebo_axis_coefs_t* c, double dt, int stage)
+static void transport_stage_range(const nebo_field_t* out, const nebo_field_t* phin,
+                                  const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
+                                  const nebo_axis_coefs_t* c, double dt, int stage, const nebo_limits_t* L)
 {
-  nebo_limits_t L;
-  interior_limits(out, &L);
   double* o = (double*)out->values;
-  for (int z = L.lo[2]; z < L.hi[2]; ++z)
-    for (int y = L.lo[1]; y < L.hi[1]; ++y)
-      for (int x = L.lo[0]; x < L.hi[0]; ++x) {
+  for (int z = L->lo[2]; z < L->hi[2]; ++z)
+    for (int y = L->lo[1]; y < L->hi[1]; ++y)
+      for (int x = L->lo[0]; x < L->hi[0]; ++x) {
         double r = transport_rhs_cell(s, k, u, c, x, y, z);
         double pn = AT(phin, x, y, z), sv = AT(s, x, y, z);
         double v;
@@ -1011,6 +1009,56 @@ void oracle_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
         else v = (1.0 / 3.0) * pn + (2.0 / 3.0) * (sv + dt * r);
         o[field_index(out, x, y, z)] = v;
       }
+}
+
+void oracle_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
+                            const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
+                            const nebo_axis_coefs_t* c, double dt, int stage)
+{
+  nebo_limits_t L;
+  interior_limits(out, &L);
+  transport_stage_range(out, phin, s, k, u, c, dt, stage, &L);
+  oracle_fill_ghosts_periodic(out);
+}
+
+typedef struct transport_task {
+  const nebo_field_t *out, *phin, *s, *k, *u;
+  const nebo_axis_coefs_t* c;
+  double dt;
+  int stage;
+  nebo_limits_t lim;
+} transport_task_t;
+
+static void* task_transport(void* p)
+{
+  transport_task_t* t = (transport_task_t*)p;
+  transport_stage_range(t->out, t->phin, t->s, t->k, t->u, t->c, t->dt, t->stage, &t->lim);
+  return NULL;
+}
+
+/* The same stage with the interior split into the z slabs of the thread-parallel
+ * walk (NeboBasic.h:112-191); each cell is computed by the same code, so the
+ * result equals oracle_transport_stage bit for bit. */
+void oracle_transport_stage_threads(const nebo_field_t* out, const nebo_field_t* phin,
+                                    const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
+                                    const nebo_axis_coefs_t* c, double dt, int stage, int nthreads)
+{
+  nebo_limits_t L;
+  interior_limits(out, &L);
+  task_arg_t* parts;
+  int n = make_tasks(&L, nthreads, &parts);
+  transport_task_t* t = (transport_task_t*)calloc((size_t)n, sizeof(transport_task_t));
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    transport_task_t v = {out, phin, s, k, u, c, dt, stage, parts[i].lim};
+    t[i] = v;
+  }
+  for (int i = 1; i < n; ++i) pthread_create(&th[i], NULL, task_transport, &t[i]);
+  task_transport(&t[0]);
+  for (int i = 1; i < n; ++i) pthread_join(th[i], NULL);
+  free(th);
+  free(t);
+  free(parts);
   oracle_fill_ghosts_periodic(out);
 }
 

@@ -101,6 +101,11 @@ void oracle_diffusion_adjoint_step(const nebo_field_t* lam_out, double* gk,
 void oracle_transport_stage(const nebo_field_t* out, const nebo_field_t* phin,
                             const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
                             const nebo_axis_coefs_t* c, double dt, int stage);
+/* oracle_transport_stage with the interior split into z slabs over nthreads threads
+ * (bit-identical to the serial stage). */
+void oracle_transport_stage_threads(const nebo_field_t* out, const nebo_field_t* phin,
+                                    const nebo_field_t* s, const nebo_field_t* k, const nebo_field_t* u,
+                                    const nebo_axis_coefs_t* c, double dt, int stage, int nthreads);
 void oracle_transport_adjoint_stage(const nebo_field_t* out, double* acc_out, double* gk,
                                     const nebo_field_t* a, const double* acc_in,
                                     const nebo_field_t* s, const nebo_field_t* k,

@@ -250,6 +250,7 @@ extern "C" int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk, con
     return set_error(NEBO_ERR_INVALID, kCtx, "null argument");
   if (n_steps < 1 || every < 1 || n_steps % every)
     return set_error(NEBO_ERR_INVALID, kCtx, "n_steps must be a positive multiple of the checkpoint interval");
+  if (every > 1024) return set_error(NEBO_ERR_TOO_LARGE, kCtx, "checkpoint interval above 1024 steps");
   int rc;
   if ((rc = check_periodic_field(lam0, lamT)) || (rc = check_periodic_field(lamT, lamT)) ||
       (rc = check_periodic_field(k, lamT)))
@@ -267,8 +268,7 @@ extern "C" int nebo_diffusion_backward(const nebo_field_t* lam0, double* gk, con
   const int nseg = n_steps / every;
   for (int seg = nseg - 1; seg >= 0 && !rc; --seg) {
     // recompute states every*seg ... every*seg + every - 1 (state 0 of the segment is the checkpoint)
-    const double* states[1024];
-    if (every > 1024) return set_error(NEBO_ERR_TOO_LARGE, kCtx, "checkpoint interval above 1024 steps");
+    const double* states[1024];   // every <= 1024, checked on entry
     states[0] = ckpt + (size_t)seg * np;
     for (int t = 1; t < every && !rc; ++t) {
       nebo_field_t fin = with_values(lamT, const_cast<double*>(states[t - 1]));

@@ -95,9 +95,11 @@ __device__ __forceinline__ double lane_value(double v, int j)
 // every axis: rows y0-1 .. y0+RY of its column, and one edge load: lanes 0..RY-1
 // take the left neighbours (x = xw0-1) of rows 0..RY-1, lanes 32..32+RY-1 the
 // right neighbours (x = xw0+64), other lanes re-read their row 0.  Offsets are in
-// bytes from the plane's base (the frame origin of the plane, minus X + 1), so
-// they are non-negative 32-bit values and every load is base (uniform) + offset;
-// for an active lane the offset of row j+1 is also the store offset of row j.
+// bytes from the plane's base, the point (lo[0]-1, lo[1]-1) of the plane (see
+// plane_base), so they are non-negative 32-bit values whatever the limits (an
+// assignment that includes ghost cells starts below the frame origin) and every
+// load is base (uniform) + offset; for an active lane the offset of row j+1 is
+// also the store offset of row j.
 template <int RY>
 struct ColumnGeom {
   uint32_t row[RY + 2];
@@ -109,7 +111,8 @@ template <int RY>
 __device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int xw0, int y0, int lane)
 {
   ColumnGeom<RY> g;
-  auto off = [X](int x, int y) { return (uint32_t)((x + 1) + X * (y + 1)) * 8u; };
+  const int ax = T.lo[0] - 1, ay = T.lo[1] - 1;
+  auto off = [X, ax, ay](int x, int y) { return (uint32_t)((x - ax) + X * (y - ay)) * 8u; };
   const int xc = min(xw0 + lane, T.hi[0]);
 #pragma unroll
   for (int r = 0; r < RY + 2; ++r) g.row[r] = off(xc, min(y0 - 1 + r, T.hi[1]));
@@ -123,19 +126,21 @@ __device__ __forceinline__ ColumnGeom<RY> column_geom(const Tiles& T, int X, int
   return g;
 }
 
-// Plane z's base: frame origin + XY*z - X - 1 (uniform across the wave).
-template <typename T>
-__device__ __forceinline__ T* plane_base(T* origin, int X, int XY, int z)
+// Plane z's base: the point (lo[0]-1, lo[1]-1, z) of the frame, the lowest
+// point of the plane any stencil of the tile reads (uniform across the wave).
+template <typename V>
+__device__ __forceinline__ V* plane_base(V* origin, const Tiles& T, int X, int XY, int z)
 {
-  return origin + ((long)XY * z - X - 1);
+  return origin + ((long)XY * z + (T.lo[0] - 1) + (long)X * (T.lo[1] - 1));
 }
 
 // Raw buffer resource over one plane (base from plane_base): loads and stores
 // take the lane's 32-bit byte offset, so no per-lane address arithmetic is left
 // in the loop.  The range, (XY + 2X + 2) doubles, covers every offset the column
-// geometry forms (the hardware returns 0 for a load beyond it instead of
-// faulting; the launcher keeps it below 2^31 bytes).  Word 3 is the gfx9 raw
-// buffer format.
+// geometry forms (the points [lo-1, hi] of a box inside the block lie less than
+// XY points past the base; the hardware returns 0 for a load beyond the range
+// instead of faulting; the launcher keeps it below 2^31 bytes).  Word 3 is the
+// gfx9 raw buffer format.
 using Rsrc = __amdgpu_buffer_rsrc_t;
 using U2 = __attribute__((__vector_size__(2 * sizeof(unsigned int)))) unsigned int;
 

@@ -275,8 +275,8 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
   struct Col { double p[RY + 2], k[RY + 2], pe, ke, pi, ki; };
   auto load = [&](int z) {
     Col v;
-    const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z), X, XY);
-    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z), X, XY);
+    const Rsrc pb = plane_rsrc(plane_base(p, T, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z), X, XY);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) {
       v.p[q] = load_at(pb, g.row[q]);
@@ -297,8 +297,8 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
   for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
   double fz[RY];
   {
-    const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z0 - 1), X, XY);
-    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
+    const Rsrc pb = plane_rsrc(plane_base(p, T, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z0 - 1), X, XY);
 #pragma unroll
     for (int j = 0; j < RY; ++j)
       fz[j] = face_flux(load_at(kb, g.row[j + 1]), S[0].k[j + 1], load_at(pb, g.row[j + 1]), S[0].p[j + 1], c.iz0,
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
         fz[j] = fz_hi;
         fy_lo = fy_hi;
         const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
-        if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(r, X, XY, z), X, XY), g.row[j + 1], v);
+        if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(r, T, X, XY, z), X, XY), g.row[j + 1], v);
       }
       S[jj] = load(min(z + NS, z1));      // plane z is consumed: its slot takes z+NS
     }
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
   struct Col { double p[RY + 2], pe; };
   auto load = [&](int z) {
     Col v;
-    const Rsrc pb = plane_rsrc(plane_base(p, X, XY, z), X, XY);
+    const Rsrc pb = plane_rsrc(plane_base(p, T, X, XY, z), X, XY);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) v.p[q] = load_at(pb, g.row[q]);
     v.pe = load_at(pb, g.edge);
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
   double gz[RY];
 #pragma unroll
   for (int j = 0; j < RY; ++j)
-    gz[j] = load_at(plane_rsrc(plane_base(p, X, XY, z0 - 1), X, XY), g.row[j + 1]) * c.gz0 + S[0].p[j + 1] * c.gz1;
+    gz[j] = load_at(plane_rsrc(plane_base(p, T, X, XY, z0 - 1), X, XY), g.row[j + 1]) * c.gz0 + S[0].p[j + 1] * c.gz1;
   auto step = [&](auto JJ, const int z) {
     constexpr int jj = decltype(JJ)::value;
     const Col& C0 = S[jj];
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
       const double gz_hi = pc * c.gz0 + C1.p[j + 1] * c.gz1;
       const double v = lapl_cell(pc, pxm, pxp, C0.p[j], C0.p[j + 2], gz[j], gz_hi, c, s);
       gz[j] = gz_hi;
-      if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(out, X, XY, z), X, XY), g.row[j + 1], v);
+      if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(out, T, X, XY, z), X, XY), g.row[j + 1], v);
     }
     S[jj] = load(min(z + 3, z1));
   };

@@ -152,11 +152,11 @@ void transport_column_kernel(
   };
   auto load = [&](int z) __attribute__((always_inline)) {
     Col v;
-    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z), X, XY);
-    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z), X, XY);
-    const Rsrc xb = plane_rsrc(plane_base(ux, X, XY, z), X, XY);
-    const Rsrc yb = plane_rsrc(plane_base(uy, X, XY, z), X, XY);
-    const Rsrc zb = plane_rsrc(plane_base(uz, X, XY, z), X, XY);
+    const Rsrc sb = plane_rsrc(plane_base(s, T, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z), X, XY);
+    const Rsrc xb = plane_rsrc(plane_base(ux, T, X, XY, z), X, XY);
+    const Rsrc yb = plane_rsrc(plane_base(uy, T, X, XY, z), X, XY);
+    const Rsrc zb = plane_rsrc(plane_base(uz, T, X, XY, z), X, XY);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) {
       v.s[q] = load_at(sb, g.row[q]);
@@ -173,7 +173,7 @@ void transport_column_kernel(
 #pragma unroll
     for (int j = 0; j < RY + 1; ++j) v.uy[j] = load_at(yb, g.row[j + 1]);
     if constexpr (STAGE != 1) {
-      const Rsrc nb = plane_rsrc(plane_base(pn, X, XY, z), X, XY);
+      const Rsrc nb = plane_rsrc(plane_base(pn, T, X, XY, z), X, XY);
 #pragma unroll
       for (int j = 0; j < RY; ++j) v.pn[j] = load_at(nb, g.row[j + 1]);
     } else {
@@ -190,8 +190,8 @@ void transport_column_kernel(
   // low z faces of plane z0
   double fz[RY], vz[RY];
   {
-    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z0 - 1), X, XY);
-    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
+    const Rsrc sb = plane_rsrc(plane_base(s, T, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z0 - 1), X, XY);
 #pragma unroll
     for (int j = 0; j < RY; ++j) {
       const double szm = load_at(sb, g.row[j + 1]), kzm = load_at(kb, g.row[j + 1]);
@@ -206,7 +206,7 @@ void transport_column_kernel(
     {
       const Col& C0 = S[jj];              // plane z
       const Col& C1 = S[(jj + 1) % NS];   // plane z+1
-      const Rsrc ob = plane_rsrc(plane_base(out, X, XY, z), X, XY);
+      const Rsrc ob = plane_rsrc(plane_base(out, T, X, XY, z), X, XY);
 #pragma unroll
       for (int j = 0; j < RY; ++j) {
         const double sc = C0.s[j + 1], kc = C0.k[j + 1];
@@ -357,13 +357,13 @@ void transport_adjoint_column_kernel(
   };
   auto load = [&](int z) __attribute__((always_inline)) {
     Col v;
-    const Rsrc ab = plane_rsrc(plane_base(a, X, XY, z), X, XY);
-    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z), X, XY);
-    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z), X, XY);
-    const Rsrc xb = plane_rsrc(plane_base(ux, X, XY, z), X, XY);
-    const Rsrc yb = plane_rsrc(plane_base(uy, X, XY, z), X, XY);
-    const Rsrc zb = plane_rsrc(plane_base(uz, X, XY, z), X, XY);
-    const Rsrc gb = plane_rsrc(plane_base(static_cast<const double*>(gk), X, XY, z), X, XY);
+    const Rsrc ab = plane_rsrc(plane_base(a, T, X, XY, z), X, XY);
+    const Rsrc sb = plane_rsrc(plane_base(s, T, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z), X, XY);
+    const Rsrc xb = plane_rsrc(plane_base(ux, T, X, XY, z), X, XY);
+    const Rsrc yb = plane_rsrc(plane_base(uy, T, X, XY, z), X, XY);
+    const Rsrc zb = plane_rsrc(plane_base(uz, T, X, XY, z), X, XY);
+    const Rsrc gb = plane_rsrc(plane_base(static_cast<const double*>(gk), T, X, XY, z), X, XY);
 #pragma unroll
     for (int q = 0; q < RY + 2; ++q) {
       v.a[q] = load_at(ab, g.row[q]);
@@ -383,7 +383,7 @@ void transport_adjoint_column_kernel(
 #pragma unroll
     for (int j = 0; j < RY + 1; ++j) v.uy[j] = load_at(yb, g.row[j + 1]);
     if constexpr (STAGE != 3) {
-      const Rsrc ib = plane_rsrc(plane_base(acc_in, X, XY, z), X, XY);
+      const Rsrc ib = plane_rsrc(plane_base(acc_in, T, X, XY, z), X, XY);
 #pragma unroll
       for (int j = 0; j < RY; ++j) v.ain[j] = load_at(ib, g.row[j + 1]);
     } else {
@@ -399,9 +399,9 @@ void transport_adjoint_column_kernel(
   for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
   AdjFace zlo[RY];
   {
-    const Rsrc ab = plane_rsrc(plane_base(a, X, XY, z0 - 1), X, XY);
-    const Rsrc sb = plane_rsrc(plane_base(s, X, XY, z0 - 1), X, XY);
-    const Rsrc kb = plane_rsrc(plane_base(k, X, XY, z0 - 1), X, XY);
+    const Rsrc ab = plane_rsrc(plane_base(a, T, X, XY, z0 - 1), X, XY);
+    const Rsrc sb = plane_rsrc(plane_base(s, T, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z0 - 1), X, XY);
 #pragma unroll
     for (int j = 0; j < RY; ++j)
       zlo[j] = adj_face(load_at(ab, g.row[j + 1]), S[0].a[j + 1], load_at(sb, g.row[j + 1]), S[0].s[j + 1],
@@ -412,8 +412,8 @@ void transport_adjoint_column_kernel(
     {
       const Col& C0 = S[jj];
       const Col& C1 = S[(jj + 1) % NS];
-      const Rsrc ob = plane_rsrc(plane_base(out, X, XY, z), X, XY);
-      const Rsrc gb = plane_rsrc(plane_base(gk, X, XY, z), X, XY);
+      const Rsrc ob = plane_rsrc(plane_base(out, T, X, XY, z), X, XY);
+      const Rsrc gb = plane_rsrc(plane_base(gk, T, X, XY, z), X, XY);
 #pragma unroll
       for (int j = 0; j < RY; ++j) {
         const double ac = C0.a[j + 1], sc = C0.s[j + 1], kc = C0.k[j + 1];
@@ -440,10 +440,10 @@ void transport_adjoint_column_kernel(
           const uint32_t o = g.row[j + 1];
           if constexpr (STAGE == 3) {
             store_nt_at(ob, o, cs * Xv);
-            store_nt_at(plane_rsrc(plane_base(acc_out, X, XY, z), X, XY), o, (1.0 / 3.0) * ac);
+            store_nt_at(plane_rsrc(plane_base(acc_out, T, X, XY, z), X, XY), o, (1.0 / 3.0) * ac);
           } else if constexpr (STAGE == 2) {
             store_nt_at(ob, o, cs * Xv);
-            store_nt_at(plane_rsrc(plane_base(acc_out, X, XY, z), X, XY), o, C0.ain[j] + 0.75 * ac);
+            store_nt_at(plane_rsrc(plane_base(acc_out, T, X, XY, z), X, XY), o, C0.ain[j] + 0.75 * ac);
           } else {
             store_nt_at(ob, o, C0.ain[j] + Xv);
           }

@@ -46,6 +46,9 @@ _SIGS = {
                                              _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double]),
     "oracle_transport_stage": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT),
                                       _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double, C.c_int]),
+    "oracle_transport_stage_threads": (None, [_P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.FieldT),
+                                              _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double,
+                                              C.c_int, C.c_int]),
     "oracle_transport_adjoint_stage": (None, [_P(_abi.FieldT), _vp, _vp, _P(_abi.FieldT), _vp, _P(_abi.FieldT),
                                               _P(_abi.FieldT), _P(_abi.FieldT), _P(_abi.AxisCoefsT), C.c_double,
                                               C.c_int]),
@@ -231,11 +234,16 @@ def _face_array(us):
     return arr
 
 
-def transport_stage(out, phin, s, k, us, coefs, dt, stage):
-    """One SSP-RK3 stage of the transport right-hand side (fills out's periodic ghosts)."""
+def transport_stage(out, phin, s, k, us, coefs, dt, stage, threads=0):
+    """One SSP-RK3 stage of the transport right-hand side (fills out's periodic ghosts);
+    threads > 0 splits the interior into z slabs (same bits)."""
     do, dn, ds, dk = out.desc("host"), phin.desc("host"), s.desc("host"), k.desc("host")
-    lib().oracle_transport_stage(C.byref(do), C.byref(dn), C.byref(ds), C.byref(dk), _face_array(us),
-                                 C.byref(coefs), float(dt), int(stage))
+    if threads:
+        lib().oracle_transport_stage_threads(C.byref(do), C.byref(dn), C.byref(ds), C.byref(dk),
+                                             _face_array(us), C.byref(coefs), float(dt), int(stage), int(threads))
+    else:
+        lib().oracle_transport_stage(C.byref(do), C.byref(dn), C.byref(ds), C.byref(dk), _face_array(us),
+                                     C.byref(coefs), float(dt), int(stage))
     return out
 
 

@@ -164,3 +164,24 @@ def test_pending_entry_points_fail_loudly():
     out = C.c_void_p()
     rc = nh.lib().nebo_batch_create(None, 0, None, 0.0, None, C.byref(out))
     assert rc in (_abi.NEBO_ERR_UNSUPPORTED, _abi.NEBO_ERR_INVALID)
+
+
+def test_backward_checkpoint_interval_checked_on_entry():
+    """An interval above 1024 steps is refused before any work is queued (no
+    copy, no clearing of g_k): the descriptors here are host memory marked as
+    GPU 0, so reaching a HIP call would fail differently."""
+    L = nh.Layout.build((4, 4, 4), 1)
+    fs = [host_field(L, 1.0) for _ in range(3)]
+    ds = []
+    for f in fs:
+        d = f.layout.desc(f.host.ctypes.data, 0)
+        ds.append(d)
+    gk = np.full(L.npts, 3.0)
+    work = np.zeros(L.npts)
+    c = nh.axis_coefs(nh.build_stencils(4, 4, 4, 1, 1, 1))
+    rc = nh.lib().nebo_diffusion_backward(C.byref(ds[0]), gk.ctypes.data_as(C.c_void_p), C.byref(ds[1]),
+                                          C.byref(ds[2]), C.byref(c), 1e-3, 2048, 2048,
+                                          work.ctypes.data_as(C.c_void_p), work.ctypes.data_as(C.c_void_p), None)
+    assert rc == _abi.NEBO_ERR_TOO_LARGE
+    assert "checkpoint interval" in nh.lib().nebo_last_error().decode()
+    assert np.all(gk == 3.0)

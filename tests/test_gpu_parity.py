@@ -469,6 +469,63 @@ def test_every_stencil_tile_bit_exact(lengths):
         nh.stencil_reset_tile(1)
 
 
+def _two_ghost_case(n, seed):
+    L = nh.Layout.build(n, 2)
+    rng = np.random.default_rng(seed)
+    k = both(L, rng.uniform(0.5, 2.0, L.npts))
+    phi = both(L, rng.standard_normal(L.npts))
+    _, ops = _ops(n)
+    s = 0.1 / (n[0] ** 2)
+    lap = ops["DivX"](ops["GradX"](phi)) + ops["DivY"](ops["GradY"](phi)) + ops["DivZ"](ops["GradZ"](phi))
+    trees = {"divk form 0": _divkgrad_tree(ops, k, phi, 0), "divk form 1": _divkgrad_tree(ops, k, phi, 1),
+             "lapl+euler": phi + s * lap}
+    refs = {}
+    for name, t in trees.items():
+        for use_ghost in (True, False):
+            ref = nh.Field.from_numpy(L, np.full(L.npts, 5.0), where="cpu")
+            orc.assign(ref, t, use_ghost=use_ghost)
+            refs[name, use_ghost] = ref.host
+    return L, trees, refs
+
+
+def _check_hand_path(L, trees, refs, label):
+    for (name, use_ghost), want in refs.items():
+        out = gpu_empty_like(L, 5.0)
+        kid = nh.assign_plan(out, trees[name], use_ghost=use_ghost)
+        assert kid in (_abi.KERNEL_DIVKGRAD, _abi.KERNEL_LAPL_EULER), (label, name)
+        nh.assign(out, trees[name], use_ghost=use_ghost)
+        assert np.array_equal(fetch(out), want), (label, name, use_ghost)
+
+
+def test_hand_kernels_two_ghost_layers_every_tile():
+    """With two ghost layers a ghost-inclusive <<= of a one-cell stencil chain runs
+    from -1 to n+1 on every axis: the hand-written kernels then read the points
+    x = -2 and y = -2 of the frame.  Every tile, through nebo_assign (the plan picks
+    the hand-written kernel), equals the oracle's walk bit for bit, with the ghost
+    cells assigned (use_ghost) and with the interior only."""
+    L, trees, refs = _two_ghost_case((70, 37, 45), 21)
+    try:
+        for tile in nh.stencil_tiles():
+            print(f"two ghost layers, tile {tile}", flush=True)
+            nh.stencil_set_tile(0, *tile)
+            nh.stencil_set_tile(1, *tile)
+            _check_hand_path(L, trees, refs, tile)
+    finally:
+        nh.stencil_reset_tile(0)
+        nh.stencil_reset_tile(1)
+
+
+def test_hand_kernels_two_ghost_layers_default_tile():
+    """A 254^3 interior with two ghost layers: the ghost-inclusive region is 256^3,
+    for which the box-size rule picks the default column tile (128 x 8 rows, 64-plane
+    chunks); the interior-only region (254^3) is checked too."""
+    n = (254, 254, 254)
+    print("two ghost layers, 254^3 default tile", flush=True)
+    assert nh.stencil_select_tile(0, (256, 256, 256)) == (128, 8, 64, 7)
+    L, trees, refs = _two_ghost_case(n, 22)
+    _check_hand_path(L, trees, refs, "default")
+
+
 def test_tile_selection_by_box_size():
     """Without a pinned or tuned tile, boxes that the default tile fills (at
     least half the CUs get a workgroup) run it, smaller ones the direct

@@ -61,6 +61,19 @@ def test_stage_closed_form_equals_tree_walk(problem, stage):
     assert np.array_equal(closed.host, walk.host)
 
 
+@pytest.mark.parametrize("threads", [2, 3, 8])
+def test_threaded_stage_equals_serial(problem, threads):
+    """The z-slab split of the stage (used at 512^3) gives the serial stage's bits,
+    ghosts included, also with more threads than planes."""
+    n, L, Ls, ops, k, phi, u, us, c, dt = problem
+    s = periodic_random(L, np.random.default_rng(threads), lambda r, sh: r.standard_normal(sh))
+    for stage in (1, 2, 3):
+        a, b = hf(L, np.zeros(L.npts)), hf(L, np.zeros(L.npts))
+        orc.transport_stage(a, hf(L, phi), hf(L, s), hf(L, k), us, c, dt, stage)
+        orc.transport_stage(b, hf(L, phi), hf(L, s), hf(L, k), us, c, dt, stage, threads=threads)
+        assert np.array_equal(a.host, b.host), stage
+
+
 def _inner(L, a, b):
     return float(np.dot(interior(L, a).ravel(), interior(L, b).ravel()))
 
