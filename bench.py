@@ -295,61 +295,61 @@ def generic_leg(k, phi, ops, reps=20):
     return res
 
 
-def heat_leg(steps=100, cpu_steps=5):
+def heat_leg(steps=100, cpu_steps=5, per_graph=10):
     """The heat equation of examples/heat_equation.cpp at 64^3: each step
     phi' <<= phi + s*lap(phi) (hand-written Laplacian + Euler kernel), the
     zero-Dirichlet ghosts reset by phi' <<= cond(ghost mask, 0)(phi') (a
-    compiled kernel) and field_max, field_sum of phi' into device memory (no
-    host synchronisation inside the loop).  Timed with HIP events around
-    `steps` steps; the oracle runs `cpu_steps` of the same steps on the host."""
-    import ctypes as C
+    compiled kernel) and field_max, field_sum of phi' into device memory (one
+    kernel each, no host synchronisation inside the loop).  Timed with HIP events
+    around `steps` eager steps, and around the same steps replayed from a hipGraph
+    of `per_graph` captured steps (both runs must end in the same bits); the
+    oracle runs `cpu_steps` of the same steps on the host."""
     import numpy as np
     import torch
     import nebo_hip as nh
     from nebo_hip import workloads as W
     n = 64
-    L, ops, s, mask = W.heat_problem(n)
-    mask.add_consumer(0)
-    phi0 = W.heat_initial(n)
-    g = [nh.Field.from_numpy(L, phi0, where="gpu"), nh.Field.from_numpy(L, phi0, where="gpu")]
-    red = torch.zeros(2 * steps, dtype=torch.float64, device="cuda")
-    stream = torch.cuda.current_stream()
-    sp = C.c_void_p(stream.cuda_stream)
-    lib = nh.lib()
-    plans = []   # descriptors serialised once, as a time loop keeps them
-    for src in (0, 1):
-        e1, cond_of = W.heat_step_exprs(ops, g[src], s, mask)
-        dst = g[1 - src]
-        plans.append((dst.desc("gpu"), nh.serialize(e1, "gpu"), nh.serialize(cond_of(dst), "gpu"),
-                      nh.serialize(nh.E(dst), "gpu")))
-
-    def check(rc):
-        if rc:
-            raise RuntimeError(lib.nebo_last_error().decode())
-
-    def step(i):
-        d, e1, e2, ef = plans[i & 1]
-        check(lib.nebo_assign(C.byref(d), e1.ref, 1, sp))
-        check(lib.nebo_assign(C.byref(d), e2.ref, 1, sp))
-        check(lib.nebo_reduce_async(ef.ref, nh._abi.RED_MAX, 1, C.c_void_p(red[2 * i].data_ptr()), sp))
-        check(lib.nebo_reduce_async(ef.ref, nh._abi.RED_SUM, 1, C.c_void_p(red[2 * i + 1].data_ptr()), sp))
-
-    for i in range(2):   # warm: compiles the cond and reduction kernels
-        step(i)
-    g[0].gpu.copy_(torch.from_numpy(phi0))
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for i in range(steps):
-        step(i)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    gpu_ms = e0.elapsed_time(e1) / steps
-    maxes = red[0::2].cpu().numpy()
+    H = W.HeatStepper(n, slots=steps)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for i in range(2):   # warm: compiles the cond and reduction kernels, creates this stream's buffer
+            H.step(i, stream)
+        H.reset()
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(steps):
+            H.step(i, stream)
+        e1.record(stream)
+        stream.synchronize()
+        gpu_ms = e0.elapsed_time(e1) / steps
+        eager_phi = H.fields[steps % 2].gpu.clone()
+        eager_red = H.red.clone()
+        graph = torch.cuda.CUDAGraph()
+        H.reset()
+        stream.synchronize()
+    with torch.cuda.graph(graph, stream=stream):
+        for i in range(per_graph):
+            H.step(i, stream)
+    with torch.cuda.stream(stream):
+        H.reset()
+        H.red.zero_()
+        stream.synchronize()
+        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        g0.record(stream)
+        for _ in range(steps // per_graph):
+            graph.replay()
+        g1.record(stream)
+        stream.synchronize()
+        graph_ms = g0.elapsed_time(g1) / (steps // per_graph * per_graph)
+    same = bool(torch.equal(H.fields[steps % 2].gpu, eager_phi)
+                and torch.equal(H.red[:2 * per_graph], eager_red[2 * (steps - per_graph):2 * steps]))
+    maxes = eager_red[0::2].cpu().numpy()
     # the same steps on the host with the oracle (the reference's CPU backend)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
-    h = [nh.Field.from_numpy(L, phi0, where="cpu"), nh.Field.from_numpy(L, phi0, where="cpu")]
+    L, _, s, mask = W.heat_problem(n)
+    h = [nh.Field.from_numpy(L, H.phi0, where="cpu"), nh.Field.from_numpy(L, H.phi0, where="cpu")]
     c = nh.axis_coefs(nh.build_stencils(n, n, n, 1.0, 1.0, 1.0))
     t0 = time.perf_counter()
     for i in range(cpu_steps):
@@ -360,7 +360,9 @@ def heat_leg(steps=100, cpu_steps=5):
         orc.reduce(dst, nh._abi.RED_SUM)
     cpu_ms = (time.perf_counter() - t0) * 1e3 / cpu_steps
     return {"n": n, "steps": steps, "gpu_ms_per_step": round(gpu_ms, 4),
-            "kernels_per_step": "Laplacian+Euler, cond mask, 2 reductions (2 kernels each)",
+            "graph_ms_per_step": round(graph_ms, 4), "steps_per_graph": per_graph,
+            "graph_equals_eager": same,
+            "kernels_per_step": "Laplacian+Euler, cond mask, max, sum (one kernel each)",
             "max_first_last": [float(maxes[0]), float(maxes[-1])],
             "max_decreasing": bool(np.all(np.diff(maxes) <= 0)),
             "oracle_cpu_ms_per_step_serial": round(cpu_ms, 3), "cpu_steps_timed": cpu_steps}

@@ -101,6 +101,8 @@ enum nebo_opcode {
   NEBO_MASK_SHIFT,      /* a = NEBO_MASK node, idx = stencil table entry whose one
                            point P is the shift: mask(x + P) (boolean)
                            NeboMaskShift, NeboStencils.h:1048-1213             */
+  NEBO_INV_ERF,         /* erf^-1(a)  (boost::math::erf_inv)   NeboOperators.h:6150-6330 */
+  NEBO_INV_ERFC,        /* erfc^-1(a) (boost::math::erfc_inv), the InvErfcFcn after it */
   NEBO_NUM_OPCODES
 };
 

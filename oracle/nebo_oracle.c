@@ -185,7 +185,7 @@ int oracle_node_ghosts(const nebo_expr_t* e, int node, int with_bc, oracle_ghost
   case NEBO_NEG: case NEBO_ABS: case NEBO_SQRT: case NEBO_SQUARE: case NEBO_CUBE:
   case NEBO_POWINT: case NEBO_EXP: case NEBO_LOG: case NEBO_LOG10: case NEBO_SIN:
   case NEBO_COS: case NEBO_TAN: case NEBO_TANH: case NEBO_ERF: case NEBO_ERFC:
-  case NEBO_NOT:
+  case NEBO_INV_ERF: case NEBO_INV_ERFC: case NEBO_NOT:
     return oracle_node_ghosts(e, n->a, with_bc, out);
   default:
     if (n->op < 0 || n->op >= NEBO_NUM_OPCODES) return NEBO_ERR_INVALID;
@@ -220,7 +220,7 @@ static int node_extents(const nebo_expr_t* e, int node, int32_t ext[3])
   case NEBO_NEG: case NEBO_ABS: case NEBO_SQRT: case NEBO_SQUARE: case NEBO_CUBE:
   case NEBO_POWINT: case NEBO_EXP: case NEBO_LOG: case NEBO_LOG10: case NEBO_SIN:
   case NEBO_COS: case NEBO_TAN: case NEBO_TANH: case NEBO_ERF: case NEBO_ERFC:
-  case NEBO_NOT:
+  case NEBO_INV_ERF: case NEBO_INV_ERFC: case NEBO_NOT:
     return node_extents(e, n->a, ext);
   default:
     if (node_extents(e, n->a, ext)) return 1;
@@ -280,6 +280,93 @@ static inline int mask_bit(const nebo_mask_t* m, int x, int y, int z)
   return (m->bits[pos / 32] >> (pos % 32)) & 1u;
 }
 
+/* erf^-1 and erfc^-1: boost::math::erf_inv / erfc_inv in the reference's CPU
+ * walk (NeboOperators.h:6252-6262 and the InvErfcFcn that follows; tested at 5
+ * and 6 ulp in test/NeboTest.cpp:521-523).  Boost.Math is not available here, so
+ * they are restated from libm's erf and erfc: a start from Winitzki's closed-form
+ * approximation (S. Winitzki, "A handy approximation for the error function and
+ * its inverse", 2008; relative error below 2e-3) refined by Newton steps on the
+ * equation that keeps the residual accurate: erf(x) = y where x is small
+ * (|y| <= 0.5), ln(erfc(x)/q) = 0 in the tails.  End points give +-inf and
+ * arguments outside the domain NaN, as the GPU branch's erfinv/erfcinv do
+ * (Boost's default policy throws there instead). */
+static double winitzki_start(double L)   /* L = ln(1 - y^2); returns |x| */
+{
+  const double a = 0.147;
+  const double t = 2.0 / (M_PI * a) + 0.5 * L;
+  const double r = sqrt(t * t - L / a) - t;
+  return r > 0.0 ? sqrt(r) : 0.0;
+}
+
+/* e^{-x^2} / erfc(x) for x >= 0, used only as the Newton slope (its accuracy sets
+ * the convergence rate, not the root): exact form while e^{-x^2} is a normal
+ * number, the asymptotic series of erfc beyond. */
+static double erfc_slope_ratio(double x)
+{
+  if (x < 26.0) return exp(-x * x) / erfc(x);
+  const double u = 1.0 / (2.0 * x * x);
+  return sqrt(M_PI) * x / (1.0 - u + 3.0 * u * u - 15.0 * u * u * u);
+}
+
+/* ln(erfc(x)/q) for x >= 0: through erfc while erfc(x) is a normal number,
+ * else ln erfc(x) from its asymptotic series
+ *   erfc(x) = e^{-x^2} / (x sqrt(pi)) * sum_k (-1)^k (2k-1)!! / (2x^2)^k
+ * (10 terms; at x >= 26 the last one is below 1e-30), so a denormal q still has
+ * an accurate residual. */
+static double tail_residual(double x, double q)
+{
+  if (x < 26.0) return log1p((erfc(x) - q) / q);
+  const double u = 1.0 / (2.0 * x * x);
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k <= 10; ++k) {
+    term *= -(2.0 * k - 1.0) * u;
+    sum += term;
+  }
+  return (-x * x - log(x) - 0.5 * log(M_PI) + log(sum)) - log(q);
+}
+
+/* x >= 0 with erfc(x) = q, 0 < q <= 0.5 */
+static double inv_erfc_tail(double q)
+{
+  double x = winitzki_start(log(q * (2.0 - q)));
+  for (int it = 0; it < 8; ++it) {
+    const double h = tail_residual(x, q);
+    const double hp = -(2.0 / sqrt(M_PI)) * erfc_slope_ratio(x);
+    double xn = x - h / hp;
+    if (xn > 28.0) xn = 28.0;   /* erfc_inv of the least denormal is 27.21 */
+    if (xn < 0.0) xn = 0.0;
+    x = xn;
+  }
+  return x;
+}
+
+/* x with erf(x) = y, |y| <= 0.5 */
+static double inv_erf_centre(double y)
+{
+  double x = copysign(winitzki_start(log1p(-y * y)), y);
+  for (int it = 0; it < 6; ++it) x = x - (erf(x) - y) / ((2.0 / sqrt(M_PI)) * exp(-x * x));
+  return x;
+}
+
+double oracle_inv_erf(double y)
+{
+  if (isnan(y) || fabs(y) > 1.0) return NAN;
+  if (y == 0.0) return y;
+  if (fabs(y) == 1.0) return copysign(INFINITY, y);
+  if (fabs(y) <= 0.5) return inv_erf_centre(y);
+  return copysign(inv_erfc_tail(1.0 - fabs(y)), y);   /* 1 - |y| is exact here */
+}
+
+double oracle_inv_erfc(double q)
+{
+  if (isnan(q) || q < 0.0 || q > 2.0) return NAN;
+  if (q == 0.0) return INFINITY;
+  if (q == 2.0) return -INFINITY;
+  if (q > 1.0) return -oracle_inv_erfc(2.0 - q);   /* exact: 1 < q < 2 */
+  if (q >= 0.5) return q == 1.0 ? 0.0 : inv_erf_centre(1.0 - q);   /* erf(x) = 1 - q, exact */
+  return inv_erfc_tail(q);
+}
+
 double oracle_eval(const nebo_expr_t* e, int node, int x, int y, int z)
 {
   const nebo_node_t* n = &e->nodes[node];
@@ -327,6 +414,8 @@ double oracle_eval(const nebo_expr_t* e, int node, int x, int y, int z)
   case NEBO_TANH: return tanh(oracle_eval(e, n->a, x, y, z));
   case NEBO_ERF: return erf(oracle_eval(e, n->a, x, y, z));
   case NEBO_ERFC: return erfc(oracle_eval(e, n->a, x, y, z));
+  case NEBO_INV_ERF: return oracle_inv_erf(oracle_eval(e, n->a, x, y, z));
+  case NEBO_INV_ERFC: return oracle_inv_erfc(oracle_eval(e, n->a, x, y, z));
   case NEBO_EQ: return oracle_eval(e, n->a, x, y, z) == oracle_eval(e, n->b, x, y, z);
   case NEBO_NE: return oracle_eval(e, n->a, x, y, z) != oracle_eval(e, n->b, x, y, z);
   case NEBO_LT: return oracle_eval(e, n->a, x, y, z) < oracle_eval(e, n->b, x, y, z);

@@ -30,6 +30,10 @@ int oracle_assign_limits(const nebo_field_t* lhs, const nebo_expr_t* rhs, int us
 
 /* Value of node at (x,y,z) in the frame of `frame_field`'s valid-ghost-excluded
  * origin (NeboConstField SeqWalk eval, NeboRhs.h:271-285). */
+/* boost::math::erf_inv / erfc_inv restated from libm erf/erfc (Newton from
+ * Winitzki's approximation); +-inf at the end points, NaN outside the domain. */
+double oracle_inv_erf(double y);
+double oracle_inv_erfc(double q);
 double oracle_eval(const nebo_expr_t* e, int node, int x, int y, int z);
 
 /* lhs <<= rhs, serial z,y,x walk (NeboLhs.h:315-332, 611-623). */

@@ -480,6 +480,8 @@ NEBO_FUNCTION(log, NEBO_LOG)
 NEBO_FUNCTION(log10, NEBO_LOG10)
 NEBO_FUNCTION(erf, NEBO_ERF)
 NEBO_FUNCTION(erfc, NEBO_ERFC)
+NEBO_FUNCTION(inv_erf, NEBO_INV_ERF)
+NEBO_FUNCTION(inv_erfc, NEBO_INV_ERFC)
 NEBO_FUNCTION(square, NEBO_SQUARE)
 NEBO_FUNCTION(cube, NEBO_CUBE)
 #undef NEBO_FUNCTION

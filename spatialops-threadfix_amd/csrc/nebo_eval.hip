@@ -20,6 +20,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <unordered_map>
 
@@ -29,6 +31,20 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kVmV = 2;           // z cells per thread in the box modes
 constexpr int kRedMaxBlocks = 2048;
+
+// A reduction's working memory: 2 * kRedMaxBlocks partial doubles, a result
+// slot and the last-block ticket, kRedBufBytes in all.
+struct RedBuf {
+  double* partial;
+  double* result;
+  unsigned* ticket;
+};
+constexpr size_t kRedBufBytes = sizeof(double) * (2 * kRedMaxBlocks + 1) + 256;
+inline RedBuf red_buf_at(char* p)
+{
+  double* d = reinterpret_cast<double*>(p);
+  return RedBuf{d, d + 2 * kRedMaxBlocks, reinterpret_cast<unsigned*>(d + 2 * kRedMaxBlocks + 1)};
+}
 
 __device__ __noinline__ double vm_math(int op, double x)
 {
@@ -42,6 +58,8 @@ __device__ __noinline__ double vm_math(int op, double x)
   case VM_TANH: return tanh(x);
   case VM_ERF: return erf(x);
   case VM_ERFC: return erfc(x);
+  case VM_INV_ERF: return erfinv(x);
+  case VM_INV_ERFC: return erfcinv(x);
   default: return x;
   }
 }
@@ -83,7 +101,7 @@ __device__ __forceinline__ void vm_exec(double (&S)[kVmMaxDepth][V], const VmPro
 {
   constexpr int pops = vm_pops<OP>();
   constexpr bool push = (OP == VM_LOAD || OP == VM_LOADG || OP == VM_CONST || OP == VM_MASK);
-  constexpr bool math = (OP == VM_POW || OP == VM_POWC || (OP >= VM_EXP && OP <= VM_ERFC));
+  constexpr bool math = (OP == VM_POW || OP == VM_POWC || (OP >= VM_EXP && OP <= VM_INV_ERFC));
   if constexpr (D < pops || (push && D >= kVmMaxDepth) || (math && !MATH)) {
     return;  // never emitted by the host for this instantiation
   } else if constexpr (OP == VM_LOAD) {
@@ -172,10 +190,11 @@ __device__ __forceinline__ void vm_exec(double (&S)[kVmMaxDepth][V], const VmPro
   VM_CD(VM_AND) VM_CD(VM_OR) VM_CD(VM_NEG) VM_CD(VM_ABS) VM_CD(VM_SQRT)                \
   VM_CD(VM_SQUARE) VM_CD(VM_CUBE) VM_CD(VM_POWC) VM_CD(VM_EXP) VM_CD(VM_LOG)           \
   VM_CD(VM_LOG10) VM_CD(VM_SIN) VM_CD(VM_COS) VM_CD(VM_TAN) VM_CD(VM_TANH)             \
-  VM_CD(VM_ERF) VM_CD(VM_ERFC) VM_CD(VM_NOT) VM_CD(VM_SELECT) VM_CD(VM_LOADG)
+  VM_CD(VM_ERF) VM_CD(VM_ERFC) VM_CD(VM_INV_ERF) VM_CD(VM_INV_ERFC) VM_CD(VM_NOT)           \
+  VM_CD(VM_SELECT) VM_CD(VM_LOADG)
 
 static_assert(kVmMaxDepth == 10, "VM_CD covers depths 0..10");
-static_assert(VM_NUM_OPS == 37, "VM_ALL_CASES lists every opcode");
+static_assert(VM_NUM_OPS == 39, "VM_ALL_CASES lists every opcode");
 
 template <int NCLS, int V, bool MATH>
 __device__ __forceinline__ void vm_run(const VmProgram& P, double (&S)[kVmMaxDepth][V],
@@ -320,9 +339,51 @@ __device__ DD block_reduce(DD v)
   return v;
 }
 
+// The result of a reduction from its block partials: the partials folded in
+// index order (thread t takes partials t, t + kBlock, ...) and then the block's
+// fixed tree, so the value does not depend on which block finished last;
+// max/min return the selected element itself (a -0.0 stays -0.0).
+template <int OP>
+__device__ void fold_partials(const double* partial, int n, double* out)
+{
+  DD acc = red_init<OP>();
+  for (int i = threadIdx.x; i < n; i += kBlock)
+    acc = red_combine<OP>(acc, DD{partial[2 * i], partial[2 * i + 1]});
+  DD r = block_reduce<OP>(acc);
+  if (threadIdx.x == 0) {
+    double v = (OP == NEBO_RED_MAX || OP == NEBO_RED_MIN) ? r.hi : r.hi + r.lo;
+    if constexpr (OP == NEBO_RED_NORM) v = __builtin_sqrt(v);
+    *out = v;
+  }
+}
+
+// Last-block fold: thread 0 of every block has stored the block's partial; it
+// releases it (agent-scope fence: the partials of blocks on other XCDs become
+// visible) and takes a ticket.  The block holding the last ticket acquires every
+// partial, folds them into *out and resets the ticket to 0 for the next
+// reduction that uses this buffer (the ticket starts at 0 when the buffer is
+// created).
+template <int OP>
+__device__ void fold_if_last(const double* partial, int n, unsigned* ticket, double* out)
+{
+  __shared__ int last;
+  __syncthreads();   // the block tree is done with its shared words
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == (unsigned)(n - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  fold_partials<OP>(partial, n, out);
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+}
+
 template <int NCLS, int V, int OP, bool MATH>
 __global__ __launch_bounds__(kBlock) void vm_reduce_kernel(const VmProgram P, Box B,
-                                                           double* __restrict__ partial)
+                                                           double* __restrict__ partial,
+                                                           unsigned* __restrict__ ticket,
+                                                           double* __restrict__ out)
 {
   DD acc = red_init<OP>();
   const long items = (long)B.nplane * B.zchunks;
@@ -359,22 +420,7 @@ __global__ __launch_bounds__(kBlock) void vm_reduce_kernel(const VmProgram P, Bo
     partial[2 * blockIdx.x] = r.hi;
     partial[2 * blockIdx.x + 1] = r.lo;
   }
-}
-
-template <int OP>
-__global__ __launch_bounds__(kBlock) void reduce_final_kernel(const double* __restrict__ partial,
-                                                              int n, double* __restrict__ out)
-{
-  DD acc = red_init<OP>();
-  // fixed assignment of partials to threads, then a fixed tree: deterministic
-  for (int i = threadIdx.x; i < n; i += kBlock)
-    acc = red_combine<OP>(acc, DD{partial[2 * i], partial[2 * i + 1]});
-  DD r = block_reduce<OP>(acc);
-  if (threadIdx.x == 0) {
-    double v = r.hi + r.lo;
-    if constexpr (OP == NEBO_RED_NORM) v = __builtin_sqrt(v);
-    *out = v;
-  }
+  fold_if_last<OP>(partial, gridDim.x, ticket, out);
 }
 
 Box make_box(const nebo_limits_t& L, int V)
@@ -422,7 +468,7 @@ bool subtree_reads(const nebo_expr_t* e, int node, const void* values)
     if (subtree_reads(e, n.a, values)) return true;
     if (n.op == NEBO_STENCIL || n.op == NEBO_NEG || n.op == NEBO_ABS || n.op == NEBO_SQRT ||
         n.op == NEBO_SQUARE || n.op == NEBO_CUBE || n.op == NEBO_POWINT || n.op == NEBO_NOT ||
-        (n.op >= NEBO_EXP && n.op <= NEBO_ERFC))
+        (n.op >= NEBO_EXP && n.op <= NEBO_ERFC) || n.op == NEBO_INV_ERF || n.op == NEBO_INV_ERFC)
       return false;
     if (subtree_reads(e, n.b, values)) return true;
     return n.op == NEBO_COND && subtree_reads(e, n.c, values);
@@ -515,7 +561,7 @@ bool uses_math(const VmProgram& P)
 {
   for (int i = 0; i < P.n_insns; ++i) {
     const int op = P.insns[i].code >> 4;
-    if (op == VM_POW || op == VM_POWC || (op >= VM_EXP && op <= VM_ERFC)) return true;
+    if (op == VM_POW || op == VM_POWC || (op >= VM_EXP && op <= VM_INV_ERFC)) return true;
   }
   return false;
 }
@@ -582,82 +628,105 @@ int do_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, vo
 }
 
 template <int NCLS, int OP, bool MATH>
-void launch_reduce_op(const VmProgram& prog, const Box& B, int blocks, double* partial,
-                      double* out_dev, hipStream_t s)
+void launch_reduce_op(const VmProgram& prog, const Box& B, int blocks, const RedBuf& buf, double* out_dev,
+                      hipStream_t s)
 {
-  hipLaunchKernelGGL((vm_reduce_kernel<NCLS, kVmV, OP, MATH>), dim3(blocks), dim3(kBlock), 0, s, prog, B, partial);
-  hipLaunchKernelGGL((reduce_final_kernel<OP>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev);
+  hipLaunchKernelGGL((vm_reduce_kernel<NCLS, kVmV, OP, MATH>), dim3(blocks), dim3(kBlock), 0, s, prog, B,
+                     buf.partial, buf.ticket, out_dev);
 }
 
 template <int NCLS, bool MATH>
-void launch_reduce_any(const VmProgram& prog, const Box& B, int op, int blocks, double* partial,
+void launch_reduce_any(const VmProgram& prog, const Box& B, int op, int blocks, const RedBuf& buf,
                        double* out_dev, hipStream_t s)
 {
   switch (op) {
-  case NEBO_RED_MAX: launch_reduce_op<NCLS, NEBO_RED_MAX, MATH>(prog, B, blocks, partial, out_dev, s); break;
-  case NEBO_RED_MIN: launch_reduce_op<NCLS, NEBO_RED_MIN, MATH>(prog, B, blocks, partial, out_dev, s); break;
-  case NEBO_RED_SUM: launch_reduce_op<NCLS, NEBO_RED_SUM, MATH>(prog, B, blocks, partial, out_dev, s); break;
-  default: launch_reduce_op<NCLS, NEBO_RED_NORM, MATH>(prog, B, blocks, partial, out_dev, s); break;
+  case NEBO_RED_MAX: launch_reduce_op<NCLS, NEBO_RED_MAX, MATH>(prog, B, blocks, buf, out_dev, s); break;
+  case NEBO_RED_MIN: launch_reduce_op<NCLS, NEBO_RED_MIN, MATH>(prog, B, blocks, buf, out_dev, s); break;
+  case NEBO_RED_SUM: launch_reduce_op<NCLS, NEBO_RED_SUM, MATH>(prog, B, blocks, buf, out_dev, s); break;
+  default: launch_reduce_op<NCLS, NEBO_RED_NORM, MATH>(prog, B, blocks, buf, out_dev, s); break;
   }
 }
 
-void launch_reduce_final(int op, const double* partial, int blocks, double* out_dev, hipStream_t s)
-{
-  switch (op) {
-  case NEBO_RED_MAX: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MAX>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  case NEBO_RED_MIN: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MIN>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  case NEBO_RED_SUM: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_SUM>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  default: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_NORM>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  }
-}
-
-int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, double* out_dev, hipStream_t s)
+// One launch: the kernel compiled for the tree's shape (compiled now if
+// `compile`, else only when a previous call compiled it) or the interpreter's
+// kernel; either folds its block partials in its last block.
+int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, double* out_dev, const RedBuf& buf,
+                  bool compile, hipStream_t s)
 {
   const Box B = make_box(lim, kVmV);
   const long items = (long)B.nplane * B.zchunks;
   int blocks = (int)((items + kBlock - 1) / kBlock);
   if (blocks > kRedMaxBlocks) blocks = kRedMaxBlocks;
   if (blocks < 1) blocks = 1;
-  double* partial = nullptr;
-  // room for the largest grid of either kernel
-  int rc = check_hip(hipMallocAsync((void**)&partial, sizeof(double) * 2 * kRedMaxBlocks, s), "Nebo Reduction");
-  if (rc) return rc;
   bool done = false;
-  int jit_blocks = 0;
-  rc = jit_reduce(prog, lim, op, partial, kRedMaxBlocks, s, &jit_blocks, &done);
-  if (rc || done) {
-    if (done) {
-      launch_reduce_final(op, partial, jit_blocks, out_dev, s);
-      rc = check_hip(hipGetLastError(), "Nebo Reduction");
-    }
-    (void)hipFreeAsync(partial, s);
+  int rc = jit_reduce(prog, lim, op, buf.partial, kRedMaxBlocks, buf.ticket, out_dev, compile, s, &done);
+  if (rc || done) return rc;
+  const bool one = prog.n_classes == 1, math = uses_math(prog);
+  if (one && !math) launch_reduce_any<1, false>(prog, B, op, blocks, buf, out_dev, s);
+  else if (one) launch_reduce_any<1, true>(prog, B, op, blocks, buf, out_dev, s);
+  else if (!math) launch_reduce_any<kVmMaxClasses, false>(prog, B, op, blocks, buf, out_dev, s);
+  else launch_reduce_any<kVmMaxClasses, true>(prog, B, op, blocks, buf, out_dev, s);
+  return check_hip(hipGetLastError(), "Nebo Reduction");
+}
+
+__global__ void zero_words_kernel(unsigned* p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// Reduction buffers: one per (device, stream), created by the first reduction
+// on that stream and kept until the process ends.  Reductions on one stream run
+// in stream order and share it; reductions on other streams use their own.
+struct StreamRedBuf {
+  RedBuf buf;
+  std::mutex* mu;   // held by nebo_reduce from its launch to its copy of the result
+};
+std::mutex g_redbuf_mu;
+std::map<std::pair<int, void*>, StreamRedBuf> g_redbuf;
+
+int stream_reduce_buffer(void* stream, StreamRedBuf* out)
+{
+  static const char* ctx = "Nebo Reduction";
+  int device = 0;
+  int rc = check_hip(hipGetDevice(&device), ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_redbuf_mu);
+  const auto key = std::make_pair(device, stream);
+  auto it = g_redbuf.find(key);
+  if (it != g_redbuf.end()) {
+    *out = it->second;
+    return NEBO_OK;
+  }
+  char* p = nullptr;
+  if ((rc = check_hip(hipMalloc((void**)&p, kRedBufBytes), ctx))) return rc;
+  // zeroed by a kernel on the caller's stream (ordered before the first reduction)
+  const int words = (int)(kRedBufBytes / sizeof(unsigned));
+  hipLaunchKernelGGL(zero_words_kernel, dim3((words + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<unsigned*>(p), words);
+  if ((rc = check_hip(hipGetLastError(), ctx))) {
+    (void)hipFree(p);
     return rc;
   }
-  const bool one = prog.n_classes == 1, math = uses_math(prog);
-  if (one && !math) launch_reduce_any<1, false>(prog, B, op, blocks, partial, out_dev, s);
-  else if (one) launch_reduce_any<1, true>(prog, B, op, blocks, partial, out_dev, s);
-  else if (!math) launch_reduce_any<kVmMaxClasses, false>(prog, B, op, blocks, partial, out_dev, s);
-  else launch_reduce_any<kVmMaxClasses, true>(prog, B, op, blocks, partial, out_dev, s);
-  rc = check_hip(hipGetLastError(), "Nebo Reduction");
-  (void)hipFreeAsync(partial, s);
-  return rc;
+  StreamRedBuf b{red_buf_at(p), new std::mutex};
+  g_redbuf[key] = b;
+  *out = b;
+  return NEBO_OK;
 }
 
 __global__ void fill_value_kernel(double* out, double v) { *out = v; }
 
-int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void* stream)
+// Reduction of e into out_dev (device memory) on stream, with buf as working
+// memory; `compile`: a tree shape met for the first time is compiled now (else
+// it runs on the interpreter's kernel until an allocating call compiles it).
+int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void* stream, const RedBuf& buf,
+              bool compile)
 {
   static const char* ctx = "Nebo Reduction";
-  if (!e || !out_dev) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
-  if (op < NEBO_RED_MAX || op > NEBO_RED_NORM) return set_error(NEBO_ERR_INVALID, ctx, "unknown reduction");
   nebo_limits_t lim;
   int empty = 0;
   int rc = reduce_limits(e, use_ghost, &lim, &empty);
   if (rc) return rc;
-  int device = -1;
-  for (int i = 0; i < e->n_fields; ++i) { device = e->fields[i].device; break; }
-  if (e->n_fields > 0 && (rc = check_devices(e, device, ctx))) return rc;
-  if ((rc = check_index_range(e, nullptr, ctx))) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (empty) {
     const double v = op == NEBO_RED_MAX ? -INFINITY : op == NEBO_RED_MIN ? INFINITY : 0.0;
@@ -667,7 +736,23 @@ int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void
   VmProgram prog;
   int depth = 0;
   if ((rc = lower_program(e, &prog, &depth))) return rc;
-  return launch_reduce(prog, lim, op, out_dev, s);
+  return launch_reduce(prog, lim, op, out_dev, buf, compile, s);
+}
+
+// Argument checks shared by the reduction entry points.
+int check_reduce(const nebo_expr_t* e, int op, int use_ghost)
+{
+  static const char* ctx = "Nebo Reduction";
+  if (!e) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  if (op < NEBO_RED_MAX || op > NEBO_RED_NORM) return set_error(NEBO_ERR_INVALID, ctx, "unknown reduction");
+  nebo_limits_t lim;
+  int empty = 0;
+  int rc = reduce_limits(e, use_ghost, &lim, &empty);   // the tree's own errors first
+  if (rc) return rc;
+  int device = -1;
+  for (int i = 0; i < e->n_fields; ++i) { device = e->fields[i].device; break; }
+  if (e->n_fields > 0 && (rc = check_devices(e, device, ctx))) return rc;
+  return check_index_range(e, nullptr, ctx);
 }
 
 }  // namespace
@@ -745,21 +830,32 @@ extern "C" int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
                                  double* out_dev, void* stream)
 {
   clear_error();
-  return do_reduce(expr, op, use_ghost, out_dev, stream);
+  int rc = check_reduce(expr, op, use_ghost);
+  if (rc) return rc;
+  if (!out_dev) return set_error(NEBO_ERR_INVALID, "Nebo Reduction", "null argument");
+  StreamRedBuf b;
+  if ((rc = stream_reduce_buffer(stream, &b))) return rc;
+  return do_reduce(expr, op, use_ghost, out_dev, stream, b.buf, true);
 }
 
 extern "C" int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
                            void* stream)
 {
+  static const char* ctx = "Nebo Reduction";
   clear_error();
-  if (!out) return set_error(NEBO_ERR_INVALID, "Nebo Reduction", "null argument");
-  hipStream_t s = (hipStream_t)stream;
-  double* dev = nullptr;
-  int rc = check_hip(hipMallocAsync((void**)&dev, sizeof(double), s), "Nebo Reduction");
+  int rc = check_reduce(expr, op, use_ghost);
   if (rc) return rc;
-  rc = do_reduce(expr, op, use_ghost, dev, stream);
-  if (!rc) rc = check_hip(hipMemcpyAsync(out, dev, sizeof(double), hipMemcpyDeviceToHost, s), "Nebo Reduction");
-  (void)hipFreeAsync(dev, s);
-  if (!rc) rc = check_hip(hipStreamSynchronize(s), "Nebo Reduction");
+  if (!out) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  StreamRedBuf b;
+  if ((rc = stream_reduce_buffer(stream, &b))) return rc;
+  {
+    // the result slot is the stream's: no other nebo_reduce on this stream may
+    // write it between this reduction and the copy of its result
+    std::lock_guard<std::mutex> lk(*b.mu);
+    rc = do_reduce(expr, op, use_ghost, b.buf.result, stream, b.buf, true);
+    if (!rc) rc = check_hip(hipMemcpyAsync(out, b.buf.result, sizeof(double), hipMemcpyDeviceToHost, s), ctx);
+  }
+  if (!rc) rc = check_hip(hipStreamSynchronize(s), ctx);
   return rc;
 }

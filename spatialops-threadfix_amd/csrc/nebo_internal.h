@@ -52,7 +52,8 @@ enum VmOp : uint8_t {
   VM_ADD, VM_SUB, VM_MUL, VM_DIV, VM_MAX, VM_MIN, VM_POW,
   VM_EQ, VM_NE, VM_LT, VM_LE, VM_GT, VM_GE, VM_AND, VM_OR,
   VM_NEG, VM_ABS, VM_SQRT, VM_SQUARE, VM_CUBE, VM_POWC,  // POWC: pow(top, const[imm])
-  VM_EXP, VM_LOG, VM_LOG10, VM_SIN, VM_COS, VM_TAN, VM_TANH, VM_ERF, VM_ERFC, VM_NOT,
+  VM_EXP, VM_LOG, VM_LOG10, VM_SIN, VM_COS, VM_TAN, VM_TANH, VM_ERF, VM_ERFC, VM_INV_ERF, VM_INV_ERFC,
+  VM_NOT,
   VM_SELECT,     // c = pop, b = pop, a = pop, push (a != 0 ? b : c)
   VM_LOADG,      // VM_LOAD that yields NaN where the read leaves the field's block
   VM_NUM_OPS
@@ -107,11 +108,13 @@ int device_cu_count(int device);
 // compilation is off or failed for this program (the interpreter then runs it).
 bool jit_enabled();
 int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done);
-// Reduction: writes one (hi, lo) partial per block into partial (room for
-// max_blocks) and the block count into *blocks; the caller folds the partials
-// with reduce_final_kernel.
-int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks, void* stream,
-               int* blocks, bool* done);
+// Reduction: one launch whose blocks write their (hi, lo) partials into
+// partial (room for max_blocks) and whose last block folds them into *out and
+// resets *ticket to 0.  Sets *done, or leaves it false when run-time compilation
+// is off, failed for this program, or (compile false) the program's kernel has
+// not been compiled by an earlier call.
+int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks,
+               unsigned* ticket, double* out, bool compile, void* stream, bool* done);
 
 // One (field, offset) read of an expression; edgeless when an edgeless stencil
 // lies between the read and the root.

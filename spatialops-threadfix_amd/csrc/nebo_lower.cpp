@@ -92,7 +92,7 @@ bool is_unary(int op)
   case NEBO_NEG: case NEBO_ABS: case NEBO_SQRT: case NEBO_SQUARE: case NEBO_CUBE:
   case NEBO_POWINT: case NEBO_EXP: case NEBO_LOG: case NEBO_LOG10: case NEBO_SIN:
   case NEBO_COS: case NEBO_TAN: case NEBO_TANH: case NEBO_ERF: case NEBO_ERFC:
-  case NEBO_NOT: case NEBO_STENCIL: case NEBO_MASK_SHIFT:
+  case NEBO_INV_ERF: case NEBO_INV_ERFC: case NEBO_NOT: case NEBO_STENCIL: case NEBO_MASK_SHIFT:
     return true;
   default:
     return false;
@@ -449,6 +449,7 @@ struct Lowerer {
     case NEBO_EXP: return VM_EXP; case NEBO_LOG: return VM_LOG; case NEBO_LOG10: return VM_LOG10;
     case NEBO_SIN: return VM_SIN; case NEBO_COS: return VM_COS; case NEBO_TAN: return VM_TAN;
     case NEBO_TANH: return VM_TANH; case NEBO_ERF: return VM_ERF; case NEBO_ERFC: return VM_ERFC;
+    case NEBO_INV_ERF: return VM_INV_ERF; case NEBO_INV_ERFC: return VM_INV_ERFC;
     case NEBO_NOT: return VM_NOT;
     default: return -1;
     }

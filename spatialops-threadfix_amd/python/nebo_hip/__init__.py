@@ -6,7 +6,7 @@ builds expression descriptors, owns test fields and calls that ABI.
 from ._abi import NeboError, lib, header_symbols, LIB_PATH  # noqa: F401
 from .fields import Layout, Field, Mask, LOCATIONS, FACE_TYPES  # noqa: F401
 from .expr import (E, Expr, Stencil, cond, serialize, sin, cos, tan, exp, tanh, log, log10,  # noqa: F401
-                   sqrt, square, cube, erf, erfc, abs_, max_, min_, pow_, powint, mask_shift)
+                   sqrt, square, cube, erf, erfc, inv_erf, inv_erfc, abs_, max_, min_, pow_, powint, mask_shift)
 from .ops import build_stencils, basic_op_types, axis_coefs, OperatorDatabase  # noqa: F401
 from .api import (lower_stats, assign, interior_assign, masked_assign, assign_points,  # noqa: F401
                   assign_limits, assign_plan,

@@ -31,7 +31,7 @@ OPCODES = [
     "SCALAR", "FIELD", "MASK", "ADD", "SUB", "MUL", "DIV", "NEG", "MAX", "MIN",
     "ABS", "SQRT", "SQUARE", "CUBE", "POW", "POWINT", "EXP", "LOG", "LOG10", "SIN",
     "COS", "TAN", "TANH", "ERF", "ERFC", "EQ", "NE", "LT", "LE", "GT", "GE",
-    "AND", "OR", "NOT", "STENCIL", "COND", "MASK_SHIFT",
+    "AND", "OR", "NOT", "STENCIL", "COND", "MASK_SHIFT", "INV_ERF", "INV_ERFC",
 ]
 OP = {name: i for i, name in enumerate(OPCODES)}
 

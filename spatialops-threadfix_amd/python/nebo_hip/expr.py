@@ -74,6 +74,7 @@ def _un(op):
 
 sin, cos, tan, exp, tanh, log, log10 = (_un(o) for o in ("SIN", "COS", "TAN", "EXP", "TANH", "LOG", "LOG10"))
 sqrt, square, cube, erf, erfc = (_un(o) for o in ("SQRT", "SQUARE", "CUBE", "ERF", "ERFC"))
+inv_erf, inv_erfc = _un("INV_ERF"), _un("INV_ERFC")   # erf^-1, erfc^-1 (NeboOperators.h:6150-6330)
 abs_ = _un("ABS")
 
 

@@ -57,6 +57,54 @@ def heat_step_exprs(ops, phi, s, mask):
     return phi + s * lap, (lambda out: cond(mask, 0.0)(out))
 
 
+class HeatStepper:
+    """The heat-equation time step of examples/heat_equation.cpp:216-242 through
+    the C-ABI, with the expression descriptors serialised once (as a time loop
+    keeps them): phi' <<= phi + s*lap(phi) (the Laplacian + Euler kernel),
+    phi' <<= cond(ghost mask, 0)(phi'), then field_max(phi') and field_sum(phi')
+    written to device memory (red[2i], red[2i + 1]) without a host
+    synchronisation.  Step i reads block i % 2 and writes the other, so an even
+    number of steps ends in block 0.  Every call allocates nothing, so a run of
+    steps on one stream can be captured in a hipGraph once a step has run on that
+    stream (the first reduction on a stream creates its working buffer)."""
+
+    def __init__(self, n=64, slots=100):
+        import torch
+        from .expr import serialize
+        L, ops, s, mask = heat_problem(n)
+        mask.add_consumer(0)
+        self.n, self.layout = n, L
+        self.phi0 = heat_initial(n)
+        self.fields = [Field.from_numpy(L, self.phi0, where="gpu"), Field.from_numpy(L, self.phi0, where="gpu")]
+        self.red = torch.zeros(2 * slots, dtype=torch.float64, device="cuda")
+        self.slots = slots
+        self._plans = []
+        for src in (0, 1):
+            e1, cond_of = heat_step_exprs(ops, self.fields[src], s, mask)
+            dst = self.fields[1 - src]
+            self._plans.append((dst.desc("gpu"), serialize(e1, "gpu"), serialize(cond_of(dst), "gpu"),
+                                serialize(E(dst), "gpu")))
+        self._keep = (ops, mask)
+
+    def reset(self):
+        import torch
+        self.fields[0].gpu.copy_(torch.from_numpy(self.phi0))
+
+    def step(self, i, stream=None):
+        import ctypes as C
+        from . import _abi
+        from .api import _stream
+        lib = _abi.lib()
+        sp = _stream(stream)
+        d, e1, e2, ef = self._plans[i & 1]
+        slot = i % self.slots
+        for rc in (lib.nebo_assign(C.byref(d), e1.ref, 1, sp), lib.nebo_assign(C.byref(d), e2.ref, 1, sp),
+                   lib.nebo_reduce_async(ef.ref, _abi.RED_MAX, 1, C.c_void_p(self.red[2 * slot].data_ptr()), sp),
+                   lib.nebo_reduce_async(ef.ref, _abi.RED_SUM, 1, C.c_void_p(self.red[2 * slot + 1].data_ptr()),
+                                         sp)):
+            _abi.check(rc)
+
+
 # --------------------------------------------------------------------------
 # scalar transport
 # --------------------------------------------------------------------------
@@ -148,7 +196,7 @@ def transport_problem(n, seed_k=2, seed_phi=3, seed_u=7):
     return L, (Lx, Ly, Lz), ops, k, phi, u, dt
 
 
-__all__ = ["heat_initial", "ghost_points", "heat_problem", "heat_step_exprs", "transport_velocity",
+__all__ = ["heat_initial", "ghost_points", "heat_problem", "heat_step_exprs", "HeatStepper", "transport_velocity",
            "upwind_ops", "transport_rhs", "rk3_stage_exprs", "transport_problem", "Field"]
 
 

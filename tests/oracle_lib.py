@@ -59,6 +59,8 @@ _SIGS = {
     "oracle_point_to_field": (None, [_vp, _vp, _vp, C.c_int64]),
     "oracle_field_to_point": (None, [_vp, _vp, _vp, C.c_int64]),
     "oracle_splitmix64": (C.c_uint64, [_P(C.c_uint64)]),
+    "oracle_inv_erf": (C.c_double, [C.c_double]),
+    "oracle_inv_erfc": (C.c_double, [C.c_double]),
     "oracle_fill_uniform": (None, [_vp, C.c_size_t, C.c_uint64, C.c_double, C.c_double]),
     "oracle_fill_normal": (None, [_vp, C.c_size_t, C.c_uint64]),
 }
@@ -289,3 +291,31 @@ def transport_backward_cpu(L, lamT, k, us, coefs, dt, states):
         transport_adjoint_stage(out, None, gk, mu1, acc2, hf(phin), k, us, coefs, dt, 1)
         lam = out.host
     return lam, gk
+
+
+def ulp_distance(a, b):
+    """Representable doubles between a and b, elementwise (the count
+    boost::math::float_distance gives, which the reference's ulp comparisons in
+    spatialops/structured/FieldComparisons.h use): 0 for equal values (and for two
+    NaNs), inf when exactly one is NaN."""
+    a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+    b = np.atleast_1d(np.asarray(b, dtype=np.float64))
+    ia, ib = a.view(np.int64), b.view(np.int64)
+    lowest = np.int64(-2 ** 63)
+    with np.errstate(over="ignore"):
+        oa = np.where(ia < 0, lowest - ia, ia)
+        ob = np.where(ib < 0, lowest - ib, ib)
+        same_side = (oa >= 0) == (ob >= 0)
+        d = np.where(same_side, np.abs(oa - ob).astype(np.float64),
+                     np.abs(oa.astype(np.float64) - ob.astype(np.float64)))
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.where(na & nb, 0.0, np.where(na | nb, np.inf, d))
+
+
+def golden_special(name):
+    """(inputs, correctly rounded outputs) of tests/golden/special_functions.json."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "special_functions.json")
+    pairs = json.load(open(path))[name]
+    return (np.array([float.fromhex(x) for x, _ in pairs]), np.array([float.fromhex(v) for _, v in pairs]))

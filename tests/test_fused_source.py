@@ -33,7 +33,7 @@ def rtc():
 
 
 READELF = "/opt/rocm/llvm/bin/llvm-readelf"
-ARGS_BYTES = 1280   # sizeof(JitArgs) in csrc/nebo_jit.hip
+ARGS_BYTES = 1296   # sizeof(JitArgs) in csrc/nebo_jit.hip
 
 
 def compile_src(rtc, src, want_code=False):
@@ -99,7 +99,7 @@ def random_tree(rng, leaves, stencils, masks, depth):
                                    "GT", "GE", "AND", "OR"]), (sub(), sub()))
     if k < 0.85:
         u = rng.choice(["NEG", "ABS", "SQRT", "SQUARE", "CUBE", "EXP", "LOG", "LOG10", "SIN", "COS", "TAN",
-                        "TANH", "ERF", "ERFC", "NOT"])
+                        "TANH", "ERF", "ERFC", "INV_ERF", "INV_ERFC", "NOT"])
         return nh.Expr(u, (sub(),))
     if k < 0.9:
         return nh.powint(sub(), rng.choice([2, 3, -1, 5]))

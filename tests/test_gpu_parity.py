@@ -565,14 +565,25 @@ def test_tile_selection_by_box_size():
 
 
 @pytest.mark.parametrize("fn,budget", [("SIN", 1), ("COS", 1), ("TAN", 2), ("EXP", 1), ("TANH", 2),
-                                       ("LOG", 1), ("LOG10", 2), ("ERF", 2), ("ERFC", 4)])
+                                       ("LOG", 1), ("LOG10", 2), ("ERF", 2), ("ERFC", 4), ("INV_ERF", 5),
+                                       ("INV_ERFC", 6)])
 def test_transcendental_ulp_budgets(fn, budget):
-    """test/NeboTest.cpp:497-523 budgets against the host libm (erf/erfc: the
-    reference compares with Boost.Math, unavailable here: parity unpinned)."""
+    """test/NeboTest.cpp:497-523 budgets against the oracle (host libm; erf^-1 and
+    erfc^-1 restated from libm's erf/erfc, within 2 ulp of the exact values; the
+    reference compares erf, erfc and the inverses with Boost.Math, which the
+    correctly rounded vectors of test_special_functions_vs_correctly_rounded stand
+    in for)."""
     n = (40, 30, 20)
     L = nh.Layout.build(n, 0)
     rng = np.random.default_rng(1)
-    x = rng.uniform(0.01, 3.0, L.npts) if fn in ("LOG", "LOG10") else rng.uniform(-3, 3, L.npts)
+    if fn in ("LOG", "LOG10"):
+        x = rng.uniform(0.01, 3.0, L.npts)
+    elif fn == "INV_ERF":
+        x = rng.uniform(-1.0, 1.0, L.npts)
+    elif fn == "INV_ERFC":
+        x = rng.uniform(0.0, 2.0, L.npts)
+    else:
+        x = rng.uniform(-3, 3, L.npts)
     a = both(L, x)
     e = nh.Expr(fn, (nh.E(a),))
     ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
@@ -582,6 +593,68 @@ def test_transcendental_ulp_budgets(fn, budget):
     got = fetch(out)
     ulp = np.abs(got - ref.host) / np.spacing(np.abs(ref.host))
     assert ulp.max() <= budget, (fn, ulp.max())
+
+
+@pytest.mark.parametrize("name,fn,budget", [("erf", "ERF", 2), ("erfc", "ERFC", 4), ("erf_inv", "INV_ERF", 5),
+                                            ("erfc_inv", "INV_ERFC", 6)])
+def test_special_functions_vs_correctly_rounded(name, fn, budget):
+    """erf, erfc, erf^-1, erfc^-1 on the GPU against correctly rounded values
+    (tests/golden/special_functions.json: the reference's sin-pattern inputs of
+    FieldHelper.h:82-85 plus tails and edges), within the budgets the reference
+    holds the GPU to against Boost.Math (test/NeboTest.cpp:519-523); the domain
+    ends give +-inf and points outside NaN."""
+    x, want = orc.golden_special(name)
+    L = nh.Layout.build((len(x), 1, 1), 0)
+    a = both(L, x)
+    out = gpu_empty_like(L)
+    nh.assign(out, nh.Expr(fn, (nh.E(a),)))
+    got = fetch(out)
+    d = orc.ulp_distance(got, want)
+    assert d.max() <= budget, (name, x[d.argmax()], got[d.argmax()], want[d.argmax()], d.max())
+    if fn.startswith("INV"):
+        edges = np.array([1.0, -1.0, 1.5, np.nan] if fn == "INV_ERF" else [0.0, 2.0, -0.5, 2.5])
+        expect = [np.inf, -np.inf, np.nan, np.nan]
+        Le = nh.Layout.build((4, 1, 1), 0)
+        e = gpu_empty_like(Le)
+        nh.assign(e, nh.Expr(fn, (nh.E(both(Le, edges)),)))
+        assert np.array_equal(fetch(e), np.array(expect), equal_nan=True), fn
+
+
+def _field_helper_values(L, start):
+    """spatialops/structured/FieldHelper.h:82-85 over the whole window:
+    sin(start + x + y*X + z*X*Y) with 1-based x, y, z."""
+    X, Y, Z = L.glob
+    z, y, x = np.meshgrid(np.arange(1, Z + 1), np.arange(1, Y + 1), np.arange(1, X + 1), indexing="ij")
+    return np.sin(start + x + y * X + z * X * Y).reshape(-1)
+
+
+def test_pow_two_operands_within_two_ulp():
+    """pow(input1, input2) of test/NeboTest.cpp:509 with its inputs (FieldHelper.h
+    sin patterns, start 0 and total): within 2 ulp of the host pow, NaN exactly
+    where the host gives NaN (negative bases with non-integer exponents), as
+    run_test_ignore_nan_ulp compares (NeboTest.cpp:632-637); plus integer
+    exponents, zero and infinite operands."""
+    L = nh.Layout.build((30, 20, 10), 1)
+    total = L.npts
+    a = both(L, _field_helper_values(L, 0.0))
+    b = both(L, _field_helper_values(L, float(total)) * 4.0)
+    out = gpu_empty_like(L)
+    nh.assign(out, nh.pow_(a, b))
+    ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+    orc.assign(ref, nh.pow_(a, b))
+    got = fetch(out)
+    nan = np.isnan(ref.host)
+    assert np.array_equal(np.isnan(got), nan) and nan.any() and (~nan).sum() > L.npts // 3
+    d = orc.ulp_distance(got[~nan], ref.host[~nan])
+    assert d.max() <= 2, d.max()
+    Ls = nh.Layout.build((8, 1, 1), 0)
+    base = both(Ls, np.array([-2.0, -2.0, 0.0, 0.0, np.inf, 0.5, -np.inf, 1.0]))
+    expo = both(Ls, np.array([3.0, -1.0, -1.0, 0.0, -2.0, np.inf, 3.0, np.nan]))
+    o = gpu_empty_like(Ls)
+    nh.assign(o, nh.pow_(base, expo))
+    r = nh.Field.from_numpy(Ls, np.zeros(8), where="cpu")
+    orc.assign(r, nh.pow_(base, expo))
+    assert np.array_equal(fetch(o), r.host, equal_nan=True)
 
 
 # --------------------------------------------------------------------------

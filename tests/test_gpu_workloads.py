@@ -83,3 +83,38 @@ def test_transport_velocity_is_divergence_free():
     torch.cuda.synchronize()
     inner = d.gpu.cpu().numpy().reshape(L.glob[::-1])[1:-1, 1:-1, 1:-1]
     assert np.abs(inner).max() < 1e-9 * n
+
+
+def test_heat_step_graph_replays_match_eager():
+    """Ten heat steps (Laplacian + Euler, cond mask, max and sum into device
+    memory) captured in a hipGraph once the stream has run a step, replayed three
+    times with the same steps run eagerly on other fields and host
+    synchronisations between the replays: after every replay the field and the
+    max/sum values equal the eager run's bit for bit."""
+    per = 10
+    H = W.HeatStepper(64, slots=per)
+    Eg = W.HeatStepper(64, slots=per)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        H.step(0, stream)
+        H.step(1, stream)
+        H.reset()
+        stream.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        for i in range(per):
+            H.step(i, stream)
+    with torch.cuda.stream(stream):
+        H.reset()
+        Eg.reset()
+        stream.synchronize()
+        for rep in range(3):
+            print(f"heat graph replay {rep}", flush=True)
+            graph.replay()
+            for i in range(per):
+                Eg.step(i, stream)
+            stream.synchronize()
+            assert torch.equal(H.fields[0].gpu, Eg.fields[0].gpu), rep
+            assert torch.equal(H.red, Eg.red), rep
+    mx = H.red[0::2].cpu().numpy()
+    assert np.all(np.diff(mx) <= 0)
