@@ -151,6 +151,19 @@ typedef struct nebo_limits { int32_t lo[3]; int32_t hi[3]; } nebo_limits_t;
 /* Last error message of the calling thread ("" when none). */
 const char* nebo_last_error(void);
 
+/* Backend reporting, the reference's NEBO_REPORT_BACKEND messages
+ * (spatialops/NeboLhs.h:167-170, 279-282, 321-331, 407, 475): when on, every
+ * assignment, masked assignment and reduction prints "Starting Nebo HIP
+ * <kernel>" before its launch and "Finished Nebo HIP <kernel>" once it is queued.
+ * The reference switches the messages on at compile time; here the
+ * NEBO_REPORT_BACKEND environment variable (any value but "0") or this call
+ * switches them at run time. */
+int nebo_set_report_backend(int on);
+
+/* The kernel the calling thread's last assignment / reduction ran (e.g.
+ * "div(k grad phi) kernel, tile 128x8x64 ..."; "" before any). */
+const char* nebo_last_backend(void);
+
 /* Number of visible GPUs (CUDADeviceInterface::get_device_count,
  * spatialops/structured/ExternalAllocators.h:59-106). */
 int nebo_device_count(int* count);

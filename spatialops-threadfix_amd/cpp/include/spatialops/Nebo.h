@@ -317,27 +317,59 @@ class Flat {
 // ---------------------------------------------------------------------------
 // expression handles
 // ---------------------------------------------------------------------------
-template <typename FieldT>
+namespace nebo_detail {
+// What an expression handle carries: the lowered tree of the expression.  The
+// reference's handles carry an expression-template object whose type encodes
+// the tree; here the tree is data, so every expression has this one operand
+// type and is evaluated by the engine behind the C-ABI.
+class Lowered {
+ public:
+  explicit Lowered(NodeP n) : node_(std::move(n)) {}
+  const NodeP& node() const { return node_; }
+
+ private:
+  NodeP node_;
+};
+}  // namespace nebo_detail
+
+// NeboExpression<Operand, FieldType> / NeboBooleanExpression<Operand, FieldType>
+// with the reference's typedefs and expr() (NeboBasic.h:193-243).
+template <typename Operand, typename FieldType>
 class NeboExpression {
  public:
-  typedef FieldT field_type;
-  explicit NeboExpression(nebo_detail::NodeP n) : node_(std::move(n)) {}
-  const nebo_detail::NodeP& node() const { return node_; }
+  typedef FieldType field_type;
+  typedef Operand Expression;
+  explicit NeboExpression(const Operand& given) : expr_(given) {}
+  template <typename O = Operand,
+            typename = typename std::enable_if<std::is_same<O, nebo_detail::Lowered>::value>::type>
+  explicit NeboExpression(nebo_detail::NodeP n) : expr_(std::move(n)) {}
+  const Operand& expr() const { return expr_; }
+  const nebo_detail::NodeP& node() const { return expr_.node(); }
 
  private:
-  nebo_detail::NodeP node_;
+  Operand expr_;
 };
 
-template <typename FieldT>
+template <typename Operand, typename FieldType>
 class NeboBooleanExpression {
  public:
-  typedef FieldT field_type;
-  explicit NeboBooleanExpression(nebo_detail::NodeP n) : node_(std::move(n)) {}
-  const nebo_detail::NodeP& node() const { return node_; }
+  typedef FieldType field_type;
+  typedef Operand Expression;
+  explicit NeboBooleanExpression(const Operand& given) : expr_(given) {}
+  template <typename O = Operand,
+            typename = typename std::enable_if<std::is_same<O, nebo_detail::Lowered>::value>::type>
+  explicit NeboBooleanExpression(nebo_detail::NodeP n) : expr_(std::move(n)) {}
+  const Operand& expr() const { return expr_; }
+  const nebo_detail::NodeP& node() const { return expr_.node(); }
 
  private:
-  nebo_detail::NodeP node_;
+  Operand expr_;
 };
+
+namespace nebo_detail {
+template <typename FieldT> using Expr = NeboExpression<Lowered, FieldT>;
+template <typename FieldT> using BoolExpr = NeboBooleanExpression<Lowered, FieldT>;
+}  // namespace nebo_detail
 
 namespace nebo_detail {
 
@@ -355,17 +387,17 @@ struct Operand<SpatialField<L, T>> {
   typedef SpatialField<L, T> field;
   static NodeP node(const SpatialField<L, T>& f) { return field_leaf(f); }
 };
-template <typename F>
-struct Operand<NeboExpression<F>> {
+template <typename R, typename F>
+struct Operand<NeboExpression<R, F>> {
   static constexpr bool ok = true, boolean = false;
   typedef F field;
-  static NodeP node(const NeboExpression<F>& e) { return e.node(); }
+  static NodeP node(const NeboExpression<R, F>& e) { return e.node(); }
 };
-template <typename F>
-struct Operand<NeboBooleanExpression<F>> {
+template <typename R, typename F>
+struct Operand<NeboBooleanExpression<R, F>> {
   static constexpr bool ok = true, boolean = true;
   typedef F field;
-  static NodeP node(const NeboBooleanExpression<F>& e) { return e.node(); }
+  static NodeP node(const NeboBooleanExpression<R, F>& e) { return e.node(); }
 };
 template <typename F>
 struct Operand<SpatialMask<F>> {
@@ -387,22 +419,22 @@ template <typename A, typename B>
 using ArithOK = typename std::enable_if<
     Operand<A>::ok && Operand<B>::ok && !Operand<A>::boolean && !Operand<B>::boolean &&
         !(std::is_arithmetic<A>::value && std::is_arithmetic<B>::value),
-    NeboExpression<typename Common<A, B>::type>>::type;
+    nebo_detail::Expr<typename Common<A, B>::type>>::type;
 
 template <typename A, typename B>
 using CmpOK = typename std::enable_if<
     Operand<A>::ok && Operand<B>::ok && !Operand<A>::boolean && !Operand<B>::boolean &&
         !(std::is_arithmetic<A>::value && std::is_arithmetic<B>::value),
-    NeboBooleanExpression<typename Common<A, B>::type>>::type;
+    nebo_detail::BoolExpr<typename Common<A, B>::type>>::type;
 
 template <typename A, typename B>
 using LogicOK = typename std::enable_if<Operand<A>::ok && Operand<B>::ok && Operand<A>::boolean &&
                                             Operand<B>::boolean,
-                                        NeboBooleanExpression<typename Common<A, B>::type>>::type;
+                                        nebo_detail::BoolExpr<typename Common<A, B>::type>>::type;
 
 template <typename A>
 using UnaryOK = typename std::enable_if<Operand<A>::ok && !Operand<A>::boolean && !std::is_arithmetic<A>::value,
-                                        NeboExpression<typename Operand<A>::field>>::type;
+                                        nebo_detail::Expr<typename Operand<A>::field>>::type;
 
 }  // namespace nebo_detail
 
@@ -447,14 +479,14 @@ nebo_detail::LogicOK<A, B> operator||(const A& a, const B& b)
       nebo_detail::op2(NEBO_OR, nebo_detail::Operand<A>::node(a), nebo_detail::Operand<B>::node(b)));
 }
 template <typename F>
-NeboBooleanExpression<F> operator!(const NeboBooleanExpression<F>& a)
+nebo_detail::BoolExpr<F> operator!(const nebo_detail::BoolExpr<F>& a)
 {
-  return NeboBooleanExpression<F>(nebo_detail::op1(NEBO_NOT, a.node()));
+  return nebo_detail::BoolExpr<F>(nebo_detail::op1(NEBO_NOT, a.node()));
 }
 template <typename F>
-NeboBooleanExpression<F> operator!(const SpatialMask<F>& m)
+nebo_detail::BoolExpr<F> operator!(const SpatialMask<F>& m)
 {
-  return NeboBooleanExpression<F>(nebo_detail::op1(NEBO_NOT, m.node()));
+  return nebo_detail::BoolExpr<F>(nebo_detail::op1(NEBO_NOT, m.node()));
 }
 
 template <typename A>
@@ -526,12 +558,12 @@ class CondBuilder {
     return CondBuilder<FieldT>(std::move(c));
   }
   template <typename Value>
-  NeboExpression<FieldT> operator()(const Value& v) const
+  nebo_detail::Expr<FieldT> operator()(const Value& v) const
   {
     nebo_detail::NodeP n = nebo_detail::Operand<Value>::node(v);
     for (auto it = clauses_.rbegin(); it != clauses_.rend(); ++it)
       n = nebo_detail::op3(NEBO_COND, it->first, it->second, n);
-    return NeboExpression<FieldT>(n);
+    return nebo_detail::Expr<FieldT>(n);
   }
 
  private:
@@ -580,8 +612,11 @@ short device_of(const X& x)
 
 }  // namespace nebo_detail
 
+// lhs <<= scalar | field | expression (NeboAssignment.h:29-133): returns the
+// assigned field as FieldType const&, as the reference does.
 template <typename L, typename T, typename R>
-typename std::enable_if<nebo_detail::Operand<R>::ok && !nebo_detail::Operand<R>::boolean, SpatialField<L, T>&>::type
+typename std::enable_if<nebo_detail::Operand<R>::ok && !nebo_detail::Operand<R>::boolean,
+                        const SpatialField<L, T>&>::type
 operator<<=(SpatialField<L, T>& lhs, const R& rhs)
 {
   typedef typename nebo_detail::Operand<R>::field FR;
@@ -592,7 +627,7 @@ operator<<=(SpatialField<L, T>& lhs, const R& rhs)
 }
 
 template <typename L, typename T, typename R>
-SpatialField<L, T>& interior_assign(SpatialField<L, T>& lhs, const R& rhs)
+const SpatialField<L, T>& interior_assign(SpatialField<L, T>& lhs, const R& rhs)
 {
   nebo_detail::assign(lhs, nebo_detail::Operand<R>::node(rhs), false);
   return lhs;
@@ -614,7 +649,7 @@ void assign_points(F& lhs, const std::vector<int32_t>& pts, const NodeP& rhs)
 }  // namespace nebo_detail
 
 template <typename F, typename R>
-F& masked_assign(const SpatialMask<F>& mask, F& lhs, const R& rhs)
+const F& masked_assign(const SpatialMask<F>& mask, F& lhs, const R& rhs)
 {
   nebo_detail::assign_points(lhs, mask.flat_points(), nebo_detail::Operand<R>::node(rhs));
   return lhs;

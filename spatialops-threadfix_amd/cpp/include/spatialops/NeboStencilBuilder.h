@@ -100,13 +100,13 @@ class NeboStencilBuilder {
                                "trying to access a coefficient that does not exist");
     spec_ = s;
   }
-  NeboExpression<DestT> operator()(const SrcT& src) const
+  nebo_detail::Expr<DestT> operator()(const SrcT& src) const
   {
-    return NeboExpression<DestT>(nebo_detail::stencil(spec_, nebo_detail::field_leaf(src)));
+    return nebo_detail::Expr<DestT>(nebo_detail::stencil(spec_, nebo_detail::field_leaf(src)));
   }
-  NeboExpression<DestT> operator()(const NeboExpression<SrcT>& src) const
+  nebo_detail::Expr<DestT> operator()(const nebo_detail::Expr<SrcT>& src) const
   {
-    return NeboExpression<DestT>(nebo_detail::stencil(spec_, src.node()));
+    return nebo_detail::Expr<DestT>(nebo_detail::stencil(spec_, src.node()));
   }
   void apply_to_field(const SrcT& src, DestT& dest) const { dest <<= (*this)(src); }
   const std::vector<double>& coefs() const { return spec_->coefs; }
@@ -129,16 +129,16 @@ class NeboSumStencilBuilder {
     s->pts = std::move(pts);
     spec_ = s;
   }
-  NeboExpression<DestT> operator()(const SrcT& src) const { return wrap(nebo_detail::field_leaf(src)); }
-  NeboExpression<DestT> operator()(const NeboExpression<SrcT>& src) const { return wrap(src.node()); }
+  nebo_detail::Expr<DestT> operator()(const SrcT& src) const { return wrap(nebo_detail::field_leaf(src)); }
+  nebo_detail::Expr<DestT> operator()(const nebo_detail::Expr<SrcT>& src) const { return wrap(src.node()); }
   void apply_to_field(const SrcT& src, DestT& dest) const { dest <<= (*this)(src); }
 
  private:
-  NeboExpression<DestT> wrap(nebo_detail::NodeP arg) const
+  nebo_detail::Expr<DestT> wrap(nebo_detail::NodeP arg) const
   {
     nebo_detail::NodeP n = nebo_detail::stencil(spec_, std::move(arg));
     if (scale_ != 0.0) n = nebo_detail::op2(NEBO_MUL, n, nebo_detail::scalar(scale_));
-    return NeboExpression<DestT>(n);
+    return nebo_detail::Expr<DestT>(n);
   }
   std::shared_ptr<const nebo_detail::StencilSpec> spec_;
   double scale_;
@@ -410,13 +410,13 @@ struct NeboMaskShiftBuilder {
   {
     return nebo_detail::MaskShiftPoints<typename OperatorT::type, SrcFieldT, DestFieldT>::get(false);
   }
-  NeboBooleanExpression<DestFieldT> minus(const SpatialMask<SrcFieldT>& src) const
+  nebo_detail::BoolExpr<DestFieldT> minus(const SpatialMask<SrcFieldT>& src) const
   {
-    return NeboBooleanExpression<DestFieldT>(nebo_detail::mask_shift(src.node(), minus_point()));
+    return nebo_detail::BoolExpr<DestFieldT>(nebo_detail::mask_shift(src.node(), minus_point()));
   }
-  NeboBooleanExpression<DestFieldT> plus(const SpatialMask<SrcFieldT>& src) const
+  nebo_detail::BoolExpr<DestFieldT> plus(const SpatialMask<SrcFieldT>& src) const
   {
-    return NeboBooleanExpression<DestFieldT>(nebo_detail::mask_shift(src.node(), plus_point()));
+    return nebo_detail::BoolExpr<DestFieldT>(nebo_detail::mask_shift(src.node(), plus_point()));
   }
 };
 
@@ -459,7 +459,7 @@ struct NeboBoundaryConditionBuilder {
   }
 
   void operator()(const SpatialMask<GammaFieldType>& mask, PhiFieldType& phi,
-                  const NeboExpression<GammaFieldType>& gamma, bool minus) const
+                  const nebo_detail::Expr<GammaFieldType>& gamma, bool minus) const
   {
     apply(mask, phi, gamma.node(), minus);
   }
@@ -475,13 +475,13 @@ struct NeboBoundaryConditionBuilder {
   }
 
   // The inverted value at phi points, (gamma term - phi term) / coefficient.
-  NeboExpression<PhiFieldType> rhs(const PhiFieldType& phi, const nebo_detail::NodeP& gamma,
+  nebo_detail::Expr<PhiFieldType> rhs(const PhiFieldType& phi, const nebo_detail::NodeP& gamma,
                                    bool minus) const
   {
     using namespace nebo_detail;
     NodeP g = stencil(minus ? minusGamma_ : plusGamma_, gamma);
     NodeP f = stencil(minus ? minusPhi_ : plusPhi_, field_leaf(phi));
-    return NeboExpression<PhiFieldType>(
+    return nebo_detail::Expr<PhiFieldType>(
         op2(NEBO_DIV, op2(NEBO_SUB, g, f), scalar(minus ? lowCoef_ : highCoef_)));
   }
 

@@ -579,6 +579,7 @@ int launch_generic_assign(const nebo_field_t* lhs, const VmProgram& prog, int ma
   const int oX = lhs->glob[0], oXY = lhs->glob[0] * lhs->glob[1];
   hipStream_t s = (hipStream_t)stream;
   const bool one = prog.n_classes == 1, math = uses_math(prog);
+  BackendScope bs("stack-program interpreter (" + std::to_string(prog.n_insns) + " instructions)");
   if (one && !math)
     hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
   else if (one)
@@ -662,6 +663,7 @@ int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, doubl
   int rc = jit_reduce(prog, lim, op, buf.partial, kRedMaxBlocks, buf.ticket, out_dev, compile, s, &done);
   if (rc || done) return rc;
   const bool one = prog.n_classes == 1, math = uses_math(prog);
+  BackendScope bs("stack-program interpreter reduction (" + std::to_string(blocks) + " blocks, last block folds)");
   if (one && !math) launch_reduce_any<1, false>(prog, B, op, blocks, buf, out_dev, s);
   else if (one) launch_reduce_any<1, true>(prog, B, op, blocks, buf, out_dev, s);
   else if (!math) launch_reduce_any<kVmMaxClasses, false>(prog, B, op, blocks, buf, out_dev, s);
@@ -813,6 +815,7 @@ extern "C" int nebo_masked_assign(const nebo_field_t* lhs, const int32_t* points
   if ((rc = check_hip(hipMemcpyAsync(pts, points, bytes, hipMemcpyHostToDevice, s), ctx))) return rc;
   dim3 grid((n_points + kBlock - 1) / kBlock);
   const bool one = prog.n_classes == 1, math = uses_math(prog);
+  BackendScope bs("masked assignment (" + std::to_string(n_points) + " points)");
   if (one && !math)
     hipLaunchKernelGGL((vm_masked_kernel<1, false>), grid, dim3(kBlock), 0, s, prog, pts, n_points, out, oX, oXY);
   else if (one)

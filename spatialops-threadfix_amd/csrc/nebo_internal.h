@@ -15,6 +15,18 @@ namespace nebo {
 int set_error(int code, const char* ctx, const std::string& msg);
 void clear_error();
 
+// Names the kernel that evaluates an assignment or reduction: recorded for
+// nebo_last_backend (per thread) and, when reporting is on, printed as
+// "Starting Nebo HIP <name>" now and "Finished Nebo HIP <name>" when the scope
+// ends (the kernel is then queued).
+bool report_backend();
+struct BackendScope {
+  explicit BackendScope(const std::string& name);
+  ~BackendScope();
+  BackendScope(const BackendScope&) = delete;
+  BackendScope& operator=(const BackendScope&) = delete;
+};
+
 // ---------------------------------------------------------------------------
 // ghost algebra (GhostData.h:54-257): six counts; all >= NEBO_GHOST_MAX = infinite
 // ---------------------------------------------------------------------------

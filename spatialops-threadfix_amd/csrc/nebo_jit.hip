@@ -919,6 +919,8 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
     if (total >= (1L << 31))
       return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
     A.nblocks = (int)total;
+    BackendScope bs("compiled kernel, column schedule (" + std::to_string(cs.waves) + " waves per block, " +
+                    std::to_string(cp.RY) + " rows per lane, " + std::to_string(cs.zc) + " planes per block)");
     if ((rc = launch(e, &A, (unsigned)total, 1, (unsigned)(64 * cs.waves), stream))) return rc;
     *done = true;
     return NEBO_OK;
@@ -942,6 +944,8 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
     return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
   A.ntx = (int)ntx;
   A.nblocks = (int)total;
+  BackendScope bs("compiled kernel, " + std::to_string(V) + " z cells per thread (" + std::to_string(sh.block) +
+                  "-thread blocks)");
   if ((rc = launch(e, &A, (unsigned)total, 1, (unsigned)sh.block, stream))) return rc;
   *done = true;
   return NEBO_OK;
@@ -967,6 +971,7 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
   if (nb > max_blocks) nb = max_blocks;
   if (nb < 1) nb = 1;
   A.nblocks = (int)nb;
+  BackendScope bs("compiled reduction kernel (" + std::to_string(nb) + " blocks, last block folds)");
   if ((rc = launch(e, &A, (unsigned)nb, 1, kReduceBlock, stream))) return rc;
   *done = true;
   return NEBO_OK;

@@ -8,6 +8,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -29,6 +32,47 @@ int set_error(int code, const char* ctx, const std::string& msg)
 }
 
 void clear_error() { t_error.clear(); }
+
+// ---- backend reporting (the reference's NEBO_REPORT_BACKEND, NeboLhs.h:167-170,
+// 321-331, 407, 475: "Starting Nebo <backend>" / "Finished Nebo <backend>").  The
+// reference compiles the messages in with a macro; this library is prebuilt, so
+// they are switched on at run time, by the NEBO_REPORT_BACKEND environment
+// variable (any value but "0") or nebo_set_report_backend.  The launches are
+// asynchronous: "Finished" means the kernel is queued on its stream.
+namespace {
+thread_local std::string t_backend;
+std::atomic<int> g_report{-1};   // -1: environment not read yet
+}
+
+bool report_backend()
+{
+  int r = g_report.load();
+  if (r < 0) {
+    const char* v = std::getenv("NEBO_REPORT_BACKEND");
+    r = (v && *v && std::string(v) != "0") ? 1 : 0;
+    int expected = -1;
+    g_report.compare_exchange_strong(expected, r);
+    r = g_report.load();
+  }
+  return r == 1;
+}
+
+BackendScope::BackendScope(const std::string& name)
+{
+  t_backend = name;
+  if (report_backend()) {
+    std::printf("Starting Nebo HIP %s\n", name.c_str());
+    std::fflush(stdout);
+  }
+}
+
+BackendScope::~BackendScope()
+{
+  if (report_backend()) {
+    std::printf("Finished Nebo HIP %s\n", t_backend.c_str());
+    std::fflush(stdout);
+  }
+}
 
 int check_hip(int st, const char* ctx)
 {
@@ -67,6 +111,15 @@ int with_device(int device, const char* ctx, F f)
 using namespace nebo;
 
 extern "C" const char* nebo_last_error(void) { return t_error.c_str(); }
+
+extern "C" int nebo_set_report_backend(int on)
+{
+  clear_error();
+  g_report.store(on ? 1 : 0);
+  return NEBO_OK;
+}
+
+extern "C" const char* nebo_last_backend(void) { return t_backend.c_str(); }
 
 extern "C" int nebo_device_count(int* count)
 {

@@ -724,6 +724,35 @@ int usable_shape(int shape, int X, int XY)
   return shape;
 }
 
+// "div(k grad phi) kernel, tile 128x8x64, column schedule, 4 rows per lane, 2x2
+// waves, loads 3 planes ahead" (built once per kernel and tile: launches of a
+// time loop only copy it)
+std::string tile_backend_text(const char* kernel, int shape)
+{
+  const TileShape sh = kShapes[shape];
+  std::ostringstream o;
+  o << kernel << ", tile " << sh.tx << "x" << sh.ty << "x" << sh.zc;
+  if (sh.variant == 0) o << ", direct loads";
+  else if (sh.variant <= 2) o << ", LDS-staged rows, " << sh.variant << " planes of prefetch";
+  else
+    o << ", column schedule, " << rows_per_lane(sh) << " rows per lane, " << sh.tx / kTx << "x"
+      << stacked_waves(sh) << " waves, loads " << prefetch_planes(sh) << " planes ahead";
+  return o.str();
+}
+
+const std::string& tile_backend(int kind, int shape)
+{
+  static const char* names[3] = {"div(k grad phi) kernel", "diffusion Euler step kernel",
+                                 "Laplacian + Euler kernel"};
+  static const std::vector<std::string>* table = [] {
+    auto* t = new std::vector<std::string>;
+    for (int k = 0; k < 3; ++k)
+      for (int sh = 0; sh < kNumShapes; ++sh) t->push_back(tile_backend_text(names[k], sh));
+    return t;
+  }();
+  return (*table)[kind * kNumShapes + shape];
+}
+
 int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_field_t* k,
                       const nebo_field_t* phi, const nebo_axis_coefs_t& c, double dt,
                       const nebo_limits_t& lim, void* stream)
@@ -734,6 +763,7 @@ int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_fie
   Tiles T = make_tiles(lim, sh.tx, geo.tile_y, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
+  BackendScope bs(tile_backend(form == 2 ? 1 : 0, shape));
   hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(geo.threads), 0, (hipStream_t)stream,
                      frame_origin(r), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
   return check_hip(hipGetLastError(), "Nebo Assignment");
@@ -748,6 +778,7 @@ int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* ph
   Tiles T = make_tiles(lim, sh.tx, geo.tile_y, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
+  BackendScope bs(tile_backend(2, shape));
   hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(geo.threads), 0, (hipStream_t)stream,
                      frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
   return check_hip(hipGetLastError(), "Nebo Assignment");

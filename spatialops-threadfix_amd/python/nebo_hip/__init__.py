@@ -15,7 +15,7 @@ from .api import (lower_stats, assign, interior_assign, masked_assign, assign_po
                   fill_ghosts_periodic, divkgrad, lapl_euler, device_count, stencil_tiles,
                   stencil_set_tile, stencil_get_tile, stencil_reset_tile, stencil_select_tile,
                   stencil_autotune, KERNEL_DIVKGRAD, KERNEL_LAPL,
-                  set_fused_jit, fused_jit_stats, fused_jit_tune, fused_jit_column, fused_source)
+                  set_fused_jit, set_report_backend, last_backend, fused_jit_stats, fused_jit_tune, fused_jit_column, fused_source)
 from .bc import MaskShift, BoundaryCondition, shift_points  # noqa: F401,E402
 from .batch import Batch, alloc_patches  # noqa: F401,E402
 from . import adjoint  # noqa: F401,E402

@@ -124,6 +124,8 @@ P = C.POINTER
 _vp = C.c_void_p
 _SIGS = {
     "nebo_last_error": (C.c_char_p, []),
+    "nebo_set_report_backend": (C.c_int, [C.c_int]),
+    "nebo_last_backend": (C.c_char_p, []),
     "nebo_device_count": (C.c_int, [P(C.c_int)]),
     "nebo_init": (C.c_int, [C.c_int]),
     "nebo_alloc": (C.c_int, [C.c_int, C.c_size_t, P(_vp)]),

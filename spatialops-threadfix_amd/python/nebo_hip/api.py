@@ -107,6 +107,18 @@ def lapl_euler(out: Field, phi: Field, coefs, s, stream=None):
     return out
 
 
+def set_report_backend(on: bool):
+    """Print "Starting Nebo HIP <kernel>" / "Finished Nebo HIP <kernel>" around
+    every assignment and reduction (the reference's NEBO_REPORT_BACKEND; the
+    NEBO_REPORT_BACKEND environment variable switches it on too)."""
+    _abi.call("nebo_set_report_backend", int(bool(on)))
+
+
+def last_backend() -> str:
+    """The kernel the calling thread's last assignment or reduction ran."""
+    return _abi.lib().nebo_last_backend().decode()
+
+
 def set_fused_jit(enable: bool):
     """Run generic trees as kernels compiled at run time (True, the default) or
     on the stack-program interpreter (False); both give the same bits."""

@@ -185,3 +185,12 @@ def test_backward_checkpoint_interval_checked_on_entry():
     assert rc == _abi.NEBO_ERR_TOO_LARGE
     assert "checkpoint interval" in nh.lib().nebo_last_error().decode()
     assert np.all(gk == 3.0)
+
+
+def test_backend_reporting_switch():
+    """nebo_set_report_backend / nebo_last_backend exist and the switch is
+    accepted both ways (the names themselves are checked on the GPU)."""
+    lib = nh.lib()
+    assert lib.nebo_set_report_backend(1) == 0
+    assert lib.nebo_set_report_backend(0) == 0
+    assert isinstance(lib.nebo_last_backend(), bytes)

@@ -57,6 +57,49 @@ void f(SVolField& r, const SVolField& a, const SSurfXField& b) { r <<= a + b; }
 """
 
 
+SIGNATURES = r"""
+#include <spatialops/structured/FVStaggered.h>
+#include <type_traits>
+using namespace SpatialOps;
+// user code written against the reference's signatures (NeboAssignment.h:29-133,
+// NeboBasic.h:193-243): it names NeboExpression<RhsType, FieldType> and
+// NeboBooleanExpression<RhsType, FieldType> and binds FieldType const& results
+template <typename RhsType, typename FieldType>
+FieldType const& assign_expr(FieldType& lhs, const NeboExpression<RhsType, FieldType>& rhs)
+{
+  return lhs <<= rhs;
+}
+template <typename RhsType, typename FieldType>
+FieldType const& select(FieldType& out, const NeboBooleanExpression<RhsType, FieldType>& test)
+{
+  return out <<= cond(test, 1.0)(0.0);
+}
+void f(SVolField& r, const SVolField& a, const SVolField& b, const SpatialMask<SVolField>& m)
+{
+  SVolField const& x = (r <<= a + b);
+  SVolField const& y = assign_expr(r, a * 2.0);
+  SVolField const& z = interior_assign(r, a - b);
+  SVolField const& w = masked_assign(m, r, 3.0);
+  SVolField const& v = select(r, a > b);
+  auto e = a * b + 1.0;
+  static_assert(std::is_same<decltype(e)::field_type, SVolField>::value, "field_type");
+  typedef decltype(e)::Expression RhsT;
+  NeboExpression<RhsT, SVolField> copy(e.expr());
+  r <<= copy;
+  r <<= 2.0;
+  r <<= a;
+  (void)x; (void)y; (void)z; (void)w; (void)v;
+}
+"""
+
+
+def test_reference_signatures_compile(tmp_path):
+    """Reference-style user code that names the two-parameter expression types and
+    binds the FieldType const& that <<=, interior_assign and masked_assign return."""
+    r = _compile(SIGNATURES, tmp_path)
+    assert r.returncode == 0, r.stderr
+
+
 def test_front_end_compiles(tmp_path):
     r = _compile(GOOD, tmp_path)
     assert r.returncode == 0, r.stderr

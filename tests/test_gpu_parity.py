@@ -740,3 +740,39 @@ def test_errors_on_gpu():
         assert ei.value.code == _abi.NEBO_ERR_INVALID and "unbounded" in str(ei.value)
     assert nh.field_max_interior(c) == -math.inf and nh.field_min_interior(c) == math.inf
     assert nh.field_sum_interior(c) == 0.0 and nh.field_norm_interior(c) == 0.0
+
+
+def test_backend_reported_per_assignment(capfd):
+    """NEBO_REPORT_BACKEND: each assignment and reduction names the kernel it ran
+    (hand-written tile, compiled kernel, interpreter, masked assignment) and,
+    when reporting is on, prints the reference's Starting/Finished lines."""
+    n = (70, 37, 45)
+    L = nh.Layout.build(n, 1)
+    rng = np.random.default_rng(30)
+    k = both(L, rng.uniform(0.5, 2.0, L.npts))
+    phi = both(L, rng.standard_normal(L.npts))
+    _, ops = _ops(n)
+    out = gpu_empty_like(L)
+    nh.set_report_backend(True)
+    try:
+        nh.assign(out, _divkgrad_tree(ops, k, phi, 0))
+        assert nh.last_backend().startswith("div(k grad phi) kernel, tile ")
+        nh.assign(out, phi * k + 1.0)
+        assert nh.last_backend().startswith("compiled kernel")
+        nh.set_fused_jit(False)
+        nh.assign(out, phi * k + 2.0)
+        assert nh.last_backend().startswith("stack-program interpreter")
+        nh.set_fused_jit(True)
+        nh.field_max(phi * k)
+        assert nh.last_backend().startswith("compiled reduction kernel")
+        m = nh.Mask(L, [(1, 2, 3), (4, 5, 6)]).add_consumer(0)
+        nh.masked_assign(m, out, 3.0)
+        assert nh.last_backend() == "masked assignment (2 points)"
+        torch.cuda.synchronize()
+    finally:
+        nh.set_report_backend(False)
+        nh.set_fused_jit(True)
+    printed = capfd.readouterr().out
+    assert "Starting Nebo HIP div(k grad phi) kernel, tile " in printed
+    assert "Finished Nebo HIP stack-program interpreter" in printed
+    assert printed.count("Starting Nebo HIP") == printed.count("Finished Nebo HIP") >= 5
