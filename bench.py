@@ -47,7 +47,8 @@ def _tile_name(t):
     """tx x ty x zchunk with the schedule: direct loads, LDS-staged + prefetch depth,
     or the column schedule (ty rows per lane)."""
     sched = {0: "", 1: "/staged-pf1", 2: "/staged-pf2", 3: "/column", 4: "/column-xreuse",
-             5: "/stacked-ry4", 6: "/stacked-ry2", 7: "/stacked-ry4-pf3", 8: "/stacked-ry4-pf4"}[t[3]]
+             5: "/stacked-ry4", 6: "/stacked-ry2", 7: "/stacked-ry4-pf3", 8: "/stacked-ry4-pf4",
+             9: "/stacked-ry4-pf3-wb"}[t[3]]
     return f"{t[0]}x{t[1]}x{t[2]}{sched}"
 
 
@@ -225,6 +226,40 @@ def _time_calls(fn, stream, reps, warm=3):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def ceiling_leg(n=256, reps=20):
+    """The streaming ceiling on blocks of the benchmark's size ((n+2)^3 fp64):
+    r = a + s*b by nebo_stream_triad, two read streams and one write stream as
+    div(k grad phi) has, without halos or arithmetic.  Each variant (access width,
+    store policy) timed over `reps` back-to-back launches; GB/s counts 24 B per
+    point."""
+    import ctypes as C
+    import torch
+    import nebo_hip as nh
+    npts = (n + 2) ** 3
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = torch.randn(npts, generator=g, dtype=torch.float64, device="cuda")
+    b = torch.randn(npts, generator=g, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(a)
+    lib = nh.lib()
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+    names = {0: "8B-nt", 1: "16B-nt", 2: "16B-writeback", 3: "2x16B-nt"}
+    rates, ok = {}, True
+    for v in range(4):
+        def call():
+            rc = lib.nebo_stream_triad(C.c_void_p(r.data_ptr()), C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                                       0.5, npts, v, sp)
+            if rc:
+                raise RuntimeError(lib.nebo_last_error().decode())
+        ms = _time_calls(call, stream, reps)
+        rates[names[v]] = round(24 * npts / (ms * 1e-3) / 1e9, 1)
+        ok = ok and bool(torch.equal(r, a + 0.5 * b))
+    best = max(rates, key=rates.get)
+    return {"workload": f"r = a + s*b over 3 blocks of {n + 2}^3 fp64", "bytes_per_launch": 24 * npts,
+            "GBps": rates, "best": best, "best_GBps": rates[best],
+            "best_frac_of_peak": round(rates[best] * 1e9 / HBM_PEAK, 4), "results_correct": ok}
 
 
 def generic_leg(k, phi, ops, reps=20):
@@ -519,6 +554,12 @@ def pmc_probe(n, tile=None):
         dst.copy_(phi.gpu)
     for _ in range(3):
         nh.divkgrad(r, k, phi, c, 0)
+    # the streaming ceiling's kernels on the same blocks (every triad variant twice)
+    import ctypes as C
+    sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for v in (0, 1, 2, 3, 0, 1, 2, 3):
+        nh._abi.check(nh.lib().nebo_stream_triad(C.c_void_p(dst.data_ptr()), C.c_void_p(phi.gpu.data_ptr()),
+                                                 C.c_void_p(k.gpu.data_ptr()), 0.5, L.npts, v, sp))
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": True, "copy_bytes_read": L.npts * 8, "copy_bytes_written": L.npts * 8,
                       "kernel_bytes_algorithmic": BYTES_PER_CELL * n ** 3,
@@ -608,6 +649,7 @@ def main():
         except (OSError, ValueError, AttributeError):
             traffic = None
 
+    ceiling = ceiling_leg(n)
     extras = {}
     if not a.no_extras:
         extras["batched"] = batched_leg()
@@ -632,7 +674,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "bytes_per_launch_algorithmic": BYTES_PER_CELL * cells,
-                     "avg_kernel_ms": round(avg_kernel_ms, 5)},
+                     "avg_kernel_ms": round(avg_kernel_ms, 5),
+                     "hbm_GBps_measured_traffic": (round(traffic / (avg_kernel_ms * 1e-3) / 1e9, 1)
+                                                   if traffic else None),
+                     "stream_ceiling_GBps": ceiling["best_GBps"],
+                     "frac_of_stream_ceiling": round(achieved / 1e9 / ceiling["best_GBps"], 4)},
+        "stream_ceiling": ceiling,
         "cpu_baseline": cpu,
         **extras,
         "wall_s_timed_region": round(wall, 4),

@@ -395,6 +395,17 @@ int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const nebo_field_t*
                           int32_t* tx, int32_t* ty, int32_t* zchunk, int32_t* variant,
                           float* times_ms);
 
+/* Streaming ceiling: r[i] = a[i] + s*b[i] for i < n, the two-read one-write
+ * traffic of div(k grad phi) without halos or arithmetic.  No counterpart in the
+ * reference: a calibration kernel, timed beside the stencil kernels so their HBM
+ * rate is read against what a plain stream reaches on the same blocks.
+ * variant 0: 8-byte accesses, nontemporal stores; 1: 16-byte accesses,
+ * nontemporal stores; 2: 16-byte accesses, write-back stores; 3: two 16-byte
+ * accesses per lane and array, nontemporal stores (1-3 need 16-byte aligned
+ * arrays). */
+int nebo_stream_triad(double* r, const double* a, const double* b, double s, int64_t n, int32_t variant,
+                      void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* batched patches: one persistent launch over a device patch table          */
 /* ------------------------------------------------------------------------- */

@@ -160,6 +160,15 @@ __device__ __forceinline__ void store_nt_at(Rsrc rs, uint32_t off, double v)
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, 2);
 }
 
+// store with cache policy SP: 2 nontemporal, 0 write-back (the line stays in L2
+// until evicted, so neighbouring waves' parts of one line are merged there)
+template <int SP>
+__device__ __forceinline__ void store_at(Rsrc rs, uint32_t off, double v)
+{
+  static_assert(SP == 0 || SP == 2, "store policy");
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, SP);
+}
+
 // Run f(integral_constant<int, I>) for I = 0, 1, ... in order (compile-time steps).
 template <typename F, int... I>
 __device__ __forceinline__ void unroll_steps(F&& f, std::integer_sequence<int, I...>)

@@ -120,13 +120,13 @@ int device_cu_count(int device);
 // compilation is off or failed for this program (the interpreter then runs it).
 bool jit_enabled();
 int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done);
-// Reduction: one launch whose blocks write their (hi, lo) partials into
-// partial (room for max_blocks) and whose last block folds them into *out and
-// resets *ticket to 0.  Sets *done, or leaves it false when run-time compilation
-// is off, failed for this program, or (compile false) the program's kernel has
-// not been compiled by an earlier call.
-int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks,
-               unsigned* ticket, double* out, bool compile, void* stream, bool* done);
+// Reduction: writes one (hi, lo) partial per block into partial (room for
+// max_blocks) and the block count into *blocks; the caller folds the partials
+// with reduce_final_kernel.  Sets *done, or leaves it false when run-time
+// compilation is off, failed for this program, or (compile false) the program's
+// kernel has not been compiled by an earlier call.
+int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks, bool compile,
+               void* stream, int* blocks, bool* done);
 
 // One (field, offset) read of an expression; edgeless when an edgeless stencil
 // lies between the read and the root.

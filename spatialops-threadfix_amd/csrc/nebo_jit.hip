@@ -60,9 +60,7 @@ namespace {
   int mY[6];                                                                \
   int nblocks;                                                              \
   int ntx;                                                                  \
-  double c[48];                                                             \
-  double* res;                                                              \
-  unsigned* ticket;
+  double c[48];
 
 struct JitArgs {
   NEBO_JIT_ARGS_FIELDS
@@ -561,8 +559,7 @@ std::string jit_source(const VmProgram& P, int V, const Shape& sh)
 // per item, each thread folding its cells in z order into a double-double
 // (sum, norm) or a select-form max/min exactly as the interpreter's
 // vm_reduce_kernel does; then a fixed-order block reduction writes one
-// (hi, lo) partial per block to A.out, and the last block to finish folds the
-// partials into *A.res.
+// (hi, lo) partial per block to A.out, which reduce_final_kernel folds.
 std::string jit_reduce_source(const VmProgram& P, int V, int op)
 {
   const SlotUse u = slot_use(P);
@@ -607,39 +604,18 @@ std::string jit_reduce_source(const VmProgram& P, int V, int op)
   });
   o << "  }\n";
   // fixed-order block reduction: xor-shuffle tree inside each wave, then the waves in order
-  auto block_tree = [&](const char* ind) {
-    o << ind << "for (int off = 32; off >= 1; off >>= 1) {\n";
-    o << ind << "  const double h2 = __shfl_xor(hi, off, 64);\n";
-    o << ind << "  const double l2 = " << (dd ? "__shfl_xor(lo, off, 64)" : "0.0") << ";\n";
-    o << ind << "  " << combine << "\n" << ind << "}\n";
-    o << ind << "if (lane == 0) { sh[2 * wave] = hi; sh[2 * wave + 1] = lo; }\n";
-    o << ind << "__syncthreads();\n";
-    o << ind << "if (threadIdx.x == 0) {\n" << ind << "  hi = sh[0]; lo = sh[1];\n";
-    o << ind << "  for (int w = 1; w < " << B / 64 << "; ++w) {\n";
-    o << ind << "    const double h2 = sh[2 * w], l2 = sh[2 * w + 1];\n    " << ind << combine << "\n" << ind
-      << "  }\n" << ind << "}\n";
-  };
+  o << "  for (int off = 32; off >= 1; off >>= 1) {\n";
+  o << "    const double h2 = __shfl_xor(hi, off, 64);\n";
+  o << "    const double l2 = " << (dd ? "__shfl_xor(lo, off, 64)" : "0.0") << ";\n";
+  o << "    " << combine << "\n  }\n";
   o << "  __shared__ double sh[" << 2 * (B / 64) << "];\n";
-  o << "  __shared__ int last;\n";
   o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
-  block_tree("  ");
-  o << "  if (threadIdx.x == 0) {\n";
-  o << "    A.out[2 * blockIdx.x] = hi;\n    A.out[2 * blockIdx.x + 1] = " << (dd ? "lo" : "0.0") << ";\n";
-  // last-block fold (as fold_if_last in nebo_eval.hip): release the partial, take
-  // a ticket; the last block acquires all partials, folds them in index order
-  // with the same tree, writes the result and resets the ticket
-  o << "    __threadfence();\n";
-  o << "    last = atomicAdd(A.ticket, 1u) == (unsigned)(A.nblocks - 1);\n  }\n";
-  o << "  __syncthreads();\n  if (!last) return;\n  __threadfence();\n";
-  o << "  hi = " << init << "; lo = 0.0;\n";
-  o << "  for (int i = threadIdx.x; i < A.nblocks; i += " << B << ") {\n";
-  o << "    const double h2 = A.out[2 * i], l2 = A.out[2 * i + 1];\n    " << combine << "\n  }\n";
-  block_tree("  ");
-  o << "  if (threadIdx.x == 0) {\n";
-  if (op == NEBO_RED_MAX || op == NEBO_RED_MIN) o << "    double v = hi;\n";
-  else o << "    double v = hi + lo;\n";
-  if (op == NEBO_RED_NORM) o << "    v = __builtin_sqrt(v);\n";
-  o << "    *A.res = v;\n    atomicExch(A.ticket, 0u);\n  }\n";
+  o << "  if (lane == 0) { sh[2 * wave] = hi; sh[2 * wave + 1] = lo; }\n";
+  o << "  __syncthreads();\n";
+  o << "  if (threadIdx.x == 0) {\n    hi = sh[0]; lo = sh[1];\n";
+  o << "    for (int w = 1; w < " << B / 64 << "; ++w) {\n";
+  o << "      const double h2 = sh[2 * w], l2 = sh[2 * w + 1];\n      " << combine << "\n    }\n";
+  o << "    A.out[2 * blockIdx.x] = hi;\n    A.out[2 * blockIdx.x + 1] = " << (dd ? "lo" : "0.0") << ";\n  }\n";
   o << "}\n";
   return o.str();
 }
@@ -951,8 +927,8 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
   return NEBO_OK;
 }
 
-int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks,
-               unsigned* ticket, double* out, bool compile, void* stream, bool* done)
+int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks, bool compile,
+               void* stream, int* blocks, bool* done)
 {
   *done = false;
   if (!jit_enabled()) return NEBO_OK;
@@ -964,15 +940,14 @@ int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* par
   JitArgs A;
   fill_args(P, lim, V, &A);
   A.out = partial;
-  A.res = out;
-  A.ticket = ticket;
   const long items = (long)A.nplane * A.nzc;
   long nb = (items + kReduceBlock - 1) / kReduceBlock;
   if (nb > max_blocks) nb = max_blocks;
   if (nb < 1) nb = 1;
   A.nblocks = (int)nb;
-  BackendScope bs("compiled reduction kernel (" + std::to_string(nb) + " blocks, last block folds)");
+  BackendScope bs("compiled reduction kernel (" + std::to_string(nb) + " blocks)");
   if ((rc = launch(e, &A, (unsigned)nb, 1, kReduceBlock, stream))) return rc;
+  *blocks = (int)nb;
   *done = true;
   return NEBO_OK;
 }

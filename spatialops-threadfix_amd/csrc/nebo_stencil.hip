@@ -250,7 +250,7 @@ __global__ __launch_bounds__(kTx * TY) void divkgrad_staged_kernel(
 // row.  (Measured at 256^3 with 4 rows per lane: the halo rows cost ~13 us of an
 // ~84 us launch with one wave per workgroup, ~3 us with four stacked.)
 
-template <int FORM, int W, int NW, int RY, int ZC, bool XR, int PF = 2>
+template <int FORM, int W, int NW, int RY, int ZC, bool XR, int PF = 2, int SP = 2>
 __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
     double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
     Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
         fz[j] = fz_hi;
         fy_lo = fy_hi;
         const double v = divk_combine<FORM>(Dx, Dy, Dz, pc, dt);
-        if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(r, T, X, XY, z), X, XY), g.row[j + 1], v);
+        if (xin && y0 + j < T.hi[1]) store_at<SP>(plane_rsrc(plane_base(r, T, X, XY, z), X, XY), g.row[j + 1], v);
       }
       S[jj] = load(min(z + NS, z1));      // plane z is consumed: its slot takes z+NS
     }
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(kTx * TY) void lapl_euler_staged_kernel(
 }
 
 // Column schedule of the Laplacian + Euler update (see divkgrad_column_kernel).
-template <int W, int NW, int RY, int ZC>
+template <int W, int NW, int RY, int ZC, int SP = 2>
 __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
     double* __restrict__ out, const double* __restrict__ p, Tiles T, int X, int XY,
     nebo_axis_coefs_t cc, double s)
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
       const double gz_hi = pc * c.gz0 + C1.p[j + 1] * c.gz1;
       const double v = lapl_cell(pc, pxm, pxp, C0.p[j], C0.p[j + 2], gz[j], gz_hi, c, s);
       gz[j] = gz_hi;
-      if (xin && y0 + j < T.hi[1]) store_nt_at(plane_rsrc(plane_base(out, T, X, XY, z), X, XY), g.row[j + 1], v);
+      if (xin && y0 + j < T.hi[1]) store_at<SP>(plane_rsrc(plane_base(out, T, X, XY, z), X, XY), g.row[j + 1], v);
     }
     S[jj] = load(min(z + 3, z1));
   };
@@ -557,7 +557,8 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
 //    reuse and runs its column schedule for both 3 and 4);
 // 5, 6: column schedule with x-face reuse, 4 resp. 2 rows per lane, ty/4 resp.
 //    ty/2 waves stacked in y per workgroup (ty = the workgroup's rows);
-// 7, 8: as 5 with loads issued 3 resp. 4 planes ahead instead of 2.
+// 7, 8: as 5 with loads issued 3 resp. 4 planes ahead instead of 2;
+// 9: as 7 with write-back stores instead of nontemporal ones.
 struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
     {64, 4, 16, 0},   {64, 8, 16, 0},
@@ -571,7 +572,9 @@ constexpr TileShape kShapes[] = {
     {64, 8, 32, 6},   {64, 16, 32, 6},
     {128, 8, 32, 7},  {128, 8, 64, 7},  {64, 16, 64, 7},  {256, 4, 64, 7},  {128, 4, 128, 7},
     {128, 8, 128, 7}, {128, 8, 256, 7}, {128, 8, 512, 7},
-    {128, 8, 64, 8}};
+    {128, 8, 64, 8},
+    {128, 16, 32, 7}, {128, 16, 64, 7}, {256, 8, 64, 7},
+    {128, 8, 64, 9},  {128, 8, 512, 9}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // The default tile (rule 4 of select_shape): 128 x 8 rows (2 x 2 waves)
 // marching 64 planes with loads 3 planes ahead, the fastest at 256^3 on MI355X
@@ -591,10 +594,15 @@ static_assert(kDefaultShape >= 0 && kShapes[kFallbackShape].variant == 0, "defau
 // Rows per lane and waves stacked in y of a column tile.
 constexpr int rows_per_lane(const TileShape& sh)
 {
-  return (sh.variant == 5 || sh.variant == 7 || sh.variant == 8) ? 4 : sh.variant == 6 ? 2 : sh.ty;
+  return (sh.variant == 5 || sh.variant == 7 || sh.variant == 8 || sh.variant == 9) ? 4 : sh.variant == 6 ? 2 : sh.ty;
 }
 // Planes of load prefetch of a column tile.
-constexpr int prefetch_planes(const TileShape& sh) { return sh.variant == 7 ? 3 : sh.variant == 8 ? 4 : 2; }
+constexpr int prefetch_planes(const TileShape& sh)
+{
+  return (sh.variant == 7 || sh.variant == 9) ? 3 : sh.variant == 8 ? 4 : 2;
+}
+// Cache policy of a column tile's stores (2 nontemporal, 0 write-back).
+constexpr int store_policy(const TileShape& sh) { return sh.variant == 9 ? 0 : 2; }
 constexpr int stacked_waves(const TileShape& sh) { return sh.ty / rows_per_lane(sh); }
 
 // Launch geometry of one tile: rows per tile in y and threads per workgroup.
@@ -616,7 +624,7 @@ constexpr DivkFn divk_fn()
   if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
   else if constexpr (sh.variant >= 3)
     return divkgrad_column_kernel<FORM, sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc, sh.variant != 3,
-                                  prefetch_planes(sh)>;
+                                  prefetch_planes(sh), store_policy(sh)>;
   else return divkgrad_staged_kernel<FORM, sh.ty, sh.zc, sh.variant>;
 }
 
@@ -626,7 +634,7 @@ constexpr LaplFn lapl_fn()
   constexpr TileShape sh = kShapes[S];
   if constexpr (sh.variant == 0) return lapl_euler_kernel<sh.ty, sh.zc>;
   else if constexpr (sh.variant >= 3)
-    return lapl_euler_column_kernel<sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc>;
+    return lapl_euler_column_kernel<sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc, store_policy(sh)>;
   else return lapl_euler_staged_kernel<sh.ty, sh.zc, sh.variant>;
 }
 
@@ -736,7 +744,8 @@ std::string tile_backend_text(const char* kernel, int shape)
   else if (sh.variant <= 2) o << ", LDS-staged rows, " << sh.variant << " planes of prefetch";
   else
     o << ", column schedule, " << rows_per_lane(sh) << " rows per lane, " << sh.tx / kTx << "x"
-      << stacked_waves(sh) << " waves, loads " << prefetch_planes(sh) << " planes ahead";
+      << stacked_waves(sh) << " waves, loads " << prefetch_planes(sh) << " planes ahead"
+      << (store_policy(sh) == 0 ? ", write-back stores" : "");
   return o.str();
 }
 

@@ -149,6 +149,7 @@ _SIGS = {
     "nebo_divkgrad": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), C.c_int, _vp]),
     "nebo_lapl_euler": (C.c_int, [P(FieldT), P(FieldT), P(AxisCoefsT), C.c_double, _vp]),
     "nebo_stencil_tiles": (C.c_int, [P(C.c_int32)] * 5),
+    "nebo_stream_triad": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int64, C.c_int32, _vp]),
     "nebo_stencil_set_tile": (C.c_int, [C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "nebo_stencil_get_tile": (C.c_int, [C.c_int] + [P(C.c_int32)] * 4),
     "nebo_stencil_select_tile": (C.c_int, [C.c_int, C.c_int, C.c_int32, C.c_int32, C.c_int32, P(C.c_int32),

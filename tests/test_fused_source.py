@@ -33,7 +33,7 @@ def rtc():
 
 
 READELF = "/opt/rocm/llvm/bin/llvm-readelf"
-ARGS_BYTES = 1296   # sizeof(JitArgs) in csrc/nebo_jit.hip
+ARGS_BYTES = 1280   # sizeof(JitArgs) in csrc/nebo_jit.hip
 
 
 def compile_src(rtc, src, want_code=False):
