@@ -321,10 +321,10 @@ void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int
 // axis.  A wave covers 64 consecutive x, each lane RY rows, and walks a z chunk;
 // a lane keeps, per family, its own column's values of the rows and planes the
 // next z step needs in registers (one new plane loaded per step), x neighbours
-// come from the adjacent lanes (DPP), and lanes 0 and 63 load the value from
-// past the wave's edge.  Every index is clamped to its slot's [glo, ghi) so that
-// lanes and rows past the box read inside the allocation; their results are
-// not stored (column_choice keeps the indices within 32 bits).
+// come from the adjacent lanes (DPP), and the values past the wave's two edges
+// come from one load per family (and row / plane shift) for all rows.  Reads are
+// raw buffer loads bounded by the family's allocation, so lanes and rows past
+// the box read 0 past it instead of faulting; their results are not stored.
 struct ColumnPlan {
   int nf = 0, RY = 4;
   int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
@@ -405,6 +405,21 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   direct.chains.clear();
   std::ostringstream o;
   o << "struct JitArgs { " NEBO_JIT_STR(NEBO_JIT_ARGS_FIELDS) " };\n";
+  // Family reads are raw buffer loads: a resource per plane whose base is the
+  // family's lowest read point of that plane (x = lo0 + cx - 1, y = lo1 + ylo)
+  // and whose range ends at the family's allocation, plus a per-lane 32-bit byte
+  // offset computed once.  A lane or row past the box may form an offset past
+  // the allocation; the hardware returns 0 for it instead of faulting, and its
+  // results are not stored.
+  o << "__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rs_(const double* base, long pb, long ghi)\n{\n"
+       "  long n = (ghi - pb) * 8;\n  n = n < 0 ? 0 : n > 0x7ffffff8L ? 0x7ffffff8L : n;\n"
+       "  return __builtin_amdgcn_make_buffer_rsrc((void*)(base + pb), (short)0, (int)n, 0x00020000);\n}\n";
+  o << "__device__ __forceinline__ double ld_(__amdgpu_buffer_rsrc_t r, unsigned off)\n{\n"
+       "  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));\n}\n";
+  o << "__device__ __forceinline__ double rdlane_(double v, int j)\n{\n"
+       "  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);\n"
+       "  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);\n"
+       "  return __hiloint2double(hi, lo);\n}\n";
   o << "extern \"C\" __global__ __launch_bounds__(" << 64 * NW << ") void nebo_fused(const JitArgs A)\n{\n";
   if (cs.xcd) {
     o << "  const int L = (int)blockIdx.x, d = L % " << kXcds << ";\n";
@@ -424,21 +439,58 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   auto ring = [](int f, int j, int k) {
     return "R" + std::to_string(f) + "_" + std::to_string(j) + "_" + std::to_string(k);
   };
-  // ring load of family f, row j (0-based from ylo), plane slot k at plane z + p
-  auto load = [&](const std::string& ind, int f, int j, int k, const std::string& zexpr) {
+  // Edge reads: per family, one load per (dy, dz) of its x-shifted reads serves
+  // every row: lanes 0 .. RY-1 read column x0 - 1 of rows 0 .. RY-1 (shifted by
+  // dy), lanes 32 .. 32+RY-1 column x0 + 64; other lanes re-read their row 0.
+  struct EdgeRead { int f, dy, dz; };
+  std::vector<EdgeRead> edges;
+  auto edge_index = [&](int f, int dy, int dz) {
+    for (size_t e = 0; e < edges.size(); ++e)
+      if (edges[e].f == f && edges[e].dy == dy && edges[e].dz == dz) return (int)e;
+    return -1;
+  };
+  for (int s = 0; s < kVmMaxLoads; ++s)
+    if (cp.fam[s] >= 0 && cp.dx[s] != 0 && edge_index(cp.fam[s], cp.dy[s], cp.dz[s]) < 0)
+      edges.push_back(EdgeRead{cp.fam[s], cp.dy[s], cp.dz[s]});
+  // per-lane byte offsets from a plane base: ring rows, and edge reads
+  for (int f = 0; f < cp.nf; ++f) {
     const int r = cp.root[f], cls = P.loads[r].cls;
-    o << ind << "{ const int i_ = x + " << cp.cx[f] << " + A.clsX[" << cls << "] * (y0 + " << cp.ylo[f] + j
-      << ") + A.clsXY[" << cls << "] * (" << zexpr << "); " << ring(f, j, k) << " = A.base[" << r
-      << "][i_ < A.glo[" << r << "] ? A.glo[" << r << "] : i_ >= A.ghi[" << r << "] ? A.ghi[" << r << "] - 1 : i_]; }\n";
+    const int rows = RY + cp.yhi[f] - cp.ylo[f];
+    o << "  const long pb" << f << "_ = (long)(A.lo0 + " << cp.cx[f] - 1 << ") + (long)A.clsX[" << cls
+      << "] * (A.lo1 + " << cp.ylo[f] << ");\n";
+    for (int j = 0; j < rows; ++j)
+      o << "  const unsigned o" << f << "_" << j << " = (unsigned)((x - A.lo0 + 1) + A.clsX[" << cls
+        << "] * (y0 - A.lo1 + " << j << ")) * 8u;\n";
+  }
+  for (size_t e = 0; e < edges.size(); ++e) {
+    const int f = edges[e].f, cls = P.loads[cp.root[f]].cls;
+    // x of the wave's first lane is x - lane_; rows shifted by dy relative to ylo
+    o << "  const unsigned eo" << e << "_ = lane_ < " << RY << " ? (unsigned)((x - lane_ - A.lo0) + A.clsX[" << cls
+      << "] * (y0 - A.lo1 + lane_ + " << edges[e].dy - cp.ylo[f] << ")) * 8u : (lane_ >= 32 && lane_ < "
+      << 32 + RY << ") ? (unsigned)((x - lane_ + 65 - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ - 32 + "
+      << edges[e].dy - cp.ylo[f] << ")) * 8u : o" << f << "_" << -cp.ylo[f] << ";\n";
+  }
+  // the resource of family f at plane zexpr
+  auto rsrc = [&](int f, const std::string& zexpr) {
+    const int r = cp.root[f], cls = P.loads[r].cls;
+    return "plane_rs_(A.base[" + std::to_string(r) + "], pb" + std::to_string(f) + "_ + (long)A.clsXY[" +
+           std::to_string(cls) + "] * (" + zexpr + "), (long)A.ghi[" + std::to_string(r) + "])";
+  };
+  // ring rows of family f into plane slot k, plane zexpr
+  auto load_plane = [&](const std::string& ind, int f, int k, const std::string& zexpr) {
+    const int rows = RY + cp.yhi[f] - cp.ylo[f];
+    o << ind << "{\n" << ind << "  const __amdgpu_buffer_rsrc_t rs_ = " << rsrc(f, zexpr) << ";\n";
+    for (int j = 0; j < rows; ++j)
+      o << ind << "  " << ring(f, j, k) << " = ld_(rs_, o" << f << "_" << j << ");\n";
+    o << ind << "}\n";
   };
   // slots 0 .. np-1 hold planes z + zlo .. z + zhi; slot np receives plane
-  // z + zhi + 1 while step z computes (past the chunk end it is clamped, unused)
+  // z + zhi + 1 while step z computes (past the chunk end it is not used)
   for (int f = 0; f < cp.nf; ++f) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
     for (int j = 0; j < rows; ++j)
       for (int k = 0; k <= np; ++k) o << "  double " << ring(f, j, k) << " = 0.0;\n";
-    for (int k = 0; k < np; ++k)
-      for (int j = 0; j < rows; ++j) load("  ", f, j, k, "zs + " + std::to_string(cp.zlo[f] + k));
+    for (int k = 0; k < np; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
   }
   // One z step at plane zq: the wave-edge reads, the next plane's loads, then
   // RY cells.  Ring slot k of family f (plane z + zlo + k; k = np is the plane
@@ -449,19 +501,12 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     auto slot = [&](int f, int k) { return (k + u) % (cp.zhi[f] - cp.zlo[f] + 2); };
     // edge reads first: loads complete in issue order, so waiting for them does
     // not wait for the next plane
-    for (int r = 0; r < RY; ++r)
-      for (int s = 0; s < kVmMaxLoads; ++s) {
-        if (cp.fam[s] < 0 || cp.dx[s] == 0) continue;
-        const int cls = P.loads[s].cls;
-        const std::string E = "E" + std::to_string(r) + "_" + std::to_string(s);
-        o << in << "double " << E << " = 0.0;\n";
-        o << in << "if (lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << ") { const int i_ = x + A.clsX[" << cls
-          << "] * (y0 + " << r << ") + A.clsXY[" << cls << "] * zq; " << E << " = A.base[" << s << "][i_ < A.glo["
-          << s << "] ? A.glo[" << s << "] : i_ >= A.ghi[" << s << "] ? A.ghi[" << s << "] - 1 : i_]; }\n";
-      }
+    for (size_t e = 0; e < edges.size(); ++e)
+      o << in << "const double E" << e << "_ = ld_(" << rsrc(edges[e].f, "zq + " + std::to_string(edges[e].dz))
+        << ", eo" << e << "_);\n";
     for (int f = 0; f < cp.nf; ++f) {
-      const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
-      for (int j = 0; j < rows; ++j) load(in, f, j, slot(f, np), "zq + " + std::to_string(cp.zhi[f] + 1));
+      const int np = cp.zhi[f] - cp.zlo[f] + 1;
+      load_plane(in, f, slot(f, np), "zq + " + std::to_string(cp.zhi[f] + 1));
     }
     for (int r = 0; r < RY; ++r) {
       const std::string ind = in + "  ";
@@ -474,11 +519,12 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
           o << ind << "const double L0_" << s << " = " << R << ";\n";
           continue;
         }
+        const int e = edge_index(f, cp.dy[s], cp.dz[s]);
         // the exchange is taken by every lane, outside any branch
         o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << R << ", "
           << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
-        o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? E" << r << "_" << s
-          << " : D0_" << s << ";\n";
+        o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? rdlane_(E" << e
+          << "_, " << (cp.dx[s] > 0 ? 32 : 0) + r << ") : D0_" << s << ";\n";
       }
       const std::vector<std::string> res = emit_cells(o, P, direct, 1, true, ind);
       o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res[0]
@@ -731,10 +777,11 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
   cs->waves = g_column_waves.load();
   cs->zc = g_column_zc.load();
   cs->xcd = g_xcd.load();
-  // Indices are 32-bit: lanes, rows and planes past the box reach at most
-  // 66 + X (RY waves + 2) + 3 XY past a slot's last valid index before the
-  // clamp, which must not overflow (fields hold fewer than 2^31 points, so only
-  // fields within that margin of the limit keep the flattened schedule).
+  // Indices of the kernel's stores and per-lane offsets are 32-bit: lanes, rows
+  // and planes past the box reach at most 66 + X (RY waves + 2) + 3 XY past a
+  // slot's last valid index, which must not overflow (fields hold fewer than
+  // 2^31 points, so only fields within that margin of the limit keep the
+  // flattened schedule).
   for (int s = 0; s < kVmMaxLoads; ++s) {
     if (cp->fam[s] < 0) continue;
     const int k = P.loads[s].cls;

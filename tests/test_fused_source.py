@@ -189,12 +189,13 @@ def test_column_schedule_source():
     """div(k grad phi) runs on the column schedule by default: k and phi are one
     family each (reads within one point in x, y and z), each lane holds 4 rows,
     so a family keeps rows -1 .. 4 of 3 planes plus the plane loading: 2 x 6 x 3
-    loads before the z loop and 2 x 6 per step.  The four x-neighbour reads per
-    cell come from the adjacent lanes (4 x 4 lane exchanges per step), with one
-    predicated load each for the lane at the wave's edge.  The ring of 4 planes
-    makes the main loop 4 steps long (registers renamed, not copied) and a
-    remainder loop 1 step.  Trees with masks, and pointwise trees, keep the
-    flattened-plane schedule, as does every tree with the column schedule
+    loads before the z loop and 2 x 6 per step, each a raw buffer load at a
+    per-lane offset from a per-plane resource.  The four x-neighbour reads per
+    cell come from the adjacent lanes (4 x 4 lane exchanges per step); the values
+    past the wave's edges come from one load per family and step for all rows.
+    The ring of 4 planes makes the main loop 4 steps long (registers renamed, not
+    copied) and a remainder loop 1 step.  Trees with masks, and pointwise trees,
+    keep the flattened-plane schedule, as does every tree with the column schedule
     switched off."""
     t = _divkgrad_tree()
     src = nh.fused_source(t)
@@ -202,15 +203,18 @@ def test_column_schedule_source():
     pro, loops = src.split("int z = zs;   // column schedule")
     main, rest = loops.split("for (; z < zend; ++z)")
     assert "for (; z + 3 < zend; z += 4)" in main
-    ring = lambda text: len(re.findall(r"R\d+_\d+_\d+ = A\.base\[", text))
-    edge = lambda text: len(re.findall(r"E\d+_\d+ = A\.base\[", text))
+    ring = lambda text: len(re.findall(r"R\d+_\d+_\d+ = ld_\(rs_, o\d+_\d+\)", text))
+    edge = lambda text: len(re.findall(r"E\d+_ = ld_\(plane_rs_\(", text))
     assert ring(pro) == 2 * 6 * 3 and ring(main) == 4 * 2 * 6 and ring(rest) == 2 * 6
     assert main.count("__builtin_amdgcn_mov_dpp") == 4 * 16 and rest.count("__builtin_amdgcn_mov_dpp") == 16
-    assert edge(main) == 4 * 16 and edge(rest) == 16
+    assert edge(main) == 4 * 2 and edge(rest) == 2
+    assert main.count("rdlane_(E") == 4 * 16 and rest.count("rdlane_(E") == 16
     # the main loop renames (no ring copies); the remainder loop shifts the ring
     assert not re.search(r"R\d+_\d+_\d+ = R", main) and len(re.findall(r"R\d+_\d+_\d+ = R", rest)) == 2 * 6 * 3
-    # every read is clamped to its slot's allocation (lanes and rows past the box)
-    assert src.count("A.base[") == src.count("i_ < A.glo[") == 36 + 5 * (12 + 16)
+    # one resource per family and plane (3 planes before the loop, 2 ring + 2 edge
+    # per step), and no per-read index clamp
+    assert pro.count("plane_rs_(A.base[") == 6 and main.count("plane_rs_(A.base[") == 4 * 4
+    assert rest.count("plane_rs_(A.base[") == 4 and "i_ < A.glo[" not in src
     n = (6, 5, 4)
     L = nh.Layout.build(n, 1)
     f, g = host(L), host(L)
