@@ -311,9 +311,27 @@ enum nebo_reduce_op { NEBO_RED_MAX = 0, NEBO_RED_MIN = 1, NEBO_RED_SUM = 2, NEBO
 int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
                 void* stream);
 
-/* Same, asynchronous: writes one double to device memory out_dev. */
+/* Same, asynchronous: writes one double to device memory out_dev.  Two kernel
+ * launches (one partial per block, then a fixed-order fold); the partials live in
+ * a buffer the library creates at the first reduction on each stream and keeps,
+ * so later calls on that stream allocate nothing. */
 int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
                       double* out_dev, void* stream);
+
+/* nebo_reduce_async that allocates nothing, ever.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * partials go to a static device array of 32 KB per GPU (the partials of at most
+ * 2048 blocks plus the result slot; grids never exceed 2048 blocks, so no input
+ * size needs more), shared by every call on that GPU and holding the last call's
+ * contents: calls that overlap on one GPU (on different streams, or from
+ * different threads) race and must not be issued.  It runs the kernel compiled
+ * for the tree's shape when an earlier allocating call compiled it (then its
+ * result equals nebo_reduce_async's bit for bit), else the precompiled
+ * interpreter kernels (max/min equal, sum/norm within one rounding).  The
+ * operands must be on the current device. */
+int nebo_reduce_async_noalloc(const nebo_expr_t* expr, int op, int use_ghost, double* out_dev,
+                              void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* ghost cells                                                               */

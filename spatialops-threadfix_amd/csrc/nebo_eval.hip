@@ -45,6 +45,19 @@ inline RedBuf red_buf_at(char* p)
   return RedBuf{d, d + 2 * kRedMaxBlocks};
 }
 
+}  // namespace
+}  // namespace nebo
+
+// Working memory of nebo_reduce_async_noalloc: the block partials and result
+// slot of one reduction (nebo::kRedBufBytes = 32 KB), one copy per GPU, held
+// until the process ends.  Host code never passes this name to a kernel: on the
+// host it is only a handle for hipGetSymbolAddress, which gives each device's
+// address (nebo::scratch_on_device).
+static __device__ __align__(256) char scratch[nebo::kRedBufBytes];
+
+namespace nebo {
+namespace {
+
 __device__ __noinline__ double vm_math(int op, double x)
 {
   switch (op) {
@@ -652,6 +665,7 @@ int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, doubl
   int blocks = (int)((items + kBlock - 1) / kBlock);
   if (blocks > kRedMaxBlocks) blocks = kRedMaxBlocks;
   if (blocks < 1) blocks = 1;
+  static_assert(kRedBufBytes >= sizeof(double) * (2 * kRedMaxBlocks + 1), "partials and result fit the buffer");
   bool done = false;
   int jit_blocks = 0;
   int rc = jit_reduce(prog, lim, op, buf.partial, kRedMaxBlocks, compile, s, &jit_blocks, &done);
@@ -824,6 +838,49 @@ extern "C" int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
   StreamRedBuf b;
   if ((rc = stream_reduce_buffer(stream, &b))) return rc;
   return do_reduce(expr, op, use_ghost, out_dev, stream, b.buf, true);
+}
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device
+// (with that device current) and kept.
+int scratch_on_device(int device, char** out)
+{
+  static const char* ctx = "Nebo Reduction";
+  if (device < 0 || device >= kScratchDevices)
+    return set_error(NEBO_ERR_DEVICE, ctx, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    char* p = nullptr;
+    int rc = check_hip(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)), ctx);
+    if (rc) return rc;
+    if (!p) return set_error(NEBO_ERR_HIP, ctx, "no device address for the reduction scratch");
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return NEBO_OK;
+}
+}  // namespace
+
+extern "C" int nebo_reduce_async_noalloc(const nebo_expr_t* expr, int op, int use_ghost, double* out_dev,
+                                         void* stream)
+{
+  static const char* ctx = "Nebo Reduction";
+  clear_error();
+  int rc = check_reduce(expr, op, use_ghost);
+  if (rc) return rc;
+  if (!out_dev) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  int device = 0;
+  if ((rc = check_hip(hipGetDevice(&device), ctx))) return rc;
+  for (int i = 0; i < expr->n_fields; ++i)
+    if (expr->fields[i].device != device)
+      return set_error(NEBO_ERR_DEVICE, ctx, "operands must be resident on the current GPU");
+  char* p = nullptr;
+  if ((rc = scratch_on_device(device, &p))) return rc;
+  return do_reduce(expr, op, use_ghost, out_dev, stream, red_buf_at(p), false);
 }
 
 extern "C" int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,

@@ -145,6 +145,7 @@ _SIGS = {
     "nebo_masked_assign": (C.c_int, [P(FieldT), _vp, C.c_int32, P(ExprT), _vp]),
     "nebo_reduce": (C.c_int, [P(ExprT), C.c_int, C.c_int, P(C.c_double), _vp]),
     "nebo_reduce_async": (C.c_int, [P(ExprT), C.c_int, C.c_int, _vp, _vp]),
+    "nebo_reduce_async_noalloc": (C.c_int, [P(ExprT), C.c_int, C.c_int, _vp, _vp]),
     "nebo_fill_ghosts_periodic": (C.c_int, [P(FieldT), _vp]),
     "nebo_divkgrad": (C.c_int, [P(FieldT), P(FieldT), P(FieldT), P(AxisCoefsT), C.c_int, _vp]),
     "nebo_lapl_euler": (C.c_int, [P(FieldT), P(FieldT), P(AxisCoefsT), C.c_double, _vp]),
