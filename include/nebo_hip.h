@@ -266,6 +266,15 @@ int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk);
 int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);
 int nebo_fused_source(const nebo_expr_t* rhs, int32_t reduce_op, char* buf, int32_t size,
                       int32_t* needed);
+/* (host only) What nebo_assign would launch for lhs <<= rhs when the tree runs on
+ * the compiled column schedule: the kernel source (as nebo_fused_source), its
+ * argument struct (args_size bytes at least: 1280), grid and block sizes.
+ * NEBO_ERR_UNSUPPORTED when the tree takes another path.  No counterpart in the
+ * reference: it lets the tests run the generated kernels' address arithmetic on
+ * the host and check every access against the fields' allocations. */
+int nebo_fused_column_launch(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, char* src,
+                             int32_t src_size, int32_t* src_needed, void* args, int32_t args_size,
+                             int32_t* grid, int32_t* block);
 
 /* Which kernel nebo_assign would pick for this tree: 0 generic evaluator,
  * 1 var-coefficient diffusion, 2 Laplacian+Euler, 3 pointwise copy/scale.
@@ -420,7 +429,8 @@ int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const nebo_field_t*
  * variant 0: 8-byte accesses, nontemporal stores; 1: 16-byte accesses,
  * nontemporal stores; 2: 16-byte accesses, write-back stores; 3: two 16-byte
  * accesses per lane and array, nontemporal stores (1-3 need 16-byte aligned
- * arrays). */
+ * arrays); 4: 16-byte accesses, nontemporal stores, on arrays that start 8 bytes
+ * past a 16-byte boundary (as interior rows of a one-ghost field do). */
 int nebo_stream_triad(double* r, const double* a, const double* b, double s, int64_t n, int32_t variant,
                       void* stream);
 

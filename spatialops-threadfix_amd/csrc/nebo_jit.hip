@@ -441,17 +441,28 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   };
   // Edge reads: per family, one load per (dy, dz) of its x-shifted reads serves
   // every row: lanes 0 .. RY-1 read column x0 - 1 of rows 0 .. RY-1 (shifted by
-  // dy), lanes 32 .. 32+RY-1 column x0 + 64; other lanes re-read their row 0.
-  struct EdgeRead { int f, dy, dz; };
+  // dy) when the family reads x - 1 at that (dy, dz), lanes 32 .. 32+RY-1 column
+  // x0 + 64 when it reads x + 1; other lanes re-read their row 0.  A side the
+  // family does not read may lie outside the region its reads cover, below the
+  // allocation's start where the range check does not reach (a family reading
+  // x and x + 1 only, on a field whose assignment starts at the block's first
+  // column), so it is never loaded.
+  struct EdgeRead { int f, dy, dz; bool left, right; };
   std::vector<EdgeRead> edges;
   auto edge_index = [&](int f, int dy, int dz) {
     for (size_t e = 0; e < edges.size(); ++e)
       if (edges[e].f == f && edges[e].dy == dy && edges[e].dz == dz) return (int)e;
     return -1;
   };
-  for (int s = 0; s < kVmMaxLoads; ++s)
-    if (cp.fam[s] >= 0 && cp.dx[s] != 0 && edge_index(cp.fam[s], cp.dy[s], cp.dz[s]) < 0)
-      edges.push_back(EdgeRead{cp.fam[s], cp.dy[s], cp.dz[s]});
+  for (int s = 0; s < kVmMaxLoads; ++s) {
+    if (cp.fam[s] < 0 || cp.dx[s] == 0) continue;
+    int e = edge_index(cp.fam[s], cp.dy[s], cp.dz[s]);
+    if (e < 0) {
+      edges.push_back(EdgeRead{cp.fam[s], cp.dy[s], cp.dz[s], false, false});
+      e = (int)edges.size() - 1;
+    }
+    (cp.dx[s] < 0 ? edges[e].left : edges[e].right) = true;
+  }
   // per-lane byte offsets from a plane base: ring rows, and edge reads
   for (int f = 0; f < cp.nf; ++f) {
     const int r = cp.root[f], cls = P.loads[r].cls;
@@ -464,11 +475,16 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   }
   for (size_t e = 0; e < edges.size(); ++e) {
     const int f = edges[e].f, cls = P.loads[cp.root[f]].cls;
+    const std::string own = "o" + std::to_string(f) + "_" + std::to_string(-cp.ylo[f]);
     // x of the wave's first lane is x - lane_; rows shifted by dy relative to ylo
-    o << "  const unsigned eo" << e << "_ = lane_ < " << RY << " ? (unsigned)((x - lane_ - A.lo0) + A.clsX[" << cls
-      << "] * (y0 - A.lo1 + lane_ + " << edges[e].dy - cp.ylo[f] << ")) * 8u : (lane_ >= 32 && lane_ < "
-      << 32 + RY << ") ? (unsigned)((x - lane_ + 65 - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ - 32 + "
-      << edges[e].dy - cp.ylo[f] << ")) * 8u : o" << f << "_" << -cp.ylo[f] << ";\n";
+    o << "  const unsigned eo" << e << "_ = ";
+    if (edges[e].left)
+      o << "lane_ < " << RY << " ? (unsigned)((x - lane_ - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ + "
+        << edges[e].dy - cp.ylo[f] << ")) * 8u : ";
+    if (edges[e].right)
+      o << "(lane_ >= 32 && lane_ < " << 32 + RY << ") ? (unsigned)((x - lane_ + 65 - A.lo0) + A.clsX[" << cls
+        << "] * (y0 - A.lo1 + lane_ - 32 + " << edges[e].dy - cp.ylo[f] << ")) * 8u : ";
+    o << own << ";\n";
   }
   // the resource of family f at plane zexpr
   auto rsrc = [&](int f, const std::string& zexpr) {
@@ -919,6 +935,21 @@ int launch(const Entry& e, JitArgs* A, unsigned gx, unsigned gy, unsigned block,
 
 bool jit_enabled() { return g_mode.load() != 0; }
 
+// Arguments and grid of a column-schedule launch of P over lim into lhs.
+int column_args(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, const ColumnPlan& cp,
+                const ColumnShape& cs, JitArgs* A, long* total)
+{
+  fill_args(P, lim, 1, A);
+  A->out = frame_origin(lhs);
+  A->oX = lhs->glob[0];
+  A->oXY = lhs->glob[0] * lhs->glob[1];
+  *total = (((long)A->n0 + 63) / 64) * (((long)A->n1 + cp.RY * cs.waves - 1) / (cp.RY * cs.waves)) *
+           (((long)A->n2 + cs.zc - 1) / cs.zc);
+  if (*total >= (1L << 31)) return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
+  A->nblocks = (int)*total;
+  return NEBO_OK;
+}
+
 int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done)
 {
   *done = false;
@@ -933,15 +964,8 @@ int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t&
                               "nebo_fused", &e);
     if (rc || !e.fn) return rc;
     JitArgs A;
-    fill_args(P, lim, 1, &A);
-    A.out = frame_origin(lhs);
-    A.oX = lhs->glob[0];
-    A.oXY = lhs->glob[0] * lhs->glob[1];
-    const long total = (((long)A.n0 + 63) / 64) * (((long)A.n1 + cp.RY * cs.waves - 1) / (cp.RY * cs.waves)) *
-                       (((long)A.n2 + cs.zc - 1) / cs.zc);
-    if (total >= (1L << 31))
-      return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
-    A.nblocks = (int)total;
+    long total = 0;
+    if ((rc = column_args(lhs, P, lim, cp, cs, &A, &total))) return rc;
     BackendScope bs("compiled kernel, column schedule (" + std::to_string(cs.waves) + " waves per block, " +
                     std::to_string(cp.RY) + " rows per lane, " + std::to_string(cs.zc) + " planes per block)");
     if ((rc = launch(e, &A, (unsigned)total, 1, (unsigned)(64 * cs.waves), stream))) return rc;
@@ -1064,6 +1088,44 @@ extern "C" int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int3
   if (compiled) *compiled = nebo::g_compiled.load();
   if (cache_hits) *cache_hits = nebo::g_hits.load();
   if (failures) *failures = nebo::g_failures.load();
+  return NEBO_OK;
+}
+
+extern "C" int nebo_fused_column_launch(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, char* src,
+                                        int32_t src_size, int32_t* src_needed, void* args, int32_t args_size,
+                                        int32_t* grid, int32_t* block)
+{
+  static const char* ctx = "Nebo Fused Kernel";
+  nebo::clear_error();
+  if (!lhs || !rhs) return nebo::set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  nebo_limits_t lim;
+  int empty = 0;
+  int rc = nebo::assign_limits(lhs, rhs, use_ghost, &lim, &empty);
+  if (rc) return rc;
+  if (empty) return nebo::set_error(NEBO_ERR_UNSUPPORTED, ctx, "empty region: no launch");
+  nebo::VmProgram P;
+  int depth = 0;
+  if ((rc = nebo::lower_program(rhs, &P, &depth))) return rc;
+  nebo::ColumnPlan cp;
+  nebo::ColumnShape cs;
+  if (!nebo::column_choice(P, &lim, &cp, &cs))
+    return nebo::set_error(NEBO_ERR_UNSUPPORTED, ctx, "the tree does not run on the column schedule");
+  nebo::JitArgs A;
+  long total = 0;
+  if ((rc = nebo::column_args(lhs, P, lim, cp, cs, &A, &total))) return rc;
+  const std::string s = nebo::jit_column_source(P, nebo::slot_use(P), cp, cs);
+  if (src_needed) *src_needed = (int32_t)s.size() + 1;
+  if (src && src_size > 0) {
+    const size_t n = s.size() < (size_t)src_size - 1 ? s.size() : (size_t)src_size - 1;
+    std::memcpy(src, s.data(), n);
+    src[n] = '\0';
+  }
+  if (args) {
+    if (args_size < (int32_t)sizeof A) return nebo::set_error(NEBO_ERR_INVALID, ctx, "argument buffer too small");
+    std::memcpy(args, &A, sizeof A);
+  }
+  if (grid) *grid = (int32_t)total;
+  if (block) *block = 64 * cs.waves;
   return NEBO_OK;
 }
 

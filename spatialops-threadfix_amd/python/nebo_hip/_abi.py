@@ -188,6 +188,8 @@ _SIGS = {
     "nebo_fused_jit_tune": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "nebo_fused_jit_column": (C.c_int, [C.c_int32, C.c_int32, C.c_int32]),
     "nebo_fused_source": (C.c_int, [P(ExprT), C.c_int32, C.c_char_p, C.c_int32, P(C.c_int32)]),
+    "nebo_fused_column_launch": (C.c_int, [P(FieldT), P(ExprT), C.c_int, C.c_char_p, C.c_int32, P(C.c_int32), _vp,
+                                           C.c_int32, P(C.c_int32), P(C.c_int32)]),
     "nebo_host_free": (C.c_int, [_vp]),
     "nebo_particle_to_cell": (C.c_int, [P(FieldT), _vp, _vp, _vp, _vp, _vp, C.c_int64, P(ParticleGeomT), _vp,
                                         _vp]),
