@@ -48,7 +48,7 @@ def _tile_name(t):
     or the column schedule (ty rows per lane)."""
     sched = {0: "", 1: "/staged-pf1", 2: "/staged-pf2", 3: "/column", 4: "/column-xreuse",
              5: "/stacked-ry4", 6: "/stacked-ry2", 7: "/stacked-ry4-pf3", 8: "/stacked-ry4-pf4",
-             9: "/stacked-ry4-pf3-wb"}[t[3]]
+             9: "/stacked-ry4-pf3-wb", 10: "/pair-ry4-pf3", 11: "/pair-ry2-pf3", 12: "/pair-ry4-pf2"}[t[3]]
     return f"{t[0]}x{t[1]}x{t[2]}{sched}"
 
 
@@ -245,17 +245,20 @@ def ceiling_leg(n=256, reps=20):
     lib = nh.lib()
     stream = torch.cuda.current_stream()
     sp = C.c_void_p(stream.cuda_stream)
-    names = {0: "8B-nt", 1: "16B-nt", 2: "16B-writeback", 3: "2x16B-nt"}
+    names = {0: "8B-nt", 1: "16B-nt", 2: "16B-writeback", 3: "2x16B-nt", 4: "16B-nt-offset8"}
     rates, ok = {}, True
-    for v in range(4):
+    for v in range(5):
+        off = 1 if v == 4 else 0   # variant 4: every array 8 bytes past a 16-byte boundary
+        m = npts - off
+
         def call():
-            rc = lib.nebo_stream_triad(C.c_void_p(r.data_ptr()), C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
-                                       0.5, npts, v, sp)
+            rc = lib.nebo_stream_triad(C.c_void_p(r.data_ptr() + 8 * off), C.c_void_p(a.data_ptr() + 8 * off),
+                                       C.c_void_p(b.data_ptr() + 8 * off), 0.5, m, v, sp)
             if rc:
                 raise RuntimeError(lib.nebo_last_error().decode())
         ms = _time_calls(call, stream, reps)
-        rates[names[v]] = round(24 * npts / (ms * 1e-3) / 1e9, 1)
-        ok = ok and bool(torch.equal(r, a + 0.5 * b))
+        rates[names[v]] = round(24 * m / (ms * 1e-3) / 1e9, 1)
+        ok = ok and bool(torch.equal(r[off:], (a + 0.5 * b)[off:]))
     best = max(rates, key=rates.get)
     return {"workload": f"r = a + s*b over 3 blocks of {n + 2}^3 fp64", "bytes_per_launch": 24 * npts,
             "GBps": rates, "best": best, "best_GBps": rates[best],

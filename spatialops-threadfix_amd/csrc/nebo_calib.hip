@@ -10,6 +10,9 @@
 //   1: 16-byte accesses, nontemporal stores
 //   2: 16-byte accesses, default (write-back) stores
 //   3: 32-byte accesses (two 16-byte accesses per lane and array), nontemporal stores
+//   4: 16-byte accesses at an 8-byte offset (the arrays start 8 bytes past a
+//      16-byte boundary, as the interior rows of a field with one ghost layer do),
+//      nontemporal stores
 // Each thread issues all its loads before its first store; blocks take
 // consecutive chunks, renumbered so that consecutive chunks run on one XCD.
 
@@ -76,12 +79,16 @@ extern "C" int nebo_stream_triad(double* r, const double* a, const double* b, do
   static const char* ctx = "Nebo Calibration";
   clear_error();
   if (!r || !a || !b || n < 0) return set_error(NEBO_ERR_INVALID, ctx, "null argument or negative size");
-  if (variant < 0 || variant > 3) return set_error(NEBO_ERR_INVALID, ctx, "triad variant must be 0..3");
+  if (variant < 0 || variant > 4) return set_error(NEBO_ERR_INVALID, ctx, "triad variant must be 0..4");
   if (n == 0) return NEBO_OK;
   const int vec = variant == 0 ? 1 : variant == 3 ? 4 : 2;
-  if (variant > 0 && ((reinterpret_cast<uintptr_t>(r) | reinterpret_cast<uintptr_t>(a) |
-                       reinterpret_cast<uintptr_t>(b)) & 15))
-    return set_error(NEBO_ERR_INVALID, ctx, "16-byte variants need 16-byte aligned arrays");
+  const uintptr_t mis = (reinterpret_cast<uintptr_t>(r) | reinterpret_cast<uintptr_t>(a) |
+                         reinterpret_cast<uintptr_t>(b)) & 15;
+  if (variant >= 1 && variant <= 3 && mis)
+    return set_error(NEBO_ERR_INVALID, ctx, "16-byte variants 1-3 need 16-byte aligned arrays");
+  if (variant == 4 && ((reinterpret_cast<uintptr_t>(r) & 15) != 8 || (reinterpret_cast<uintptr_t>(a) & 15) != 8 ||
+                       (reinterpret_cast<uintptr_t>(b) & 15) != 8))
+    return set_error(NEBO_ERR_INVALID, ctx, "variant 4 needs arrays 8 bytes past a 16-byte boundary");
   const long blocks = (n + (long)kCalibBlock * vec - 1) / ((long)kCalibBlock * vec);
   if (blocks >= (1L << 31)) return set_error(NEBO_ERR_TOO_LARGE, ctx, "more than 2^31 blocks");
   hipStream_t st = (hipStream_t)stream;
@@ -89,7 +96,8 @@ extern "C" int nebo_stream_triad(double* r, const double* a, const double* b, do
   case 0: hipLaunchKernelGGL((triad_kernel<1, true>), dim3(blocks), dim3(kCalibBlock), 0, st, r, a, b, s, n); break;
   case 1: hipLaunchKernelGGL((triad_kernel<2, true>), dim3(blocks), dim3(kCalibBlock), 0, st, r, a, b, s, n); break;
   case 2: hipLaunchKernelGGL((triad_kernel<2, false>), dim3(blocks), dim3(kCalibBlock), 0, st, r, a, b, s, n); break;
-  default: hipLaunchKernelGGL((triad_kernel<4, true>), dim3(blocks), dim3(kCalibBlock), 0, st, r, a, b, s, n); break;
+  case 3: hipLaunchKernelGGL((triad_kernel<4, true>), dim3(blocks), dim3(kCalibBlock), 0, st, r, a, b, s, n); break;
+  default: hipLaunchKernelGGL((triad_kernel<2, true>), dim3(blocks), dim3(kCalibBlock), 0, st, r, a, b, s, n); break;
   }
   return check_hip(hipGetLastError(), ctx);
 }

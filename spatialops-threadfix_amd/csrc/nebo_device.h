@@ -169,6 +169,50 @@ __device__ __forceinline__ void store_at(Rsrc rs, uint32_t off, double v)
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, off, 0, SP);
 }
 
+// Two adjacent doubles moved by one 16-byte buffer access (the address needs only
+// 8-byte alignment: compute queues run with unaligned access enabled).
+typedef double D2 __attribute__((ext_vector_type(2)));
+typedef unsigned U4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ D2 load2_at(Rsrc rs, uint32_t off)
+{
+  return __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+}
+
+// nontemporal 16-byte store
+__device__ __forceinline__ void store2_nt_at(Rsrc rs, uint32_t off, D2 v)
+{
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, v), rs, off, 0, 2);
+}
+
+// Offsets of one pair-column lane (two adjacent columns x, x + 1 with
+// x = xw0 + 2 lane) within a plane, from the plane base of plane_base: rows
+// y0-1 .. y0+RY, and one edge load: lanes 0..RY-1 take the left neighbours
+// (x = xw0 - 1) of rows 0..RY-1, lanes 32..32+RY-1 the right neighbours
+// (x = xw0 + 128), other lanes re-read their row 0.  The box width is a multiple
+// of the 128-column wave, so every lane's columns lie inside it; rows past the
+// box are clamped to the last readable row.
+template <int RY>
+struct PairGeom {
+  uint32_t row[RY + 2];
+  uint32_t edge;
+};
+
+template <int RY>
+__device__ __forceinline__ PairGeom<RY> pair_geom(const Tiles& T, int X, int xw0, int y0, int lane)
+{
+  PairGeom<RY> g;
+  const int ax = T.lo[0] - 1, ay = T.lo[1] - 1;
+  auto off = [X, ax, ay](int x, int y) { return (uint32_t)((x - ax) + X * (y - ay)) * 8u; };
+  const int x = xw0 + 2 * lane;
+#pragma unroll
+  for (int r = 0; r < RY + 2; ++r) g.row[r] = off(x, min(y0 - 1 + r, T.hi[1]));
+  if (lane < RY) g.edge = off(xw0 - 1, min(y0 + lane, T.hi[1]));
+  else if (lane >= 32 && lane < 32 + RY) g.edge = off(xw0 + 2 * kLanes, min(y0 + lane - 32, T.hi[1]));
+  else g.edge = g.row[1];
+  return g;
+}
+
 // Run f(integral_constant<int, I>) for I = 0, 1, ... in order (compile-time steps).
 template <typename F, int... I>
 __device__ __forceinline__ void unroll_steps(F&& f, std::integer_sequence<int, I...>)

@@ -361,6 +361,114 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
                std::make_integer_sequence<int, NS - 1>{});
 }
 
+// Pair-column schedule (variants 10-12): the column schedule with two adjacent
+// columns per lane, so every load and store moves 16 bytes and a wave covers
+// 128 columns (W waves side by side, NW stacked in y, RY rows per lane, loads PF
+// planes ahead).  A lane holding columns x, x+1 takes x-1 from the lower lane's
+// second column and x+2 from the upper lane's first (DPP); the wave's two outer
+// neighbours come from one edge load per field (readlane).  Each lane forms the
+// three x faces of its two cells; faces and cells use the reference's
+// operations in its order, so results equal the other schedules bit for bit.
+// Used only on boxes whose width is a multiple of 128 W (usable_shape).
+template <int FORM, int W, int NW, int RY, int ZC, int PF>
+__global__ __launch_bounds__(kTx * W * NW) void divkgrad_pair_kernel(
+    double* __restrict__ r, const double* __restrict__ k, const double* __restrict__ p,
+    Tiles T, int X, int XY, nebo_axis_coefs_t cc, double dt)
+{
+  static_assert(RY >= 1 && RY <= 8, "rows per lane");
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % T.nx;
+  const int rest = t / T.nx;
+  const int ty = rest % T.ny;
+  const int tz = rest / T.ny;
+  const int lane = threadIdx.x & (kTx - 1);
+  const int wave = threadIdx.x / kTx;
+  const int xw0 = T.lo[0] + tx * (2 * kTx * W) + (wave % W) * (2 * kTx);
+  const int y0 = T.lo[1] + ty * (RY * NW) + (wave / W) * RY;
+  const int z0 = T.lo[2] + tz * ZC;
+  const int z1 = min(z0 + ZC, T.hi[2]);
+  const PairGeom<RY> g = pair_geom<RY>(T, X, xw0, y0, lane);
+  const DivkCoefs c = divk_coefs(cc);
+
+  struct Col { D2 p[RY + 2], k[RY + 2]; double pe, ke; };
+  auto load = [&](int z) {
+    Col v;
+    const Rsrc pb = plane_rsrc(plane_base(p, T, X, XY, z), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z), X, XY);
+#pragma unroll
+    for (int q = 0; q < RY + 2; ++q) {
+      v.p[q] = load2_at(pb, g.row[q]);
+      v.k[q] = load2_at(kb, g.row[q]);
+    }
+    v.pe = load_at(pb, g.edge);
+    v.ke = load_at(kb, g.edge);
+    return v;
+  };
+
+  constexpr int NS = PF + 1;   // register slots: planes z .. z+PF
+  Col S[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
+  double fz[RY][2];
+  {
+    const Rsrc pb = plane_rsrc(plane_base(p, T, X, XY, z0 - 1), X, XY);
+    const Rsrc kb = plane_rsrc(plane_base(k, T, X, XY, z0 - 1), X, XY);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      const D2 pm = load2_at(pb, g.row[j + 1]), km = load2_at(kb, g.row[j + 1]);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        fz[j][h] = face_flux(km[h], S[0].k[j + 1][h], pm[h], S[0].p[j + 1][h], c.iz0, c.iz1, c.gz0, c.gz1);
+    }
+  }
+  auto step = [&](auto JJ, const int z) {
+    constexpr int jj = decltype(JJ)::value;
+    const Col& C0 = S[jj];              // plane z
+    const Col& C1 = S[(jj + 1) % NS];   // plane z+1
+    double fy_lo[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      fy_lo[h] = face_flux(C0.k[0][h], C0.k[1][h], C0.p[0][h], C0.p[1][h], c.iy0, c.iy1, c.gy0, c.gy1);
+    const Rsrc rb = plane_rsrc(plane_base(r, T, X, XY, z), X, XY);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      const D2 a = C0.p[j + 1], b = C0.k[j + 1];
+      // x neighbours: the exchanges are taken by every lane, then the edge lanes
+      // replace theirs with the edge values
+      const double pl_d = from_lower_lane(a[1]), kl_d = from_lower_lane(b[1]);
+      const double pr_d = from_upper_lane(a[0]), kr_d = from_upper_lane(b[0]);
+      const double pel = lane_value(C0.pe, j), kel = lane_value(C0.ke, j);
+      const double per = lane_value(C0.pe, 32 + j), ker = lane_value(C0.ke, 32 + j);
+      const double pl = lane == 0 ? pel : pl_d, kl = lane == 0 ? kel : kl_d;
+      const double pr = lane == kTx - 1 ? per : pr_d, kr = lane == kTx - 1 ? ker : kr_d;
+      const double F0 = face_flux(kl, b[0], pl, a[0], c.ix0, c.ix1, c.gx0, c.gx1);
+      const double F1 = face_flux(b[0], b[1], a[0], a[1], c.ix0, c.ix1, c.gx0, c.gx1);
+      const double F2 = face_flux(b[1], kr, a[1], pr, c.ix0, c.ix1, c.gx0, c.gx1);
+      const double Dx[2] = {F0 * c.dx0 + F1 * c.dx1, F1 * c.dx0 + F2 * c.dx1};
+      D2 v;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double fy_hi = face_flux(b[h], C0.k[j + 2][h], a[h], C0.p[j + 2][h], c.iy0, c.iy1, c.gy0, c.gy1);
+        const double fz_hi = face_flux(b[h], C1.k[j + 1][h], a[h], C1.p[j + 1][h], c.iz0, c.iz1, c.gz0, c.gz1);
+        const double Dy = fy_lo[h] * c.dy0 + fy_hi * c.dy1;
+        const double Dz = fz[j][h] * c.dz0 + fz_hi * c.dz1;
+        fz[j][h] = fz_hi;
+        fy_lo[h] = fy_hi;
+        v[h] = divk_combine<FORM>(Dx[h], Dy, Dz, a[h], dt);
+      }
+      if (y0 + j < T.hi[1]) store2_nt_at(rb, g.row[j + 1], v);
+    }
+    S[jj] = load(min(z + NS, z1));      // plane z is consumed: its slot takes z+NS
+  };
+  // whole periods of NS steps, then the steps left over
+  const int full = (z1 - z0) / NS, rem = (z1 - z0) - NS * full;
+  int z = z0;
+  for (int q = 0; q < full; ++q, z += NS)
+    unroll_steps([&](auto J) { step(J, z + decltype(J)::value); }, std::make_integer_sequence<int, NS>{});
+  unroll_steps([&](auto J) { if (rem > decltype(J)::value) step(J, z + decltype(J)::value); },
+               std::make_integer_sequence<int, NS - 1>{});
+}
+
 // ---- phi + s*lapl(phi) -----------------------------------------------------------
 
 // out = phi + s*((Lx + Ly) + Lz),  L_d = (phi[-1]*g0 + phi[0]*g1)*d0 + (phi[0]*g0 + phi[+1]*g1)*d1
@@ -558,7 +666,11 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
 // 5, 6: column schedule with x-face reuse, 4 resp. 2 rows per lane, ty/4 resp.
 //    ty/2 waves stacked in y per workgroup (ty = the workgroup's rows);
 // 7, 8: as 5 with loads issued 3 resp. 4 planes ahead instead of 2;
-// 9: as 7 with write-back stores instead of nontemporal ones.
+// 9: as 7 with write-back stores instead of nontemporal ones;
+// 10, 11, 12: pair columns (two adjacent columns per lane, 16-byte accesses,
+//    tx = 128 x waves side by side) with 4, 2, 4 rows per lane and loads 3, 3, 2
+//    planes ahead, ty/rows waves stacked in y (div(k grad phi); the Laplacian
+//    runs its column schedule over the same tile).
 struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
     {64, 4, 16, 0},   {64, 8, 16, 0},
@@ -574,7 +686,9 @@ constexpr TileShape kShapes[] = {
     {128, 8, 128, 7}, {128, 8, 256, 7}, {128, 8, 512, 7},
     {128, 8, 64, 8},
     {128, 16, 32, 7}, {128, 16, 64, 7}, {256, 8, 64, 7},
-    {128, 8, 64, 9},  {128, 8, 512, 9}};
+    {128, 8, 64, 9},  {128, 8, 512, 9},
+    {128, 8, 64, 10}, {128, 8, 32, 10}, {256, 8, 32, 10}, {128, 16, 32, 10}, {128, 8, 512, 10},
+    {128, 4, 64, 11}, {128, 8, 64, 11}, {128, 8, 64, 12}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // The default tile (rule 4 of select_shape): 128 x 8 rows (2 x 2 waves)
 // marching 64 planes with loads 3 planes ahead, the fastest at 256^3 on MI355X
@@ -594,22 +708,32 @@ static_assert(kDefaultShape >= 0 && kShapes[kFallbackShape].variant == 0, "defau
 // Rows per lane and waves stacked in y of a column tile.
 constexpr int rows_per_lane(const TileShape& sh)
 {
-  return (sh.variant == 5 || sh.variant == 7 || sh.variant == 8 || sh.variant == 9) ? 4 : sh.variant == 6 ? 2 : sh.ty;
+  return (sh.variant == 5 || sh.variant == 7 || sh.variant == 8 || sh.variant == 9 || sh.variant == 10 ||
+          sh.variant == 12)
+             ? 4
+         : (sh.variant == 6 || sh.variant == 11) ? 2
+                                                 : sh.ty;
 }
 // Planes of load prefetch of a column tile.
 constexpr int prefetch_planes(const TileShape& sh)
 {
-  return (sh.variant == 7 || sh.variant == 9) ? 3 : sh.variant == 8 ? 4 : 2;
+  return (sh.variant == 7 || sh.variant == 9 || sh.variant == 10 || sh.variant == 11) ? 3 : sh.variant == 8 ? 4 : 2;
 }
+// Pair-column tiles: two columns per lane, 128 columns per wave.
+constexpr bool pair_columns(const TileShape& sh) { return sh.variant >= 10; }
 // Cache policy of a column tile's stores (2 nontemporal, 0 write-back).
 constexpr int store_policy(const TileShape& sh) { return sh.variant == 9 ? 0 : 2; }
 constexpr int stacked_waves(const TileShape& sh) { return sh.ty / rows_per_lane(sh); }
 
-// Launch geometry of one tile: rows per tile in y and threads per workgroup.
+// Launch geometry of one tile: rows per tile in y and threads per workgroup
+// (kernel 0 div(k grad phi), 1 Laplacian + Euler; the Laplacian runs pair tiles
+// on its one-column schedule, tx / 64 waves side by side).
 struct Geometry { int tile_y, threads; };
-constexpr Geometry tile_geometry(const TileShape& sh)
+constexpr Geometry tile_geometry(const TileShape& sh, int kernel = 0)
 {
-  return sh.variant >= 3 ? Geometry{sh.ty, sh.tx * stacked_waves(sh)} : Geometry{sh.ty, sh.tx * sh.ty};
+  return (pair_columns(sh) && kernel == 0) ? Geometry{sh.ty, sh.tx / 2 * stacked_waves(sh)}
+         : sh.variant >= 3                 ? Geometry{sh.ty, sh.tx * stacked_waves(sh)}
+                                           : Geometry{sh.ty, sh.tx * sh.ty};
 }
 
 using DivkFn = void (*)(double*, const double*, const double*, Tiles, int, int, nebo_axis_coefs_t, double);
@@ -621,7 +745,11 @@ constexpr DivkFn divk_fn()
   constexpr TileShape sh = kShapes[S];
   static_assert(sh.variant >= 3 || sh.tx == kTx, "only the column schedule spans several waves in x");
   static_assert(sh.ty % rows_per_lane(sh) == 0 && tile_geometry(sh).threads <= 1024, "whole waves, <= 1024 threads");
+  static_assert(!pair_columns(sh) || sh.tx % (2 * kTx) == 0, "pair tiles hold whole 128-column waves");
   if constexpr (sh.variant == 0) return divkgrad_kernel<FORM, sh.ty, sh.zc>;
+  else if constexpr (pair_columns(sh))
+    return divkgrad_pair_kernel<FORM, sh.tx / (2 * kTx), stacked_waves(sh), rows_per_lane(sh), sh.zc,
+                                prefetch_planes(sh)>;
   else if constexpr (sh.variant >= 3)
     return divkgrad_column_kernel<FORM, sh.tx / kTx, stacked_waves(sh), rows_per_lane(sh), sh.zc, sh.variant != 3,
                                   prefetch_planes(sh), store_policy(sh)>;
@@ -725,9 +853,13 @@ int find_shape(int tx, int ty, int zc, int variant)
 
 // The column schedule addresses a plane through a buffer range of
 // (XY + 2X + 2)*8 bytes, which must stay below 2^31; planes that large run the
-// default tile instead.
-int usable_shape(int shape, int X, int XY)
+// direct schedule instead.  Pair tiles need a box width that is a multiple of
+// their width (every lane's two columns inside the box); other boxes run the
+// default tile.  (kernel 1, the Laplacian, runs pair tiles on its one-column
+// schedule, which covers any width.)
+int usable_shape(int shape, int X, int XY, int width, int kernel)
 {
+  if (kernel == 0 && pair_columns(kShapes[shape]) && width % kShapes[shape].tx != 0) shape = kDefaultShape;
   if (kShapes[shape].variant >= 3 && ((long)XY + 2L * X + 2) * 8 >= (1L << 31)) return kFallbackShape;
   return shape;
 }
@@ -735,13 +867,16 @@ int usable_shape(int shape, int X, int XY)
 // "div(k grad phi) kernel, tile 128x8x64, column schedule, 4 rows per lane, 2x2
 // waves, loads 3 planes ahead" (built once per kernel and tile: launches of a
 // time loop only copy it)
-std::string tile_backend_text(const char* kernel, int shape)
+std::string tile_backend_text(const char* kernel, int kind, int shape)
 {
   const TileShape sh = kShapes[shape];
   std::ostringstream o;
   o << kernel << ", tile " << sh.tx << "x" << sh.ty << "x" << sh.zc;
   if (sh.variant == 0) o << ", direct loads";
   else if (sh.variant <= 2) o << ", LDS-staged rows, " << sh.variant << " planes of prefetch";
+  else if (pair_columns(sh) && kind != 2)
+    o << ", pair columns (16-byte accesses), " << rows_per_lane(sh) << " rows per lane, " << sh.tx / (2 * kTx)
+      << "x" << stacked_waves(sh) << " waves, loads " << prefetch_planes(sh) << " planes ahead";
   else
     o << ", column schedule, " << rows_per_lane(sh) << " rows per lane, " << sh.tx / kTx << "x"
       << stacked_waves(sh) << " waves, loads " << prefetch_planes(sh) << " planes ahead"
@@ -756,7 +891,7 @@ const std::string& tile_backend(int kind, int shape)
   static const std::vector<std::string>* table = [] {
     auto* t = new std::vector<std::string>;
     for (int k = 0; k < 3; ++k)
-      for (int sh = 0; sh < kNumShapes; ++sh) t->push_back(tile_backend_text(names[k], sh));
+      for (int sh = 0; sh < kNumShapes; ++sh) t->push_back(tile_backend_text(names[k], k, sh));
     return t;
   }();
   return (*table)[kind * kNumShapes + shape];
@@ -766,9 +901,9 @@ int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_fie
                       const nebo_field_t* phi, const nebo_axis_coefs_t& c, double dt,
                       const nebo_limits_t& lim, void* stream)
 {
-  shape = usable_shape(shape, r->glob[0], r->glob[0] * r->glob[1]);
+  shape = usable_shape(shape, r->glob[0], r->glob[0] * r->glob[1], lim.hi[0] - lim.lo[0], 0);
   const TileShape sh = kShapes[shape];
-  const Geometry geo = tile_geometry(sh);
+  const Geometry geo = tile_geometry(sh, 0);
   Tiles T = make_tiles(lim, sh.tx, geo.tile_y, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
@@ -781,9 +916,9 @@ int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_fie
 int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* phi,
                       const nebo_axis_coefs_t& c, double sc, const nebo_limits_t& lim, void* stream)
 {
-  shape = usable_shape(shape, out->glob[0], out->glob[0] * out->glob[1]);
+  shape = usable_shape(shape, out->glob[0], out->glob[0] * out->glob[1], lim.hi[0] - lim.lo[0], 1);
   const TileShape sh = kShapes[shape];
-  const Geometry geo = tile_geometry(sh);
+  const Geometry geo = tile_geometry(sh, 1);
   Tiles T = make_tiles(lim, sh.tx, geo.tile_y, sh.zc);
   const int nb = T.nx * T.ny * T.nz;
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];

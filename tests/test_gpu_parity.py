@@ -776,3 +776,43 @@ def test_backend_reported_per_assignment(capfd):
     assert "Starting Nebo HIP div(k grad phi) kernel, tile " in printed
     assert "Finished Nebo HIP stack-program interpreter" in printed
     assert printed.count("Starting Nebo HIP") == printed.count("Finished Nebo HIP") >= 5
+
+
+@pytest.mark.parametrize("n,ghosts", [((128, 37, 45), 1), ((256, 21, 9), 1), ((126, 30, 20), 2)])
+def test_pair_column_tiles_bit_exact(n, ghosts):
+    """The pair-column tiles (two adjacent columns per lane, 16-byte loads and
+    stores) run on boxes whose assigned width is a multiple of their 128- or
+    256-column tile: each one pinned, both forms and the Euler form, rows and
+    planes cut by the box end, and (two ghost layers, width 126 + 2) the
+    ghost-inclusive region.  Every result equals the oracle's walk bit for bit."""
+    L = nh.Layout.build(n, ghosts)
+    rng = np.random.default_rng(sum(n) + ghosts)
+    k = both(L, rng.uniform(0.5, 2.0, L.npts))
+    phi = both(L, rng.standard_normal(L.npts))
+    _, ops = _ops(n)
+    refs = {}
+    for form in (0, 1):
+        ref = nh.Field.from_numpy(L, np.full(L.npts, 2.0), where="cpu")
+        orc.assign(ref, _divkgrad_tree(ops, k, phi, form))
+        refs[form] = ref.host
+    pair_tiles = [t for t in nh.stencil_tiles() if t[3] >= 10]
+    assert pair_tiles
+    try:
+        for tile in pair_tiles:
+            print(f"pair tile {tile} on {n}, {ghosts} ghost layers", flush=True)
+            nh.stencil_set_tile(0, *tile)
+            for form in (0, 1):
+                out = gpu_empty_like(L, 2.0)
+                nh.assign(out, _divkgrad_tree(ops, k, phi, form))
+                assert np.array_equal(fetch(out), refs[form]), (tile, form)
+            if ghosts == 1:   # the Euler form (diffusion step, one periodic ghost layer)
+                from nebo_hip import adjoint as adj
+                c = nh.axis_coefs(nh.build_stencils(*n, 1.0, 1.0, 1.0))
+                ro = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+                orc.diffusion_step(ro, nh.Field(L, host=phi.host), nh.Field(L, host=k.host), c, 1e-4)
+                o = torch.zeros_like(phi.gpu)
+                adj.diffusion_step(L, o, phi.gpu, k.gpu, c, 1e-4)
+                torch.cuda.synchronize()
+                assert np.array_equal(o.cpu().numpy(), ro.host), (tile, "euler")
+    finally:
+        nh.stencil_reset_tile(0)
