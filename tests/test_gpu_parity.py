@@ -805,13 +805,14 @@ def test_pair_column_tiles_bit_exact(n, ghosts):
                 out = gpu_empty_like(L, 2.0)
                 nh.assign(out, _divkgrad_tree(ops, k, phi, form))
                 assert np.array_equal(fetch(out), refs[form]), (tile, form)
-            if ghosts == 1:   # the Euler form (diffusion step, one periodic ghost layer)
-                from nebo_hip import adjoint as adj
+            if ghosts == 1:   # the Euler form (diffusion step, periodic fields, one ghost layer)
+                from nebo_hip import adjoint as adj, inputs
                 c = nh.axis_coefs(nh.build_stencils(*n, 1.0, 1.0, 1.0))
+                pk, pp = (inputs.periodic_fill(a.host.reshape(L.glob[::-1]).copy(), 1).reshape(-1) for a in (k, phi))
                 ro = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
-                orc.diffusion_step(ro, nh.Field(L, host=phi.host), nh.Field(L, host=k.host), c, 1e-4)
+                orc.diffusion_step(ro, nh.Field(L, host=pp), nh.Field(L, host=pk), c, 1e-4)
                 o = torch.zeros_like(phi.gpu)
-                adj.diffusion_step(L, o, phi.gpu, k.gpu, c, 1e-4)
+                adj.diffusion_step(L, o, torch.from_numpy(pp).cuda(), torch.from_numpy(pk).cuda(), c, 1e-4)
                 torch.cuda.synchronize()
                 assert np.array_equal(o.cpu().numpy(), ro.host), (tile, "euler")
     finally:
