@@ -387,7 +387,14 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  * ty/2 such waves in y in one workgroup (4 resp. 2 rows per lane), so that a
  * wave's halo rows are its neighbours' own rows, loaded on the same CU; variants
  * 7 and 8 are variant 5 with loads issued 3 resp. 4 planes ahead (the Laplacian
- * kernel runs variant 3 with the same rows per lane for 5-8).  kernel 0 = the
+ * kernel runs variant 3 with the same rows per lane for 5-8); variant 9 is 7 with
+ * write-back instead of nontemporal stores.  Variants 10-16 (pair columns,
+ * div(k grad phi) only; the Laplacian runs its column schedule over the same
+ * tile) give each lane two adjacent columns, so that every load and store moves
+ * 16 bytes and a wave covers 128 columns (tx a multiple of 128; boxes whose width
+ * is not a multiple of tx run the rule-4 tile instead), with rows per lane and
+ * planes of prefetch 10: 4, 3; 11: 2, 3; 12: 4, 2; 13: 2, 4; 14: 2, 2; 15: 1, 3;
+ * 16: 1, 4.  kernel 0 = the
  * var-coefficient diffusion kernels (nebo_divkgrad, nebo_diffusion_step), 1 =
  * Laplacian+Euler.  Which tile a launch uses:
  *   1. the tile pinned by nebo_stencil_set_tile, for every box;
@@ -395,8 +402,9 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  *      dimensions on the same device;
  *   3. else 128 x 8 tiles, variant 7, with the longest z chunk of 512, 256 or
  *      128 planes that gives every CU a workgroup (512^3: 512 planes);
- *   4. else 128 x 8 x 64, variant 7, when it gives at least half the device's
- *      CUs a workgroup (256^3);
+ *   4. else 128 x 8 x 64, variant 11 for kernel 0 on boxes whose width is a
+ *      multiple of 128 and variant 7 otherwise, when it gives at least half the
+ *      device's CUs a workgroup (256^3);
  *   5. else the direct schedule with 64 x 8 x 16 tiles, or 64 x 4 x 16 when those
  *      give fewer workgroups than CUs (small boxes).
  * nebo_stencil_set_tile with tx = ty = zchunk = variant = 0 removes the pin.

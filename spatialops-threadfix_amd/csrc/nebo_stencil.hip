@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
                std::make_integer_sequence<int, NS - 1>{});
 }
 
-// Pair-column schedule (variants 10-12): the column schedule with two adjacent
+// Pair-column schedule (variants 10-16): the column schedule with two adjacent
 // columns per lane, so every load and store moves 16 bytes and a wave covers
 // 128 columns (W waves side by side, NW stacked in y, RY rows per lane, loads PF
 // planes ahead).  A lane holding columns x, x+1 takes x-1 from the lower lane's
@@ -667,10 +667,13 @@ __global__ __launch_bounds__(kTx * W * NW) void lapl_euler_column_kernel(
 //    ty/2 waves stacked in y per workgroup (ty = the workgroup's rows);
 // 7, 8: as 5 with loads issued 3 resp. 4 planes ahead instead of 2;
 // 9: as 7 with write-back stores instead of nontemporal ones;
-// 10, 11, 12: pair columns (two adjacent columns per lane, 16-byte accesses,
-//    tx = 128 x waves side by side) with 4, 2, 4 rows per lane and loads 3, 3, 2
-//    planes ahead, ty/rows waves stacked in y (div(k grad phi); the Laplacian
-//    runs its column schedule over the same tile).
+// 10-16: pair columns (two adjacent columns per lane, 16-byte accesses,
+//    tx = 128 x waves side by side), ty/rows waves stacked in y, with rows per
+//    lane and planes of load prefetch (kPairRows, kPairPrefetch):
+//      10: 4 rows, 3 planes   11: 2, 3   12: 4, 2   13: 2, 4   14: 2, 2
+//      15: 1, 3               16: 1, 4
+//    (div(k grad phi); the Laplacian runs its column schedule over the same
+//    tile with the same rows per lane).
 struct TileShape { int tx, ty, zc, variant; };
 constexpr TileShape kShapes[] = {
     {64, 4, 16, 0},   {64, 8, 16, 0},
@@ -688,7 +691,10 @@ constexpr TileShape kShapes[] = {
     {128, 16, 32, 7}, {128, 16, 64, 7}, {256, 8, 64, 7},
     {128, 8, 64, 9},  {128, 8, 512, 9},
     {128, 8, 64, 10}, {128, 8, 32, 10}, {256, 8, 32, 10}, {128, 16, 32, 10}, {128, 8, 512, 10},
-    {128, 4, 64, 11}, {128, 8, 64, 11}, {128, 8, 64, 12}};
+    {128, 4, 64, 11}, {128, 8, 64, 11}, {128, 8, 64, 12},
+    {128, 8, 32, 11}, {256, 4, 64, 11}, {256, 8, 32, 11}, {128, 16, 32, 11}, {128, 8, 512, 11},
+    {128, 8, 64, 13}, {256, 4, 64, 13}, {128, 8, 64, 14}, {128, 8, 64, 15}, {128, 4, 64, 15},
+    {128, 8, 64, 16}, {256, 4, 64, 14}};
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 // The default tile (rule 4 of select_shape): 128 x 8 rows (2 x 2 waves)
 // marching 64 planes with loads 3 planes ahead, the fastest at 256^3 on MI355X
@@ -701,26 +707,36 @@ constexpr int find_shape_c(int tx, int ty, int zc, int variant, int s = 0)
              ? s
              : find_shape_c(tx, ty, zc, variant, s + 1);
 }
+// Pair-column tiles: two columns per lane, 128 columns per wave.
+constexpr bool pair_columns(const TileShape& sh) { return sh.variant >= 10; }
+constexpr int kPairRows[] = {4, 2, 4, 2, 2, 1, 1};       // variants 10 ..
+constexpr int kPairPrefetch[] = {3, 3, 2, 4, 2, 3, 4};
 constexpr int kDefaultShape = find_shape_c(128, 8, 64, 7);
+// div(k grad phi) on boxes whose width is a multiple of 128: the same tile as
+// pair columns, 2 rows per lane (4 waves stacked), loads 3 planes ahead (256^3 on
+// MI355X: 66-69 us against 70-73 us for the default tile).
+constexpr int kDefaultPairShape = find_shape_c(128, 8, 64, 11);
 constexpr int kFallbackShape = 0;
-static_assert(kDefaultShape >= 0 && kShapes[kFallbackShape].variant == 0, "default tiles");
+static_assert(kDefaultShape >= 0 && kDefaultPairShape >= 0 && kShapes[kFallbackShape].variant == 0 &&
+                  !pair_columns(kShapes[kDefaultShape]),
+              "default tiles");
 
 // Rows per lane and waves stacked in y of a column tile.
 constexpr int rows_per_lane(const TileShape& sh)
 {
-  return (sh.variant == 5 || sh.variant == 7 || sh.variant == 8 || sh.variant == 9 || sh.variant == 10 ||
-          sh.variant == 12)
-             ? 4
-         : (sh.variant == 6 || sh.variant == 11) ? 2
-                                                 : sh.ty;
+  return pair_columns(sh) ? kPairRows[sh.variant - 10]
+         : (sh.variant == 5 || sh.variant == 7 || sh.variant == 8 || sh.variant == 9) ? 4
+         : sh.variant == 6                                                              ? 2
+                                                                                        : sh.ty;
 }
 // Planes of load prefetch of a column tile.
 constexpr int prefetch_planes(const TileShape& sh)
 {
-  return (sh.variant == 7 || sh.variant == 9 || sh.variant == 10 || sh.variant == 11) ? 3 : sh.variant == 8 ? 4 : 2;
+  return pair_columns(sh)                        ? kPairPrefetch[sh.variant - 10]
+         : (sh.variant == 7 || sh.variant == 9) ? 3
+         : sh.variant == 8                       ? 4
+                                                 : 2;
 }
-// Pair-column tiles: two columns per lane, 128 columns per wave.
-constexpr bool pair_columns(const TileShape& sh) { return sh.variant >= 10; }
 // Cache policy of a column tile's stores (2 nontemporal, 0 write-back).
 constexpr int store_policy(const TileShape& sh) { return sh.variant == 9 ? 0 : 2; }
 constexpr int stacked_waves(const TileShape& sh) { return sh.ty / rows_per_lane(sh); }
@@ -798,7 +814,8 @@ LaplFn lapl_table(int s) { return lapl_pick(s, std::make_index_sequence<kNumShap
 //     planes (512^3 on MI355X: 658 us against 775 us with 64-plane chunks, whose
 //     eight concurrent z slabs lie 135 MB apart);
 //  4. else the default tile when it gives at least half the CUs a workgroup
-//     (256^3: 64-plane chunks, one workgroup per CU);
+//     (256^3: 64-plane chunks, one workgroup per CU), for div(k grad phi) on
+//     boxes whose width is a multiple of 128 as pair columns;
 //  5. smaller boxes run the direct schedule, whose short tiles fill the GPU
 //     (64^3 8.3 us, 128^3 14.5 us against 51 us for the default tile, which
 //     gives them 32 workgroups).
@@ -839,7 +856,8 @@ int select_shape(int kernel, int device, const nebo_limits_t& lim)
   const int cus = device_cu_count(device);
   for (int sh : kLongChunks)
     if (tile_groups(sh, lim) >= cus) return sh;
-  if (2 * tile_groups(kDefaultShape, lim) >= cus) return kDefaultShape;
+  if (2 * tile_groups(kDefaultShape, lim) >= cus)
+    return kernel == 0 && n[0] % kShapes[kDefaultPairShape].tx == 0 ? kDefaultPairShape : kDefaultShape;
   return tile_groups(kSmallWide, lim) >= cus ? kSmallWide : kSmallNarrow;
 }
 

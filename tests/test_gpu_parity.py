@@ -517,25 +517,30 @@ def test_hand_kernels_two_ghost_layers_every_tile():
 
 def test_hand_kernels_two_ghost_layers_default_tile():
     """A 254^3 interior with two ghost layers: the ghost-inclusive region is 256^3,
-    for which the box-size rule picks the default column tile (128 x 8 rows, 64-plane
-    chunks); the interior-only region (254^3) is checked too."""
+    for which the box-size rule picks the default tile as pair columns (128 x 8
+    rows, 64-plane chunks); the interior-only region (254^3, a width that is not
+    a multiple of 128) runs the default column tile."""
     n = (254, 254, 254)
     print("two ghost layers, 254^3 default tile", flush=True)
-    assert nh.stencil_select_tile(0, (256, 256, 256)) == (128, 8, 64, 7)
+    assert nh.stencil_select_tile(0, (256, 256, 256)) == (128, 8, 64, 11)
+    assert nh.stencil_select_tile(0, (254, 254, 254)) == (128, 8, 64, 7)
     L, trees, refs = _two_ghost_case(n, 22)
     _check_hand_path(L, trees, refs, "default")
 
 
 def test_tile_selection_by_box_size():
     """Without a pinned or tuned tile, boxes that the default tile fills (at
-    least half the CUs get a workgroup) run it, smaller ones the direct
-    schedule; a pin applies to every box; a tuned tile to its box size only."""
+    least half the CUs get a workgroup) run it (div(k grad phi) as pair columns
+    where the width is a multiple of 128), smaller ones the direct schedule; a
+    pin applies to every box; a tuned tile to its box size only."""
     nh.stencil_reset_tile(0)
     nh.stencil_reset_tile(1)
     default = (128, 8, 64, 7)
     try:
+        assert nh.stencil_select_tile(0, (256, 256, 256)) == (128, 8, 64, 11)
+        assert nh.stencil_select_tile(0, (250, 256, 256)) == default
         for kern in (0, 1):
-            assert nh.stencil_select_tile(kern, (256, 256, 256)) == default
+            assert nh.stencil_select_tile(kern, (256, 256, 256)) == ((128, 8, 64, 11) if kern == 0 else default)
             assert nh.stencil_select_tile(kern, (512, 512, 512)) == (128, 8, 512, 7)
             assert nh.stencil_select_tile(kern, (384, 384, 384)) == (128, 8, 256, 7)
             assert nh.stencil_select_tile(kern, (128, 128, 128)) == (64, 8, 16, 0)
@@ -778,13 +783,16 @@ def test_backend_reported_per_assignment(capfd):
     assert printed.count("Starting Nebo HIP") == printed.count("Finished Nebo HIP") >= 5
 
 
-@pytest.mark.parametrize("n,ghosts", [((128, 37, 45), 1), ((256, 21, 9), 1), ((126, 30, 20), 2)])
+@pytest.mark.parametrize("n,ghosts", [((128, 37, 45), 1), ((256, 21, 9), 1), ((126, 30, 20), 2),
+                                      ((128, 6, 150), 1)])
 def test_pair_column_tiles_bit_exact(n, ghosts):
     """The pair-column tiles (two adjacent columns per lane, 16-byte loads and
     stores) run on boxes whose assigned width is a multiple of their 128- or
     256-column tile: each one pinned, both forms and the Euler form, rows and
     planes cut by the box end, and (two ghost layers, width 126 + 2) the
-    ghost-inclusive region.  Every result equals the oracle's walk bit for bit."""
+    ghost-inclusive region.  150 planes give the tiles that walk odd z chunks
+    downwards several chunks, the last one short.  Every result equals the
+    oracle's walk bit for bit."""
     L = nh.Layout.build(n, ghosts)
     rng = np.random.default_rng(sum(n) + ghosts)
     k = both(L, rng.uniform(0.5, 2.0, L.npts))
