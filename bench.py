@@ -175,12 +175,35 @@ def batched_leg(steps=10, count=512):
         stream.synchronize()
         per_patch_ms = e0.elapsed_time(e1) / steps
     B.close()
+    # a step reads every patch block whole (ghosts included) and writes its
+    # interior; the steps alternate a -> b -> a, so each step reads what the one
+    # before wrote.  The streaming rate of that pattern: device copies between two
+    # buffers of the blocks' total size, alternating direction.
+    ghosted = sum((d[0] + 2) * (d[1] + 2) * (d[2] + 2) for d in dims)
+    min_bytes = 8 * ghosted + 8 * cells
+    del a, b
+    x = torch.randn(ghosted, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    e0.record()
+    for i in range(20):
+        (y if i % 2 == 0 else x).copy_(x if i % 2 == 0 else y)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_GBps = 16 * ghosted / (e0.elapsed_time(e1) / 20 * 1e-3) / 1e9
+    del x, y
     return {"patches": count, "cells_per_step": cells, "tile": list(tile),
             "graph_ms_per_step": round(graph_ms, 5), "persistent_ms_per_step": round(eager_ms, 5),
             "per_patch_launch_ms_per_step": round(per_patch_ms, 5),
             "speedup_vs_per_patch": round(per_patch_ms / graph_ms, 2),
             "gcells_per_s": round(cells / (graph_ms * 1e-3) / 1e9, 2),
-            "hbm_frac": round(16 * cells / (graph_ms * 1e-3) / HBM_PEAK, 4)}
+            "hbm_frac": round(16 * cells / (graph_ms * 1e-3) / HBM_PEAK, 4),
+            "min_traffic_bytes": min_bytes,
+            "min_traffic_GBps": round(min_bytes / (graph_ms * 1e-3) / 1e9, 1),
+            "alternating_copy_GBps": round(copy_GBps, 1),
+            "frac_of_alternating_copy": round(min_bytes / (graph_ms * 1e-3) / 1e9 / copy_GBps, 4)}
 
 
 def adjoint_leg(n=128, steps=100, every=10):

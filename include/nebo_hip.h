@@ -463,8 +463,12 @@ typedef struct nebo_batch nebo_batch_t;   /* opaque */
  * current device.  tile[3] is the (x, y, z) extent of one work item (x*y <= 1024
  * threads), or with y = 0 a flattened item of x consecutive points of a patch's
  * x-y plane (x a multiple of 64 up to 1024) by z planes, which leaves no lane
- * idle on patches narrower than the tile; tile == NULL starts from (64, 4, 16).  Coefficients and s are shared
- * by all patches: out = in + s*((Lx + Ly) + Lz) as in nebo_lapl_euler. */
+ * idle on patches narrower than the tile, or with x = 0 the column schedule: one
+ * wave per item covering up to 64 columns of a patch (64/nx row groups side by
+ * side on patches narrower than 64), y = 1, 2 or 4 rows per lane, z planes per
+ * item, x neighbours exchanged between lanes; tile == NULL starts from
+ * (64, 4, 16).  Coefficients and s are shared by all patches:
+ * out = in + s*((Lx + Ly) + Lz) as in nebo_lapl_euler. */
 int nebo_batch_create(const nebo_patch_t* patches, int32_t n,
                       const nebo_axis_coefs_t* c, double s,
                       const int32_t* tile, nebo_batch_t** out);

@@ -13,6 +13,7 @@
 // nebo_lapl_euler (out = in + s*((Lx + Ly) + Lz)), bit for bit.
 
 #include "nebo_internal.h"
+#include "nebo_device.h"
 
 #include <hip/hip_runtime.h>
 
@@ -115,6 +116,125 @@ __global__ __launch_bounds__(1024) void batch_flat_kernel(const DevPatch* __rest
   }
 }
 
+// Column items (tile x == 0, y = RY rows per lane, z = planes per item): one
+// wave per item w = (patch, x0, y0, z0).  The wave covers a strip of up to 64
+// columns of the patch from x0; a patch narrower than 64 columns puts
+// G = min(64 / nx, 32 / RY) groups of RY rows side by side (lane = x + nx * g),
+// so a 16-wide patch keeps every lane busy.  Each lane walks RY rows of its
+// column through the item's planes, planes PF ahead held in registers: y
+// neighbours are its own rows (plus one row above and below), z neighbours the
+// neighbouring planes, x neighbours the adjacent lanes' values (DPP).  The
+// values left and right of every group's rows (2 G RY <= 64 of them per plane)
+// come from one load per plane, lane k loading value k, and reach the lanes at
+// the groups' ends through ds_bpermute.  Loads and stores address a plane
+// through a buffer resource (uniform base) and per-lane 32-bit offsets.  Four
+// waves per workgroup, each its own item; the hardware hands workgroups to CUs as
+// they free up, so uneven patches balance by themselves.  Workgroups are
+// renumbered so that consecutive items run on one XCD.  Cells use
+// patch_column's operations in its order.
+__host__ __device__ constexpr int column_groups(int nx, int ry)
+{
+  return nx < kLanes ? (kLanes / nx < 32 / ry ? kLanes / nx : 32 / ry) : 1;
+}
+
+__device__ __forceinline__ double lane_gather(double v, int src)
+{
+  const int lo = __builtin_amdgcn_ds_bpermute(src * 4, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(src * 4, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+
+template <int RY, int PF>
+__global__ __launch_bounds__(256) void batch_column_kernel(const DevPatch* __restrict__ P,
+                                                           const int4* __restrict__ items, int n_items, int zc,
+                                                           Coef6 c, double s, int swap)
+{
+  static_assert(RY == 1 || RY == 2 || RY == 4, "rows per lane");
+  // consecutive items (neighbouring rows, strips and chunks of a patch, which
+  // share halo rows and planes) go to workgroups on the same XCD (one L2)
+  const int it = __builtin_amdgcn_readfirstlane(xcd_tile((int)blockIdx.x, (int)gridDim.x) * 4 +
+                                                (int)(threadIdx.x >> 6));
+  if (it >= n_items) return;   // the whole wave: an item is one wave
+  const int lane = (int)threadIdx.x & (kLanes - 1);
+  const int4 w = items[it];
+  const DevPatch p = P[w.x];
+  const double* in = swap ? p.b : p.a;
+  double* out = swap ? p.a : p.b;
+  const int width = min(kLanes, p.nx - w.y);     // columns of this strip (== nx when G > 1)
+  const int G = column_groups(p.nx, RY);
+  const int g0 = lane / width;
+  const bool active = g0 < G;
+  const int g = min(g0, G - 1);
+  const int xl = active ? lane - g * width : width - 1;
+  const bool left = xl == 0, right = xl == width - 1;
+  const int x = w.y + xl;
+  const int yb = w.z + g * RY;                   // the lane's first row
+  const int z0 = w.w, z1 = min(z0 + zc, p.nz);
+  // byte offsets within a plane, from its point (-1, -1)
+  auto off = [&](int xx, int yy) { return (uint32_t)((xx + 1) + p.X * (yy + 1)) * 8u; };
+  uint32_t row[RY + 2];                          // rows yb-1 .. yb+RY, clamped to the ghost row
+#pragma unroll
+  for (int r = 0; r < RY + 2; ++r) row[r] = off(x, min(yb - 1 + r, p.ny));
+  uint32_t edge;                                 // lane k < 2 G RY: value k = (group, row, side)
+  {
+    const int k = lane < 2 * G * RY ? lane : 0;
+    const int ge = k / (2 * RY), je = (k >> 1) % RY;
+    edge = off((k & 1) ? w.y + width : w.y - 1, min(w.z + ge * RY + je, p.ny));
+  }
+  const int src0 = 2 * g * RY;                   // this lane's group's first edge value
+
+  auto plane = [&](const double* base, int z) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base) + (long)p.XY * (z + 1), (short)0, p.XY * 8,
+                                             0x00020000);
+  };
+  struct Plane { double v[RY + 2], e; };
+  auto load = [&](int z) {
+    Plane q;
+    const Rsrc rs = plane(in, z);
+#pragma unroll
+    for (int r = 0; r < RY + 2; ++r) q.v[r] = load_at(rs, row[r]);
+    q.e = load_at(rs, edge);
+    return q;
+  };
+  constexpr int NS = PF + 1;
+  Plane S[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) S[q] = load(min(z0 + q, z1));
+  double gz[RY];
+  {
+    const Rsrc rs = plane(in, z0 - 1);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) gz[j] = load_at(rs, row[j + 1]) * c.gz0 + S[0].v[j + 1] * c.gz1;
+  }
+  auto step = [&](auto JJ, const int z) {
+    constexpr int jj = decltype(JJ)::value;
+    const Plane& C0 = S[jj];
+    const Plane& C1 = S[(jj + 1) % NS];
+    const Rsrc ro = plane(out, z);
+#pragma unroll
+    for (int j = 0; j < RY; ++j) {
+      const double pc = C0.v[j + 1];
+      const double lo = from_lower_lane(pc), hi = from_upper_lane(pc);
+      const double el = lane_gather(C0.e, src0 + 2 * j), er = lane_gather(C0.e, src0 + 2 * j + 1);
+      const double pxm = left ? el : lo, pxp = right ? er : hi;
+      const double pym = C0.v[j], pyp = C0.v[j + 2];
+      const double gz_hi = pc * c.gz0 + C1.v[j + 1] * c.gz1;
+      const double Lx = (pxm * c.gx0 + pc * c.gx1) * c.dx0 + (pc * c.gx0 + pxp * c.gx1) * c.dx1;
+      const double Ly = (pym * c.gy0 + pc * c.gy1) * c.dy0 + (pc * c.gy0 + pyp * c.gy1) * c.dy1;
+      const double Lz = gz[j] * c.dz0 + gz_hi * c.dz1;
+      gz[j] = gz_hi;
+      if (active && yb + j < p.ny) store_nt_at(ro, row[j + 1], pc + s * ((Lx + Ly) + Lz));
+    }
+    S[jj] = load(min(z + NS, z1));
+  };
+  const int full = (z1 - z0) / NS, rem = (z1 - z0) - NS * full;
+  int z = z0;
+  for (int q = 0; q < full; ++q, z += NS)
+    unroll_steps([&](auto J) { step(J, z + decltype(J)::value); }, std::make_integer_sequence<int, NS>{});
+  unroll_steps([&](auto J) { if (rem > decltype(J)::value) step(J, z + decltype(J)::value); },
+               std::make_integer_sequence<int, NS - 1>{});
+}
+
 // Work item w = (patch, x0, y0, z0).  Workgroup b takes items b, b + G, b + 2G, ...
 __global__ __launch_bounds__(1024) void batch_kernel(const DevPatch* __restrict__ P,
                                                      const int4* __restrict__ items, int n_items,
@@ -158,6 +278,7 @@ struct nebo_batch {
   int n_items = 0;
   std::vector<int> patch_first;   // first item of each patch (items are grouped by patch)
   int blocks = 0;
+  long max_plane = 0;   // largest (nx+2)*(ny+2) of the patches
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -174,7 +295,11 @@ int build_items(nebo_batch* b)
     b->patch_first[k] = (int)items.size();
     const nebo_patch_t& p = b->patches[k];
     for (int z = 0; z < p.n[2]; z += tz) {
-      if (ty == 0) {   // flattened plane
+      if (tx == 0) {   // column schedule: strips of 64 columns, row groups of 64/nx x ty rows
+        const int rows = column_groups(p.n[0], ty) * ty;
+        for (int y = 0; y < p.n[1]; y += rows)
+          for (int x = 0; x < p.n[0]; x += kLanes) items.push_back(make_int4((int)k, x, y, z));
+      } else if (ty == 0) {   // flattened plane
         for (int q = 0; q < p.n[0] * p.n[1]; q += tx) items.push_back(make_int4((int)k, q, 0, z));
       } else {
         for (int y = 0; y < p.n[1]; y += ty)
@@ -195,6 +320,10 @@ int build_items(nebo_batch* b)
   if (rc) return rc;
   rc = check_hip(hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice), ctx);
   if (rc) return rc;
+  if (tx == 0) {   // one wave per item, four per workgroup
+    b->blocks = std::max(1, (b->n_items + 3) / 4);
+    return NEBO_OK;
+  }
   // persistent grid: a few workgroups per CU, never more than there are items
   const int threads = ty == 0 ? tx : tx * ty;
   const int per_cu = threads <= 256 ? 8 : threads <= 512 ? 4 : 2;
@@ -212,6 +341,12 @@ void drop_graph(nebo_batch* b)
 
 int check_tile(const int32_t* t)
 {
+  if (t[0] == 0) {
+    if ((t[1] != 1 && t[1] != 2 && t[1] != 4) || t[2] < 1)
+      return set_error(NEBO_ERR_INVALID, "Nebo Batch",
+                       "a column tile (x = 0) takes 1, 2 or 4 rows per lane (y) and z >= 1");
+    return NEBO_OK;
+  }
   if (t[1] == 0) {
     if (t[0] < 64 || t[0] > 1024 || t[0] % 64 || t[2] < 1)
       return set_error(NEBO_ERR_INVALID, "Nebo Batch",
@@ -226,7 +361,12 @@ int check_tile(const int32_t* t)
 int launch_items(nebo_batch* b, int first, int count, int grid, int swap, hipStream_t s)
 {
   if (count <= 0) return NEBO_OK;
-  if (b->tile[1] == 0) {
+  if (b->tile[0] == 0) {
+    const int ry = b->tile[1];
+    auto fn = ry == 1 ? batch_column_kernel<1, 3> : ry == 2 ? batch_column_kernel<2, 3> : batch_column_kernel<4, 2>;
+    hipLaunchKernelGGL(fn, dim3((count + 3) / 4), dim3(256), 0, s, b->d_patches, b->d_items + first, count,
+                       b->tile[2], coef6(b->coefs), b->s, swap);
+  } else if (b->tile[1] == 0) {
     const int tz = b->tile[2];
     auto fn = tz == 4 ? batch_flat_kernel<4> : tz == 8 ? batch_flat_kernel<8> : tz == 16 ? batch_flat_kernel<16>
                                                                                          : batch_flat_kernel<0>;
@@ -260,7 +400,14 @@ extern "C" int nebo_batch_create(const nebo_patch_t* patches, int32_t n, const n
     if ((long)(p.n[0] + 2) * (p.n[1] + 2) * (p.n[2] + 2) >= (1L << 31))
       return set_error(NEBO_ERR_TOO_LARGE, ctx, "patch has 2^31 or more points");
   }
+  long max_plane = 0;
+  for (int i = 0; i < n; ++i)
+    max_plane = std::max(max_plane, (long)(patches[i].n[0] + 2) * (patches[i].n[1] + 2));
+  // the column schedule addresses a plane with 32-bit byte offsets
+  if (tile && tile[0] == 0 && max_plane * 8 >= (1L << 31))
+    return set_error(NEBO_ERR_TOO_LARGE, ctx, "a column tile needs every patch plane below 2^31 bytes");
   nebo_batch* b = new nebo_batch;
+  b->max_plane = max_plane;
   b->patches.assign(patches, patches + n);
   b->coefs = *c;
   b->s = s;
@@ -363,7 +510,10 @@ extern "C" int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* str
       {64, 1, 64}, {128, 2, 16}, {128, 4, 8}, {256, 1, 16},
       // flattened plane (y = 0)
       {64, 0, 16}, {128, 0, 8}, {128, 0, 16}, {256, 0, 4}, {256, 0, 8}, {256, 0, 16}, {512, 0, 4},
-      {512, 0, 8}, {1024, 0, 4}, {1024, 0, 8}};
+      {512, 0, 8}, {1024, 0, 4}, {1024, 0, 8},
+      // column schedule (x = 0): rows per lane, planes per item
+      {0, 1, 16}, {0, 1, 32}, {0, 1, 64}, {0, 2, 16}, {0, 2, 32}, {0, 2, 64}, {0, 4, 16}, {0, 4, 32},
+      {0, 4, 64}, {0, 2, 8}};
   hipEvent_t e0, e1;
   int rc = check_hip(hipEventCreate(&e0), ctx);
   if (!rc) rc = check_hip(hipEventCreate(&e1), ctx);
@@ -371,6 +521,7 @@ extern "C" int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* str
   float best = 1e30f;
   int best_tile[3] = {b->tile[0], b->tile[1], b->tile[2]};
   for (const auto& t : cand) {
+    if (t[0] == 0 && b->max_plane * 8 >= (1L << 31)) continue;   // planes too large for 32-bit offsets
     std::memcpy(b->tile, t, sizeof b->tile);
     if ((rc = build_items(b))) break;
     // warm once, then time three steps

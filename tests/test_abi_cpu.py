@@ -166,6 +166,20 @@ def test_pending_entry_points_fail_loudly():
     assert rc in (_abi.NEBO_ERR_UNSUPPORTED, _abi.NEBO_ERR_INVALID)
 
 
+def test_batch_tile_checked_on_entry():
+    """Column tiles (x = 0) take 1, 2 or 4 rows per lane and at least one plane;
+    the tile is checked before any device call."""
+    arr = (_abi.PatchT * 1)()
+    arr[0].a, arr[0].b = 16, 32
+    arr[0].n[:] = [4, 4, 4]
+    c = nh.axis_coefs(nh.build_stencils(4, 4, 4, 1, 1, 1))
+    for bad in ((0, 3, 8), (0, 2, 0), (0, 0, 4), (96, 0, 4), (64, 32, 4)):
+        out = C.c_void_p()
+        t = (C.c_int32 * 3)(*bad)
+        assert nh.lib().nebo_batch_create(arr, 1, C.byref(c), 0.1, t, C.byref(out)) == _abi.NEBO_ERR_INVALID, bad
+        assert not out.value
+
+
 def test_backward_checkpoint_interval_checked_on_entry():
     """An interval above 1024 steps is refused before any work is queued (no
     copy, no clearing of g_k): the descriptors here are host memory marked as

@@ -37,10 +37,12 @@ def test_batch_matches_oracle_and_per_patch(count):
     s = 0.1 / (32.0 ** 2)
     rng = np.random.default_rng(count)
     init = [rng.standard_normal((d[0] + 2) * (d[1] + 2) * (d[2] + 2)) for d in dims]
-    # the default tile, a 2-D tile run one launch per patch, and flattened-plane
+    # the default tile, a 2-D tile run one launch per patch, flattened-plane
     # tiles: chunk lengths without (5) and with a compiled full-chunk path (4, 8,
-    # 16; patch sides 16..64 also leave cut chunks)
-    tiles = [None, (16, 8, 4), (128, 0, 5), (64, 0, 4), (256, 0, 8), (1024, 0, 16)]
+    # 16; patch sides 16..64 also leave cut chunks), and column tiles (x = 0: 1, 2
+    # or 4 rows per lane)
+    tiles = [None, (16, 8, 4), (128, 0, 5), (64, 0, 4), (256, 0, 8), (1024, 0, 16), (0, 1, 16), (0, 2, 5),
+             (0, 4, 64)]
     blocks = []
     for _ in tiles:
         a, b = nh.alloc_patches(dims)
@@ -62,6 +64,33 @@ def test_batch_matches_oracle_and_per_patch(count):
             # 4 steps end in block a
             assert np.array_equal(a[i].cpu().numpy(), host[i]), f"tile {t} patch {i} dims {dims[i]}"
     for B in batches:
+        B.close()
+
+
+@pytest.mark.parametrize("tile", [(0, 1, 7), (0, 2, 16), (0, 4, 3)])
+def test_batch_column_tiles_odd_patches(tile):
+    """Column tiles on patches narrower than a wave with widths that leave lanes
+    idle (7, 24, 63), two columns wide (32 row groups per wave), exactly one wave
+    wide, and wider than a wave (65 and 130 columns: strips of 64 and a last strip
+    of 1 or 2), in one launch and one launch per patch: every patch equals the
+    oracle's step."""
+    dims = [(7, 9, 11), (2, 2, 2), (24, 5, 2), (63, 2, 7), (64, 3, 4), (65, 3, 5), (130, 5, 3), (2, 17, 3)]
+    db = nh.build_stencils(32, 32, 32, 1.0, 1.0, 1.0)
+    coefs = nh.axis_coefs(db)
+    s = 0.1 / (32.0 ** 2)
+    rng = np.random.default_rng(sum(tile))
+    init = [rng.standard_normal((d[0] + 2) * (d[1] + 2) * (d[2] + 2)) for d in dims]
+    want = [_oracle_patch_step(d, h, coefs, s) for d, h in zip(dims, init)]
+    for per_patch in (False, True):
+        a, b = nh.alloc_patches(dims)
+        for blocks in (a, b):
+            for x, h in zip(blocks, init):
+                x.copy_(torch.from_numpy(h))
+        B = nh.Batch(a, b, dims, coefs, s, tile=tile)
+        (B.step_per_patch if per_patch else B.step)(swap=0)
+        torch.cuda.synchronize()
+        for i, d in enumerate(dims):
+            assert np.array_equal(b[i].cpu().numpy(), want[i]), (tile, per_patch, d)
         B.close()
 
 
