@@ -482,8 +482,9 @@ int nebo_batch_step_per_patch(nebo_batch_t* b, int swap, void* stream);
  * once and replayed by nebo_batch_graph_launch.  stream must not be NULL. */
 int nebo_batch_graph_build(nebo_batch_t* b, int32_t steps, void* stream);
 int nebo_batch_graph_launch(nebo_batch_t* b, void* stream);
-/* Time candidate tiles with one step each (a -> b), keep the fastest, write it
- * to tile_out (3 ints) and drop any captured graph.  Synchronises stream. */
+/* Time candidate tiles on alternating steps (a -> b, b -> a, as a time loop
+ * runs them; the blocks' contents change), keep the fastest, write it to
+ * tile_out (3 ints) and drop any captured graph.  Synchronises stream. */
 int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* stream);
 /* Tile currently used, number of work items and of persistent workgroups. */
 int nebo_batch_info(nebo_batch_t* b, int32_t* tile_out, int32_t* n_items,

@@ -524,10 +524,11 @@ extern "C" int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* str
     if (t[0] == 0 && b->max_plane * 8 >= (1L << 31)) continue;   // planes too large for 32-bit offsets
     std::memcpy(b->tile, t, sizeof b->tile);
     if ((rc = build_items(b))) break;
-    // warm once, then time three steps
-    if ((rc = launch_items(b, 0, b->n_items, b->blocks, 0, s))) break;
+    // warm once, then time four steps in the order a time loop runs them
+    // (a -> b, b -> a, ...: each step reads what the one before wrote)
+    if ((rc = launch_items(b, 0, b->n_items, b->blocks, 1, s))) break;
     if ((rc = check_hip(hipEventRecord(e0, s), ctx))) break;
-    for (int r = 0; r < 3 && !rc; ++r) rc = launch_items(b, 0, b->n_items, b->blocks, 0, s);
+    for (int r = 0; r < 4 && !rc; ++r) rc = launch_items(b, 0, b->n_items, b->blocks, r & 1, s);
     if (rc || (rc = check_hip(hipEventRecord(e1, s), ctx))) break;
     if ((rc = check_hip(hipEventSynchronize(e1), ctx))) break;
     float ms = 0;

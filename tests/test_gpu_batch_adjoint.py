@@ -109,7 +109,7 @@ def test_batch_graph_replays_and_autotune():
     B = nh.Batch(a, b, dims, coefs, s)
     tile = B.autotune()
     assert len(tile) == 3 and (tile[0] * tile[1] <= 1024 if tile[1] else tile[0] % 64 == 0)
-    for x, h in zip(a, init):       # autotune stepped a -> b; restore
+    for x, h in zip(a, init):       # autotune stepped the blocks; restore
         x.copy_(torch.from_numpy(h))
     for x, h in zip(b, init):
         x.copy_(torch.from_numpy(h))
