@@ -424,7 +424,7 @@ def heat_leg(steps=100, cpu_steps=5, per_graph=10):
     return {"n": n, "steps": steps, "gpu_ms_per_step": round(gpu_ms, 4),
             "graph_ms_per_step": round(graph_ms, 4), "steps_per_graph": per_graph,
             "graph_equals_eager": same,
-            "kernels_per_step": "Laplacian+Euler, cond mask, max, sum (one kernel each)",
+            "kernels_per_step": "Laplacian+Euler, cond mask (one kernel each), max, sum (block partials, then a fold: two kernels each)",
             "max_first_last": [float(maxes[0]), float(maxes[-1])],
             "max_decreasing": bool(np.all(np.diff(maxes) <= 0)),
             "oracle_cpu_ms_per_step_serial": round(cpu_ms, 3), "cpu_steps_timed": cpu_steps}

@@ -405,8 +405,9 @@ int nebo_lapl_euler(const nebo_field_t* out, const nebo_field_t* phi,
  *   4. else 128 x 8 x 64, variant 11 for kernel 0 on boxes whose width is a
  *      multiple of 128 and variant 7 otherwise, when it gives at least half the
  *      device's CUs a workgroup (256^3);
- *   5. else the direct schedule with 64 x 8 x 16 tiles, or 64 x 4 x 16 when those
- *      give fewer workgroups than CUs (small boxes).
+ *   5. else (small boxes) variant 3 with 64 x 2 tiles (one wave, 2 rows per
+ *      lane) and the longest z chunk of 16, 8 or 4 planes that gives the device
+ *      two workgroups per CU, else 4 planes.
  * nebo_stencil_set_tile with tx = ty = zchunk = variant = 0 removes the pin.
  * nebo_stencil_get_tile reports the tile last pinned or tuned (the default tile
  * if neither).  The tile changes only the schedule, never the results.  The

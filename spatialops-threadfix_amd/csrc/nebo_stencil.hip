@@ -679,7 +679,8 @@ constexpr TileShape kShapes[] = {
     {64, 4, 16, 0},   {64, 8, 16, 0},
     {64, 8, 16, 1},   {64, 8, 64, 1},   {64, 16, 32, 1},
     {64, 8, 24, 2},
-    {64, 4, 16, 3},   {64, 4, 24, 3},   {64, 2, 32, 3},
+    {64, 4, 16, 3},   {64, 4, 24, 3},   {64, 2, 32, 3},   {64, 2, 8, 3},    {64, 4, 8, 3},    {64, 2, 4, 3},
+    {64, 1, 8, 3},    {64, 1, 16, 3},   {64, 2, 16, 3},
     {64, 4, 16, 4},   {64, 4, 24, 4},   {64, 4, 32, 4},   {64, 2, 32, 4},   {256, 4, 16, 4},
     {256, 4, 64, 4},
     {64, 8, 32, 5},   {64, 16, 16, 5},  {64, 16, 32, 5},  {128, 8, 32, 5},  {128, 8, 64, 5},
@@ -816,9 +817,12 @@ LaplFn lapl_table(int s) { return lapl_pick(s, std::make_index_sequence<kNumShap
 //  4. else the default tile when it gives at least half the CUs a workgroup
 //     (256^3: 64-plane chunks, one workgroup per CU), for div(k grad phi) on
 //     boxes whose width is a multiple of 128 as pair columns;
-//  5. smaller boxes run the direct schedule, whose short tiles fill the GPU
-//     (64^3 8.3 us, 128^3 14.5 us against 51 us for the default tile, which
-//     gives them 32 workgroups).
+//  5. smaller boxes run one-wave column tiles of 64 columns x 2 rows per lane
+//     with the longest z chunk (16, 8 or 4 planes) that gives every CU two
+//     waves (else 4 planes): time steps that alternate two fields, timed in a
+//     hipGraph on MI355X, 64^3 3.9 us (Laplacian) and 4.6 us (div(k grad phi))
+//     against 7.8 and 9.0 us for the direct schedule's 64 x 4 x 16 tiles;
+//     128^3 8.7 and 13.9 us against 10.0 and 18.6 us.
 std::atomic<int> g_pinned[2] = {{-1}, {-1}};
 std::atomic<int> g_last[2] = {{kDefaultShape}, {kDefaultShape}};   // last pinned or tuned tile
 struct TunedTile {
@@ -827,11 +831,11 @@ struct TunedTile {
 std::mutex g_tuned_mu;
 std::vector<TunedTile> g_tuned;
 
-constexpr int kSmallWide = find_shape_c(64, 8, 16, 0), kSmallNarrow = find_shape_c(64, 4, 16, 0);
+constexpr int kSmallColumn[] = {find_shape_c(64, 2, 16, 3), find_shape_c(64, 2, 8, 3), find_shape_c(64, 2, 4, 3)};
 constexpr int kLongChunks[] = {find_shape_c(128, 8, 512, 7), find_shape_c(128, 8, 256, 7),
                                find_shape_c(128, 8, 128, 7)};
-static_assert(kSmallWide >= 0 && kSmallNarrow >= 0 && kLongChunks[0] >= 0 && kLongChunks[1] >= 0 &&
-                  kLongChunks[2] >= 0,
+static_assert(kSmallColumn[0] >= 0 && kSmallColumn[1] >= 0 && kSmallColumn[2] >= 0 && kLongChunks[0] >= 0 &&
+                  kLongChunks[1] >= 0 && kLongChunks[2] >= 0,
               "box-size tiles");
 
 
@@ -858,7 +862,9 @@ int select_shape(int kernel, int device, const nebo_limits_t& lim)
     if (tile_groups(sh, lim) >= cus) return sh;
   if (2 * tile_groups(kDefaultShape, lim) >= cus)
     return kernel == 0 && n[0] % kShapes[kDefaultPairShape].tx == 0 ? kDefaultPairShape : kDefaultShape;
-  return tile_groups(kSmallWide, lim) >= cus ? kSmallWide : kSmallNarrow;
+  for (int sh : kSmallColumn)
+    if (tile_groups(sh, lim) >= 2 * cus) return sh;
+  return kSmallColumn[2];
 }
 
 int find_shape(int tx, int ty, int zc, int variant)

@@ -531,8 +531,9 @@ def test_hand_kernels_two_ghost_layers_default_tile():
 def test_tile_selection_by_box_size():
     """Without a pinned or tuned tile, boxes that the default tile fills (at
     least half the CUs get a workgroup) run it (div(k grad phi) as pair columns
-    where the width is a multiple of 128), smaller ones the direct schedule; a
-    pin applies to every box; a tuned tile to its box size only."""
+    where the width is a multiple of 128), smaller ones one-wave column tiles
+    with short z chunks; a pin applies to every box; a tuned tile to its box
+    size only."""
     nh.stencil_reset_tile(0)
     nh.stencil_reset_tile(1)
     default = (128, 8, 64, 7)
@@ -543,8 +544,9 @@ def test_tile_selection_by_box_size():
             assert nh.stencil_select_tile(kern, (256, 256, 256)) == ((128, 8, 64, 11) if kern == 0 else default)
             assert nh.stencil_select_tile(kern, (512, 512, 512)) == (128, 8, 512, 7)
             assert nh.stencil_select_tile(kern, (384, 384, 384)) == (128, 8, 256, 7)
-            assert nh.stencil_select_tile(kern, (128, 128, 128)) == (64, 8, 16, 0)
-            assert nh.stencil_select_tile(kern, (64, 64, 64)) == (64, 4, 16, 0)
+            assert nh.stencil_select_tile(kern, (128, 128, 128)) == (64, 2, 16, 3)
+            assert nh.stencil_select_tile(kern, (64, 64, 64)) == (64, 2, 4, 3)
+            assert nh.stencil_select_tile(kern, (16, 16, 16)) == (64, 2, 4, 3)
         nh.stencil_set_tile(0, 64, 4, 24, 3)
         assert nh.stencil_select_tile(0, (64, 64, 64)) == (64, 4, 24, 3)
         assert nh.stencil_select_tile(0, (256, 256, 256)) == (64, 4, 24, 3)
@@ -557,7 +559,7 @@ def test_tile_selection_by_box_size():
         c = nh.axis_coefs(nh.build_stencils(n, n, n, 1.0, 1.0, 1.0))
         best, _ = nh.stencil_autotune(gpu_empty_like(L), k, phi, c)
         assert nh.stencil_select_tile(0, (n, n, n)) == best and nh.stencil_get_tile(0) == best
-        assert nh.stencil_select_tile(0, (128, 128, 128)) == (64, 8, 16, 0)   # the pin is gone
+        assert nh.stencil_select_tile(0, (128, 128, 128)) == (64, 2, 16, 3)   # the pin is gone
         # results do not depend on the selection
         ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
         orc.divkgrad(ref, k, phi, c, 0)
