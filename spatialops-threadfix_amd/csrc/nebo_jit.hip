@@ -439,31 +439,27 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   auto ring = [](int f, int j, int k) {
     return "R" + std::to_string(f) + "_" + std::to_string(j) + "_" + std::to_string(k);
   };
-  // Edge reads: per family, one load per (dy, dz) of its x-shifted reads serves
-  // every row: lanes 0 .. RY-1 read column x0 - 1 of rows 0 .. RY-1 (shifted by
-  // dy) when the family reads x - 1 at that (dy, dz), lanes 32 .. 32+RY-1 column
-  // x0 + 64 when it reads x + 1; other lanes re-read their row 0.  A side the
-  // family does not read may lie outside the region its reads cover, below the
-  // allocation's start where the range check does not reach (a family reading
-  // x and x + 1 only, on a field whose assignment starts at the block's first
-  // column), so it is never loaded.
-  struct EdgeRead { int f, dy, dz; bool left, right; };
-  std::vector<EdgeRead> edges;
-  auto edge_index = [&](int f, int dy, int dz) {
-    for (size_t e = 0; e < edges.size(); ++e)
-      if (edges[e].f == f && edges[e].dy == dy && edges[e].dz == dz) return (int)e;
-    return -1;
-  };
+  // Edge reads: per family, one more load per plane serves every x-shifted read
+  // of that plane: lanes 0 .. nE-1 read column x0 - 1 of rows ey .. ey+nE-1 (ey
+  // the lowest dy of the family's x-shifted reads, nE = RY + their dy span) when
+  // the family reads x - 1, lanes 32 .. 32+nE-1 column x0 + 64 when it reads
+  // x + 1; other lanes re-read a ring row.  The value travels with its plane in
+  // the register ring, loaded the same number of planes ahead as the rows.  A
+  // side the family does not read may lie outside the region its reads cover,
+  // below the allocation's start where the range check does not reach (a
+  // family reading x and x + 1 only, on a field whose assignment starts at the
+  // block's first column), so it is never loaded.
+  struct EdgeFam { bool left = false, right = false; int ey = 2, eyhi = -2; };
+  std::vector<EdgeFam> ef(cp.nf);
   for (int s = 0; s < kVmMaxLoads; ++s) {
     if (cp.fam[s] < 0 || cp.dx[s] == 0) continue;
-    int e = edge_index(cp.fam[s], cp.dy[s], cp.dz[s]);
-    if (e < 0) {
-      edges.push_back(EdgeRead{cp.fam[s], cp.dy[s], cp.dz[s], false, false});
-      e = (int)edges.size() - 1;
-    }
-    (cp.dx[s] < 0 ? edges[e].left : edges[e].right) = true;
+    EdgeFam& e = ef[cp.fam[s]];
+    (cp.dx[s] < 0 ? e.left : e.right) = true;
+    e.ey = std::min(e.ey, cp.dy[s]);
+    e.eyhi = std::max(e.eyhi, cp.dy[s]);
   }
-  // per-lane byte offsets from a plane base: ring rows, and edge reads
+  auto has_edge = [&](int f) { return ef[f].left || ef[f].right; };
+  // per-lane byte offsets from a plane base: ring rows, and the edge read
   for (int f = 0; f < cp.nf; ++f) {
     const int r = cp.root[f], cls = P.loads[r].cls;
     const int rows = RY + cp.yhi[f] - cp.ylo[f];
@@ -472,19 +468,17 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     for (int j = 0; j < rows; ++j)
       o << "  const unsigned o" << f << "_" << j << " = (unsigned)((x - A.lo0 + 1) + A.clsX[" << cls
         << "] * (y0 - A.lo1 + " << j << ")) * 8u;\n";
-  }
-  for (size_t e = 0; e < edges.size(); ++e) {
-    const int f = edges[e].f, cls = P.loads[cp.root[f]].cls;
-    const std::string own = "o" + std::to_string(f) + "_" + std::to_string(-cp.ylo[f]);
-    // x of the wave's first lane is x - lane_; rows shifted by dy relative to ylo
-    o << "  const unsigned eo" << e << "_ = ";
-    if (edges[e].left)
-      o << "lane_ < " << RY << " ? (unsigned)((x - lane_ - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ + "
-        << edges[e].dy - cp.ylo[f] << ")) * 8u : ";
-    if (edges[e].right)
-      o << "(lane_ >= 32 && lane_ < " << 32 + RY << ") ? (unsigned)((x - lane_ + 65 - A.lo0) + A.clsX[" << cls
-        << "] * (y0 - A.lo1 + lane_ - 32 + " << edges[e].dy - cp.ylo[f] << ")) * 8u : ";
-    o << own << ";\n";
+    if (!has_edge(f)) continue;
+    const int nE = RY + ef[f].eyhi - ef[f].ey, dyb = ef[f].ey - cp.ylo[f];
+    // x of the wave's first lane is x - lane_; lane i holds row y0 + ey + i
+    o << "  const unsigned eo" << f << "_ = ";
+    if (ef[f].left)
+      o << "lane_ < " << nE << " ? (unsigned)((x - lane_ - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ + "
+        << dyb << ")) * 8u : ";
+    if (ef[f].right)
+      o << "(lane_ >= 32 && lane_ < " << 32 + nE << ") ? (unsigned)((x - lane_ + 65 - A.lo0) + A.clsX[" << cls
+        << "] * (y0 - A.lo1 + lane_ - 32 + " << dyb << ")) * 8u : ";
+    o << "o" << f << "_" << -cp.ylo[f] << ";\n";
   }
   // the resource of family f at plane zexpr
   auto rsrc = [&](int f, const std::string& zexpr) {
@@ -492,12 +486,14 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     return "plane_rs_(A.base[" + std::to_string(r) + "], pb" + std::to_string(f) + "_ + (long)A.clsXY[" +
            std::to_string(cls) + "] * (" + zexpr + "), (long)A.ghi[" + std::to_string(r) + "])";
   };
-  // ring rows of family f into plane slot k, plane zexpr
+  auto edge_reg = [](int f, int k) { return "V" + std::to_string(f) + "_" + std::to_string(k); };
+  // ring rows of family f (and its edge read) into plane slot k, plane zexpr
   auto load_plane = [&](const std::string& ind, int f, int k, const std::string& zexpr) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f];
     o << ind << "{\n" << ind << "  const __amdgpu_buffer_rsrc_t rs_ = " << rsrc(f, zexpr) << ";\n";
     for (int j = 0; j < rows; ++j)
       o << ind << "  " << ring(f, j, k) << " = ld_(rs_, o" << f << "_" << j << ");\n";
+    if (has_edge(f)) o << ind << "  " << edge_reg(f, k) << " = ld_(rs_, eo" << f << "_);\n";
     o << ind << "}\n";
   };
   // slots 0 .. np-1 hold planes z + zlo .. z + zhi; slot np receives plane
@@ -506,20 +502,16 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
     for (int j = 0; j < rows; ++j)
       for (int k = 0; k <= np; ++k) o << "  double " << ring(f, j, k) << " = 0.0;\n";
+    if (has_edge(f))
+      for (int k = 0; k <= np; ++k) o << "  double " << edge_reg(f, k) << " = 0.0;\n";
     for (int k = 0; k < np; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
   }
-  // One z step at plane zq: the wave-edge reads, the next plane's loads, then
-  // RY cells.  Ring slot k of family f (plane z + zlo + k; k = np is the plane
-  // loading) lives in register (k + u) mod (np + 1), so the unrolled steps
-  // u = 0 .. U-1 rename registers instead of copying them; the remainder loop
-  // (u = 0) copies.
+  // One z step at plane zq: the next plane's loads, then RY cells.  Ring slot k
+  // of family f (plane z + zlo + k; k = np is the plane loading) lives in
+  // register (k + u) mod (np + 1), so the unrolled steps u = 0 .. U-1 rename
+  // registers instead of copying them; the remainder loop (u = 0) copies.
   auto step = [&](const std::string& in, int u) {
     auto slot = [&](int f, int k) { return (k + u) % (cp.zhi[f] - cp.zlo[f] + 2); };
-    // edge reads first: loads complete in issue order, so waiting for them does
-    // not wait for the next plane
-    for (size_t e = 0; e < edges.size(); ++e)
-      o << in << "const double E" << e << "_ = ld_(" << rsrc(edges[e].f, "zq + " + std::to_string(edges[e].dz))
-        << ", eo" << e << "_);\n";
     for (int f = 0; f < cp.nf; ++f) {
       const int np = cp.zhi[f] - cp.zlo[f] + 1;
       load_plane(in, f, slot(f, np), "zq + " + std::to_string(cp.zhi[f] + 1));
@@ -535,12 +527,12 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
           o << ind << "const double L0_" << s << " = " << R << ";\n";
           continue;
         }
-        const int e = edge_index(f, cp.dy[s], cp.dz[s]);
         // the exchange is taken by every lane, outside any branch
         o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << R << ", "
           << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
-        o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? rdlane_(E" << e
-          << "_, " << (cp.dx[s] > 0 ? 32 : 0) + r << ") : D0_" << s << ";\n";
+        o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? rdlane_("
+          << edge_reg(f, slot(f, cp.dz[s] - cp.zlo[f])) << ", " << (cp.dx[s] > 0 ? 32 : 0) + r + cp.dy[s] - ef[f].ey
+          << ") : D0_" << s << ";\n";
       }
       const std::vector<std::string> res = emit_cells(o, P, direct, 1, true, ind);
       o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res[0]
@@ -569,8 +561,10 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   step("    ", 0);
   for (int f = 0; f < cp.nf; ++f) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
-    for (int k = 0; k < np; ++k)
+    for (int k = 0; k < np; ++k) {
       for (int j = 0; j < rows; ++j) o << "    " << ring(f, j, k) << " = " << ring(f, j, k + 1) << ";\n";
+      if (has_edge(f)) o << "    " << edge_reg(f, k) << " = " << edge_reg(f, k + 1) << ";\n";
+    }
   }
   o << "  }\n}\n";
   return o.str();
@@ -785,8 +779,9 @@ std::string structural_key(const VmProgram& P, int V, const Shape& sh, int kind)
 
 // Whether an assignment runs on the column schedule (on by default for every
 // tree column_plan accepts) and its shape.  Unless set, the planes per block
-// are the most of 32, 16, 8 that still give 8 waves per compute unit (longer
-// chunks load fewer halo planes; a sweep at 256^3 measured 32 fastest), else 8.
+// are the most of 64, 32, 16 that still give 4 waves per compute unit (longer
+// chunks load fewer halo planes; a sweep at 256^3 measured 64 planes at 4
+// waves per CU fastest: div(k grad phi) 75.5-76 us, 79-84 at 32 planes), else 8.
 bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp, ColumnShape* cs)
 {
   if (!g_column.load() || !column_plan(P, slot_use(P), cp)) return false;
@@ -810,9 +805,9 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
     if (lim && hipGetDevice(&device) == hipSuccess) {
       const long cols = (long)((lim->hi[0] - lim->lo[0] + 63) / 64) *
                         ((lim->hi[1] - lim->lo[1] + cp->RY * cs->waves - 1) / (cp->RY * cs->waves)) * cs->waves;
-      const long want = 8L * device_cu_count(device);
+      const long want = 4L * device_cu_count(device);
       const int nz = lim->hi[2] - lim->lo[2];
-      for (int zc : {32, 16})
+      for (int zc : {64, 32, 16})
         if (cs->zc == kDefaultColumnZc && cols * ((nz + zc - 1) / zc) >= want) cs->zc = zc;
     }
   }

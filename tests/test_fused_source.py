@@ -192,7 +192,8 @@ def test_column_schedule_source():
     loads before the z loop and 2 x 6 per step, each a raw buffer load at a
     per-lane offset from a per-plane resource.  The four x-neighbour reads per
     cell come from the adjacent lanes (4 x 4 lane exchanges per step); the values
-    past the wave's edges come from one load per family and step for all rows.
+    past the wave's edges come from one more load per family and plane for all
+    rows, issued with the plane's rows and kept in the ring beside them.
     The ring of 4 planes makes the main loop 4 steps long (registers renamed, not
     copied) and a remainder loop 1 step.  Trees with masks, and pointwise trees,
     keep the flattened-plane schedule, as does every tree with the column schedule
@@ -204,17 +205,17 @@ def test_column_schedule_source():
     main, rest = loops.split("for (; z < zend; ++z)")
     assert "for (; z + 3 < zend; z += 4)" in main
     ring = lambda text: len(re.findall(r"R\d+_\d+_\d+ = ld_\(rs_, o\d+_\d+\)", text))
-    edge = lambda text: len(re.findall(r"E\d+_ = ld_\(plane_rs_\(", text))
+    edge = lambda text: len(re.findall(r"V\d+_\d+ = ld_\(rs_, eo\d+_\)", text))
     assert ring(pro) == 2 * 6 * 3 and ring(main) == 4 * 2 * 6 and ring(rest) == 2 * 6
     assert main.count("__builtin_amdgcn_mov_dpp") == 4 * 16 and rest.count("__builtin_amdgcn_mov_dpp") == 16
-    assert edge(main) == 4 * 2 and edge(rest) == 2
-    assert main.count("rdlane_(E") == 4 * 16 and rest.count("rdlane_(E") == 16
+    assert edge(pro) == 2 * 3 and edge(main) == 4 * 2 and edge(rest) == 2
+    assert main.count("rdlane_(V") == 4 * 16 and rest.count("rdlane_(V") == 16
     # the main loop renames (no ring copies); the remainder loop shifts the ring
     assert not re.search(r"R\d+_\d+_\d+ = R", main) and len(re.findall(r"R\d+_\d+_\d+ = R", rest)) == 2 * 6 * 3
-    # one resource per family and plane (3 planes before the loop, 2 ring + 2 edge
-    # per step), and no per-read index clamp
-    assert pro.count("plane_rs_(A.base[") == 6 and main.count("plane_rs_(A.base[") == 4 * 4
-    assert rest.count("plane_rs_(A.base[") == 4 and "i_ < A.glo[" not in src
+    # one resource per family and plane (3 planes before the loop, 2 per step),
+    # and no per-read index clamp
+    assert pro.count("plane_rs_(A.base[") == 6 and main.count("plane_rs_(A.base[") == 4 * 2
+    assert rest.count("plane_rs_(A.base[") == 2 and "i_ < A.glo[" not in src
     n = (6, 5, 4)
     L = nh.Layout.build(n, 1)
     f, g = host(L), host(L)
