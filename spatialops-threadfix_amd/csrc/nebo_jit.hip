@@ -31,7 +31,9 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <atomic>
+#include <functional>
 #include <cstring>
 #include <mutex>
 #include <sstream>
@@ -157,14 +159,73 @@ int reads_per_thread(const SlotUse& u, int V)
   return n;
 }
 
+// The expression one stack-program operation forms from the names of its
+// operands (in push order) and of its constant.
+std::string op_expr(int op, const std::string& k, const std::vector<std::string>& v)
+{
+  if (op == VM_CONST) return k;
+  if (op == VM_MULC) return v[0] + " * " + k;
+  if (op == VM_SELECT) return "(" + v[0] + " != 0.0) ? " + v[1] + " : " + v[2];
+  if (op >= VM_ADD && op <= VM_OR) {
+    const std::string& a = v[0];
+    const std::string& b = v[1];
+    switch (op) {
+    case VM_ADD: return a + " + " + b;
+    case VM_SUB: return a + " - " + b;
+    case VM_MUL: return a + " * " + b;
+    case VM_DIV: return a + " / " + b;
+    case VM_MAX: return "(" + a + " > " + b + ") ? " + a + " : " + b;
+    case VM_MIN: return "(" + a + " < " + b + ") ? " + a + " : " + b;
+    case VM_POW: return "pow(" + a + ", " + b + ")";
+    case VM_EQ: return "(" + a + " == " + b + ") ? 1.0 : 0.0";
+    case VM_NE: return "(" + a + " != " + b + ") ? 1.0 : 0.0";
+    case VM_LT: return "(" + a + " < " + b + ") ? 1.0 : 0.0";
+    case VM_LE: return "(" + a + " <= " + b + ") ? 1.0 : 0.0";
+    case VM_GT: return "(" + a + " > " + b + ") ? 1.0 : 0.0";
+    case VM_GE: return "(" + a + " >= " + b + ") ? 1.0 : 0.0";
+    case VM_AND: return "(" + a + " != 0.0 && " + b + " != 0.0) ? 1.0 : 0.0";
+    default: return "(" + a + " != 0.0 || " + b + " != 0.0) ? 1.0 : 0.0";   // VM_OR
+    }
+  }
+  const std::string& a = v[0];
+  switch (op) {
+  case VM_NEG: return "-" + a;
+  case VM_ABS: return "fabs(" + a + ")";
+  case VM_SQRT: return "__builtin_sqrt(" + a + ")";
+  case VM_SQUARE: return a + " * " + a;
+  case VM_CUBE: return a + " * " + a + " * " + a;
+  case VM_POWC: return "pow(" + a + ", " + k + ")";
+  case VM_EXP: return "exp(" + a + ")";
+  case VM_LOG: return "log(" + a + ")";
+  case VM_LOG10: return "log10(" + a + ")";
+  case VM_SIN: return "sin(" + a + ")";
+  case VM_COS: return "cos(" + a + ")";
+  case VM_TAN: return "tan(" + a + ")";
+  case VM_TANH: return "tanh(" + a + ")";
+  case VM_ERF: return "erf(" + a + ")";
+  case VM_ERFC: return "erfc(" + a + ")";
+  case VM_INV_ERF: return "erfinv(" + a + ")";
+  case VM_INV_ERFC: return "erfcinv(" + a + ")";
+  default: return "((" + a + ") != 0.0) ? 0.0 : 1.0";   // VM_NOT
+  }
+}
+
+// Operands an operation pops.
+int op_arity(int op)
+{
+  if (op == VM_LOAD || op == VM_LOADG || op == VM_CONST || op == VM_MASK) return 0;
+  if (op == VM_SELECT) return 3;
+  if (op >= VM_ADD && op <= VM_OR) return 2;
+  return 1;
+}
+
 // Index set-up, reads and operations for V cells of one z column at (x, y,
 // z0 .. z{V-1}); the generated code expects x, y and the z variables in scope
 // and returns the name of each cell's value.  With full, the V cells are
 // consecutive planes (z{v} = z0 + v) and reads along a z chain are issued once
 // per distinct plane; otherwise (a chunk cut by the box end, z clamped) every
 // cell reads for itself.
-std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int V, bool full,
-                                    const std::string& ind)
+void emit_reads(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int V, bool full, const std::string& ind)
 {
   const bool* used = u.used;
   const bool* guarded = u.guarded;
@@ -206,6 +267,12 @@ std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, c
         << " >> 5] >> (q" << v << "_" << m << " & 31)) & 1u) ? 1.0 : 0.0;\n";
     }
   }
+}
+
+std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int V, bool full,
+                                    const std::string& ind)
+{
+  emit_reads(o, P, u, V, full, ind);
   // phase 2: the program, one SSA temporary per result
   std::vector<std::string> results;
   for (int v = 0; v < V; ++v) {
@@ -216,70 +283,15 @@ std::vector<std::string> emit_cells(std::ostringstream& o, const VmProgram& P, c
       o << ind << "const double " << name << " = " << expr << ";\n";
       st.push_back(name);
     };
-    auto pop = [&]() {
-      std::string s = st.back();
-      st.pop_back();
-      return s;
-    };
     const std::string vs = std::to_string(v);
     for (int pc = 0; pc < P.n_insns; ++pc) {
       const int op = P.insns[pc].code >> 4, imm = P.insns[pc].imm;
-      const std::string k = "A.c[" + std::to_string(imm) + "]";
-      switch (op) {
-      case VM_LOAD: case VM_LOADG: st.push_back("L" + vs + "_" + std::to_string(imm)); break;
-      case VM_MASK: st.push_back("M" + vs + "_" + std::to_string(imm)); break;
-      case VM_CONST: fresh(k); break;
-      case VM_MULC: { const std::string a = pop(); fresh(a + " * " + k); break; }
-      case VM_SELECT: {
-        const std::string c = pop(), b = pop(), a = pop();
-        fresh("(" + a + " != 0.0) ? " + b + " : " + c);
-        break;
-      }
-      default: {
-        if (op >= VM_ADD && op <= VM_OR) {
-          const std::string b = pop(), a = pop();
-          switch (op) {
-          case VM_ADD: fresh(a + " + " + b); break;
-          case VM_SUB: fresh(a + " - " + b); break;
-          case VM_MUL: fresh(a + " * " + b); break;
-          case VM_DIV: fresh(a + " / " + b); break;
-          case VM_MAX: fresh("(" + a + " > " + b + ") ? " + a + " : " + b); break;
-          case VM_MIN: fresh("(" + a + " < " + b + ") ? " + a + " : " + b); break;
-          case VM_POW: fresh("pow(" + a + ", " + b + ")"); break;
-          case VM_EQ: fresh("(" + a + " == " + b + ") ? 1.0 : 0.0"); break;
-          case VM_NE: fresh("(" + a + " != " + b + ") ? 1.0 : 0.0"); break;
-          case VM_LT: fresh("(" + a + " < " + b + ") ? 1.0 : 0.0"); break;
-          case VM_LE: fresh("(" + a + " <= " + b + ") ? 1.0 : 0.0"); break;
-          case VM_GT: fresh("(" + a + " > " + b + ") ? 1.0 : 0.0"); break;
-          case VM_GE: fresh("(" + a + " >= " + b + ") ? 1.0 : 0.0"); break;
-          case VM_AND: fresh("(" + a + " != 0.0 && " + b + " != 0.0) ? 1.0 : 0.0"); break;
-          default: fresh("(" + a + " != 0.0 || " + b + " != 0.0) ? 1.0 : 0.0"); break;   // VM_OR
-          }
-        } else {
-          const std::string a = pop();
-          switch (op) {
-          case VM_NEG: fresh("-" + a); break;
-          case VM_ABS: fresh("fabs(" + a + ")"); break;
-          case VM_SQRT: fresh("__builtin_sqrt(" + a + ")"); break;
-          case VM_SQUARE: fresh(a + " * " + a); break;
-          case VM_CUBE: fresh(a + " * " + a + " * " + a); break;
-          case VM_POWC: fresh("pow(" + a + ", " + k + ")"); break;
-          case VM_EXP: fresh("exp(" + a + ")"); break;
-          case VM_LOG: fresh("log(" + a + ")"); break;
-          case VM_LOG10: fresh("log10(" + a + ")"); break;
-          case VM_SIN: fresh("sin(" + a + ")"); break;
-          case VM_COS: fresh("cos(" + a + ")"); break;
-          case VM_TAN: fresh("tan(" + a + ")"); break;
-          case VM_TANH: fresh("tanh(" + a + ")"); break;
-          case VM_ERF: fresh("erf(" + a + ")"); break;
-          case VM_ERFC: fresh("erfc(" + a + ")"); break;
-          case VM_INV_ERF: fresh("erfinv(" + a + ")"); break;
-          case VM_INV_ERFC: fresh("erfcinv(" + a + ")"); break;
-          default: fresh("((" + a + ") != 0.0) ? 0.0 : 1.0"); break;   // VM_NOT
-          }
-        }
-      }
-      }
+      if (op == VM_LOAD || op == VM_LOADG) { st.push_back("L" + vs + "_" + std::to_string(imm)); continue; }
+      if (op == VM_MASK) { st.push_back("M" + vs + "_" + std::to_string(imm)); continue; }
+      const int n = op_arity(op);
+      std::vector<std::string> args(st.end() - n, st.end());
+      st.resize(st.size() - n);
+      fresh(op_expr(op, "A.c[" + std::to_string(imm) + "]", args));
     }
     // a bare leaf has no temporary of its own; the leaf's name is the value
     results.push_back(st.empty() ? std::string("0.0") : st.back());
@@ -390,6 +402,118 @@ bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
   for (int ry = 4; ry >= 1 && !cp->RY; ry /= 2)
     if (regs1 * ry + regs0 <= kColumnRegs) cp->RY = ry;
   return cp->RY > 0;
+}
+
+// Tree form of a stack program: post-order (children before their parent, the
+// root last), each node its operation, immediate and operand nodes in push order.
+struct TNode {
+  int op, imm, nk, kid[3];
+};
+
+std::vector<TNode> program_tree(const VmProgram& P)
+{
+  std::vector<TNode> t;
+  std::vector<int> st;
+  for (int pc = 0; pc < P.n_insns; ++pc) {
+    TNode n{P.insns[pc].code >> 4, P.insns[pc].imm, 0, {-1, -1, -1}};
+    n.nk = op_arity(n.op);
+    for (int i = n.nk - 1; i >= 0; --i) {
+      n.kid[i] = st.back();
+      st.pop_back();
+    }
+    t.push_back(n);
+    st.push_back((int)t.size() - 1);
+  }
+  return t;
+}
+
+// Subtrees the column schedule evaluates once and reuses: a subtree A whose
+// every read is another subtree B's read moved one row down (or one plane down),
+// with the same operations and constants, has at row r (plane z) the value B had
+// at row r - 1 (plane z - 1), bit for bit: the same operations on the same
+// loaded values.  Stencil chains are full of such pairs (the low and high faces
+// of a divergence of a face flux), so a lane forms each y face and z face once:
+// y faces carried from row to row, z faces from plane to plane.  A is computed
+// in the first row (first plane of the chunk) only.  x-shifted pairs are not
+// reused: exchanging a value between lanes costs as many instructions as a face.
+struct ShiftPlan {
+  std::vector<int> from;    // node -> the node B it is taken from, or -1
+  std::vector<int> dir;     // 1: B one row down (y), 2: B one plane down (z)
+  std::vector<int> carry;   // node B -> its carry index, or -1
+  std::vector<int> cdir;    // carry index -> 1 (row to row) or 2 (plane to plane)
+};
+
+ShiftPlan shift_plan(const VmProgram& P, const ColumnPlan& cp, const std::vector<TNode>& t)
+{
+  const int n = (int)t.size();
+  ShiftPlan sp;
+  sp.from.assign(n, -1);
+  sp.dir.assign(n, 0);
+  sp.carry.assign(n, -1);
+  std::vector<int> first(n), lo(3 * n, 0);
+  std::vector<char> ok(n, 0), loads(n, 0);
+  for (int i = 0; i < n; ++i) {
+    const TNode& a = t[i];
+    first[i] = a.nk ? first[a.kid[0]] : i;
+    if (a.op == VM_LOAD) {
+      const int s = a.imm;
+      ok[i] = cp.fam[s] >= 0;
+      loads[i] = 1;
+      lo[3 * i] = cp.dx[s], lo[3 * i + 1] = cp.dy[s], lo[3 * i + 2] = cp.dz[s];
+      continue;
+    }
+    if (a.op == VM_LOADG || a.op == VM_MASK) continue;   // per-cell reads: never moved
+    ok[i] = 1;
+    for (int k = 0; k < a.nk; ++k) {
+      const int c = a.kid[k];
+      ok[i] = ok[i] && ok[c];
+      if (!loads[c]) continue;
+      for (int d = 0; d < 3; ++d)
+        lo[3 * i + d] = loads[i] ? std::min(lo[3 * i + d], lo[3 * c + d]) : lo[3 * c + d];
+      loads[i] = 1;
+    }
+  }
+  // signature of a subtree with its reads relative to the subtree's lowest corner
+  std::function<void(int, const int*, std::string&)> ser = [&](int i, const int* c, std::string& o) {
+    const TNode& a = t[i];
+    if (a.op == VM_LOAD) {
+      const int s = a.imm;
+      o += "L" + std::to_string(cp.fam[s]) + ":" + std::to_string(cp.dx[s] - c[0]) + "," +
+           std::to_string(cp.dy[s] - c[1]) + "," + std::to_string(cp.dz[s] - c[2]) + ";";
+      return;
+    }
+    o += "(" + std::to_string(a.op) + "," + std::to_string(a.imm);
+    for (int k = 0; k < a.nk; ++k) {
+      o += " ";
+      ser(a.kid[k], c, o);
+    }
+    o += ")";
+  };
+  std::vector<std::string> sig(n);
+  for (int i = 0; i < n; ++i)
+    if (ok[i] && loads[i] && t[i].nk) ser(i, &lo[3 * i], sig[i]);
+  std::vector<char> claimed(n, 0);
+  auto claim = [&](int i) {
+    for (int j = first[i]; j <= i; ++j) claimed[j] = 1;
+  };
+  for (int b = n - 1; b >= 0; --b) {   // parents before their children: largest subtrees first
+    if (claimed[b] || sig[b].empty()) continue;
+    for (int a = n - 1; a >= 0; --a) {
+      if (a == b || claimed[a] || sig[a] != sig[b]) continue;
+      if ((a >= first[b] && a <= b) || (b >= first[a] && b <= a)) continue;
+      const int dx = lo[3 * b] - lo[3 * a], dy = lo[3 * b + 1] - lo[3 * a + 1], dz = lo[3 * b + 2] - lo[3 * a + 2];
+      const int d = dx == 0 && dy == 1 && dz == 0 ? 1 : dx == 0 && dy == 0 && dz == 1 ? 2 : 0;
+      if (!d) continue;
+      sp.from[a] = b;
+      sp.dir[a] = d;
+      sp.carry[b] = (int)sp.cdir.size();
+      sp.cdir.push_back(d);
+      claim(a);
+      claim(b);
+      break;
+    }
+  }
+  return sp;
 }
 
 // Launch shape of the column schedule: waves per block (stacked in y) and z
@@ -506,6 +630,54 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       for (int k = 0; k <= np; ++k) o << "  double " << edge_reg(f, k) << " = 0.0;\n";
     for (int k = 0; k < np; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
   }
+  // Cells: the program as a tree, with the subtrees of shift_plan taken from the
+  // row above or the plane below instead of recomputed.  Carries are written
+  // after the whole cell, so a subtree taken from the previous plane always reads
+  // the previous plane's value, whichever operand order the tree has.
+  const std::vector<TNode> tree = program_tree(P);
+  const ShiftPlan sp = shift_plan(P, cp, tree);
+  auto cy_name = [](int c, int r) { return "cy" + std::to_string(c) + "_" + std::to_string(r); };
+  auto cz_name = [](int c, int r) { return "cz" + std::to_string(c) + "_" + std::to_string(r); };
+  auto emit_row = [&](const std::string& ind, int r) {
+    int tmp = 0;
+    std::vector<std::pair<std::string, std::string>> writes;
+    std::function<std::string(int, const std::string&, bool)> ev = [&](int i, const std::string& in,
+                                                                        bool computed) -> std::string {
+      const TNode& a = tree[i];
+      if (!computed && sp.from[i] >= 0) {
+        const int c = sp.carry[sp.from[i]];
+        if (sp.dir[i] == 1 && r > 0) return cy_name(c, r - 1);
+        if (sp.dir[i] == 2) {
+          // the chunk's first plane computes it; later planes take the carry
+          const std::string v = "zc" + std::to_string(tmp++) + "_";
+          o << in << "double " << v << ";\n" << in << "if (zq == zs) {\n";
+          const std::string w = ev(i, in + "  ", true);
+          o << in << "  " << v << " = " << w << ";\n" << in << "} else {\n"
+            << in << "  " << v << " = " << cz_name(c, r) << ";\n" << in << "}\n";
+          return v;
+        }
+      }
+      std::string name;
+      if (a.op == VM_LOAD || a.op == VM_LOADG) {
+        name = "L0_" + std::to_string(a.imm);
+      } else {
+        std::vector<std::string> args;
+        for (int k = 0; k < a.nk; ++k) args.push_back(ev(a.kid[k], in, false));
+        name = "t0_" + std::to_string(tmp++);
+        o << in << "const double " << name << " = " << op_expr(a.op, "A.c[" + std::to_string(a.imm) + "]", args)
+          << ";\n";
+      }
+      if (sp.carry[i] >= 0) {
+        const int c = sp.carry[i];
+        if (sp.cdir[c] == 2) writes.emplace_back(cz_name(c, r), name);
+        else if (r + 1 < RY) writes.emplace_back(cy_name(c, r), name);
+      }
+      return name;
+    };
+    const std::string res = ev((int)tree.size() - 1, ind, false);
+    for (const auto& w : writes) o << ind << w.first << " = " << w.second << ";\n";
+    return res;
+  };
   // One z step at plane zq: the next plane's loads, then RY cells.  Ring slot k
   // of family f (plane z + zlo + k; k = np is the plane loading) lives in
   // register (k + u) mod (np + 1), so the unrolled steps u = 0 .. U-1 rename
@@ -516,6 +688,9 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       const int np = cp.zhi[f] - cp.zlo[f] + 1;
       load_plane(in, f, slot(f, np), "zq + " + std::to_string(cp.zhi[f] + 1));
     }
+    for (size_t c = 0; c < sp.cdir.size(); ++c)
+      if (sp.cdir[c] == 1)
+        for (int r = 0; r + 1 < RY; ++r) o << in << "double " << cy_name((int)c, r) << ";\n";
     for (int r = 0; r < RY; ++r) {
       const std::string ind = in + "  ";
       o << in << "{\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = zq;\n";
@@ -534,8 +709,9 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
           << edge_reg(f, slot(f, cp.dz[s] - cp.zlo[f])) << ", " << (cp.dx[s] > 0 ? 32 : 0) + r + cp.dy[s] - ef[f].ey
           << ") : D0_" << s << ";\n";
       }
-      const std::vector<std::string> res = emit_cells(o, P, direct, 1, true, ind);
-      o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res[0]
+      emit_reads(o, P, direct, 1, true, ind);
+      const std::string res = emit_row(ind, r);
+      o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res
         << ", &A.out[x + A.oX * y + A.oXY * zq]);\n" << in << "}\n";
     }
   };
@@ -547,6 +723,9 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     U = U / a * n;
   }
   if (U > 6) U = 1;   // families of different depths: copies rather than a long unroll
+  for (size_t c = 0; c < sp.cdir.size(); ++c)
+    if (sp.cdir[c] == 2)
+      for (int r = 0; r < RY; ++r) o << "  double " << cz_name((int)c, r) << " = 0.0;\n";
   o << "  int z = zs;   // column schedule\n";
   if (U > 1) {
     o << "  for (; z + " << U - 1 << " < zend; z += " << U << ") {\n";
