@@ -268,3 +268,26 @@ def test_other_shapes_compile(rtc, shape, column):
     finally:
         nh.fused_jit_tune()
         nh.fused_jit_column()
+
+
+def test_column_faces_reused_across_rows_and_planes():
+    """In div(k grad phi) the low y face of row r is the high y face of row r - 1,
+    and the low z face of plane z the high z face of plane z - 1: the column
+    kernel forms each once, carrying y faces from row to row (3 carries for 4
+    rows) and z faces from plane to plane (one per row), with the chunk's first
+    plane computing its low faces.  A subtree that differs from its shifted
+    partner in a constant is never taken for it."""
+    src = nh.fused_source(_divkgrad_tree())
+    assert len(set(re.findall(r"\bcy\d+_\d+", src))) == 3
+    assert len(set(re.findall(r"\bcz\d+_\d+", src))) == 4
+    assert "if (zq == zs) {" in src
+    n = (8, 8, 8)
+    L = nh.Layout.build(n, 1)
+    f = host(L)
+    hi_ = nh.Stencil([(0, 0, 0), (0, 1, 0)], [1.0, 2.0])
+    same = nh.Stencil([(0, -1, 0), (0, 0, 0)], [1.0, 2.0])
+    other = nh.Stencil([(0, -1, 0), (0, 0, 0)], [3.0, 4.0])
+    for lo_, pairs in ((same, True), (other, False)):
+        src = nh.fused_source(hi_(f) * 0.5 + lo_(f) * 0.25)
+        assert "// column schedule" in src
+        assert (len(set(re.findall(r"\bcy\d+_\d+", src))) > 0) == pairs
