@@ -260,7 +260,7 @@ int nebo_fused_jit_tune(int32_t block_threads, int32_t tile_x, int32_t max_cells
  * Other trees keep the shape of nebo_fused_jit_tune.  mode 1 uses it where it
  * applies (the default), 0 never; waves per block in [1, 16], zchunk in
  * [1, 4096]; waves 0 keeps the default (1); zchunk 0 (the default) takes the
- * most of 64, 32 or 16 planes that still gives 4 waves per compute unit, else 8.
+ * most of 128, 64, 32 or 16 planes that still gives 4 waves per compute unit, else 8.
  * Results are the same bits either way. */
 int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk);
 int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);

@@ -342,7 +342,7 @@ struct ColumnPlan {
   int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
   int fam[kVmMaxLoads], dx[kVmMaxLoads], dy[kVmMaxLoads], dz[kVmMaxLoads];
 };
-constexpr int kColumnRegs = 80;   // ring doubles per lane, over all families
+constexpr int kColumnRegs = 96;   // ring doubles per lane, over all families
 // Defaults from a sweep at 256^3 (div(k grad phi) and Laplacian trees): one wave
 // per block measured as fast as 2-4 waves and gives small boxes the most
 // blocks; g_column_zc 0 picks the planes per block from the box size.
@@ -958,9 +958,10 @@ std::string structural_key(const VmProgram& P, int V, const Shape& sh, int kind)
 
 // Whether an assignment runs on the column schedule (on by default for every
 // tree column_plan accepts) and its shape.  Unless set, the planes per block
-// are the most of 64, 32, 16 that still give 4 waves per compute unit (longer
-// chunks load fewer halo planes; a sweep at 256^3 measured 64 planes at 4
-// waves per CU fastest: div(k grad phi) 75.5-76 us, 79-84 at 32 planes), else 8.
+// are the most of 128, 64, 32, 16 that still give 4 waves per compute unit
+// (longer chunks load fewer halo planes; sweeps measured 64 planes fastest at
+// 256^3, div(k grad phi) 75.5-76 us against 79-84 at 32, and 128 at 512^3, the
+// transport stages 1.51-1.60 ms against 1.56-1.68 at 64), else 8.
 bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp, ColumnShape* cs)
 {
   if (!g_column.load() || !column_plan(P, slot_use(P), cp)) return false;
@@ -986,7 +987,7 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
                         ((lim->hi[1] - lim->lo[1] + cp->RY * cs->waves - 1) / (cp->RY * cs->waves)) * cs->waves;
       const long want = 4L * device_cu_count(device);
       const int nz = lim->hi[2] - lim->lo[2];
-      for (int zc : {64, 32, 16})
+      for (int zc : {128, 64, 32, 16})
         if (cs->zc == kDefaultColumnZc && cols * ((nz + zc - 1) / zc) >= want) cs->zc = zc;
     }
   }

@@ -137,7 +137,7 @@ def fused_jit_column(enable=True, waves=0, zchunk=0):
     trees without masks whose reads of each field stay within one point in x, y
     and z, a wave covers 64 x points, each lane 1-4 rows and `zchunk` planes, and
     each plane of each field is loaded once.  0 keeps the defaults (1 wave per
-    block; planes per block from the box size: 64, 32, 16 or 8)."""
+    block; planes per block from the box size: 128, 64, 32, 16 or 8)."""
     _abi.call("nebo_fused_jit_column", int(bool(enable)), int(waves), int(zchunk))
 
 
