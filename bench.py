@@ -592,7 +592,7 @@ def pmc_probe(n, tile=None):
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": True, "copy_bytes_read": L.npts * 8, "copy_bytes_written": L.npts * 8,
                       "kernel_bytes_algorithmic": BYTES_PER_CELL * n ** 3,
-                      "tile": _tile_name(nh.stencil_get_tile(0))}))
+                      "tile": _tile_name(nh.stencil_select_tile(0, L.interior))}))
 
 
 def main():
@@ -645,7 +645,7 @@ def main():
         best, times = nh.stencil_autotune(r, k, phi, coefs)
         tune = {"tile": _tile_name(best),
                 "ms_per_tile": {_tile_name(t): round(v, 5) for t, v in times.items()}}
-    tile = nh.stencil_get_tile(0)
+    tile = nh.stencil_select_tile(0, L.interior)   # the tile the launches use (pinned, tuned or by size)
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
