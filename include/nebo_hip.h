@@ -224,6 +224,21 @@ int nebo_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost,
 int nebo_assign_generic(const nebo_field_t* lhs, const nebo_expr_t* rhs,
                         int use_ghost, void* stream);
 
+/* Repeated calls.  An assignment (nebo_assign, nebo_assign_generic) or a
+ * reduction (nebo_reduce, nebo_reduce_async, nebo_reduce_async_noalloc) whose
+ * descriptors equal those of an earlier successful call byte for byte (the LHS,
+ * and the expression's node, field, stencil and mask tables), with the same
+ * flags, the same current device and an entry point that plans alike, replays the
+ * launches that call planned: the checks, the lowering and the kernel choice are
+ * not repeated, so a time loop that keeps its descriptors pays one key
+ * comparison per call on the host besides its launches.  Values are read at
+ * launch, so new values in the same fields are seen.  The records are dropped
+ * whenever something the planning reads changes: nebo_stencil_set_tile,
+ * nebo_stencil_autotune, nebo_set_fused_jit, nebo_fused_jit_tune,
+ * nebo_fused_jit_column, and every kernel compiled at run time.  The reference
+ * has no counterpart: its expression templates make these choices at compile
+ * time. */
+
 /* The generic evaluator and the reductions run a tree as one kernel compiled at
  * run time for that tree's shape (hiprtc, gfx950, the library's flags; cached
  * per device and shape: field pointers, constants and sizes are arguments), or,
@@ -327,9 +342,10 @@ int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
 int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
                       double* out_dev, void* stream);
 
-/* nebo_reduce_async that allocates nothing, ever.  No counterpart in the
+/* nebo_reduce_async that allocates no device memory, ever.  No counterpart in the
  * reference: for callers that cannot call hipMalloc (code under HIP graph
- * capture, tight serving loops, harnesses that forbid allocation).  The
+ * capture, tight serving loops, harnesses that forbid allocation).  (The first
+ * call with a new descriptor keeps a small host-side record for its repeats.)  The
  * partials go to a static device array of 32 KB per GPU (the partials of at most
  * 2048 blocks plus the result slot; grids never exceed 2048 blocks, so no input
  * size needs more), shared by every call on that GPU and holding the last call's

@@ -575,28 +575,61 @@ int launch_generic_assign(const nebo_field_t* lhs, const VmProgram& prog, int ma
   dim3 grid((B.nplane + kBlock - 1) / kBlock, B.zchunks);
   double* out = frame_origin(lhs);
   const int oX = lhs->glob[0], oXY = lhs->glob[0] * lhs->glob[1];
-  hipStream_t s = (hipStream_t)stream;
   const bool one = prog.n_classes == 1, math = uses_math(prog);
   BackendScope bs("stack-program interpreter (" + std::to_string(prog.n_insns) + " instructions)");
-  if (one && !math)
-    hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
-  else if (one)
-    hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, true>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
-  else if (!math)
-    hipLaunchKernelGGL((vm_assign_kernel<kVmMaxClasses, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
-  else
-    hipLaunchKernelGGL((vm_assign_kernel<kVmMaxClasses, kVmV, true>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
-  return check_hip(hipGetLastError(), "Nebo Assignment");
+  return emit(stream, CallPtrs{}, [prog, B, grid, out, oX, oXY, one, math](void* sv, const CallPtrs&) {
+    hipStream_t s = (hipStream_t)sv;
+    if (one && !math)
+      hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
+    else if (one)
+      hipLaunchKernelGGL((vm_assign_kernel<1, kVmV, true>), grid, dim3(kBlock), 0, s, prog, B, out, oX, oXY);
+    else if (!math)
+      hipLaunchKernelGGL((vm_assign_kernel<kVmMaxClasses, kVmV, false>), grid, dim3(kBlock), 0, s, prog, B, out, oX,
+                         oXY);
+    else
+      hipLaunchKernelGGL((vm_assign_kernel<kVmMaxClasses, kVmV, true>), grid, dim3(kBlock), 0, s, prog, B, out, oX,
+                         oXY);
+    return check_hip(hipGetLastError(), "Nebo Assignment");
+  });
 }
 
 namespace {
 
+int plan_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, void* stream,
+                bool allow_special);
+
+// lhs <<= rhs: replays the launches of an earlier identical call, else checks,
+// plans and launches the assignment, recording its launches (GPU operands).
 int do_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, void* stream,
               bool allow_special)
 {
   static const char* ctx = "Nebo Assignment";
   clear_error();
   if (!lhs || !rhs) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  thread_local CallKey key;
+  int device = 0;
+  bool keyed = lhs->device >= 0;
+  if (keyed && hipGetDevice(&device) != hipSuccess) {
+    (void)hipGetLastError();
+    keyed = false;
+  }
+  if (keyed) {
+    key.begin(allow_special ? 'A' : 'G', device, use_ghost, 0);
+    key.field(*lhs);
+    keyed = key.expr(rhs);
+  }
+  if (!keyed) return plan_assign(lhs, rhs, use_ghost, stream, allow_special);
+  if (auto r = memo_find(key)) return memo_run(*r, stream, CallPtrs{});
+  MemoRecorder rec(key);
+  const int rc = plan_assign(lhs, rhs, use_ghost, stream, allow_special);
+  rec.commit(rc);
+  return rc;
+}
+
+int plan_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, void* stream,
+                bool allow_special)
+{
+  static const char* ctx = "Nebo Assignment";
   nebo_limits_t lim;
   int empty = 0;
   int rc = assign_limits(lhs, rhs, use_ghost, &lim, &empty);
@@ -627,38 +660,47 @@ int do_assign(const nebo_field_t* lhs, const nebo_expr_t* rhs, int use_ghost, vo
 }
 
 template <int NCLS, int OP, bool MATH>
-void launch_reduce_op(const VmProgram& prog, const Box& B, int blocks, const RedBuf& buf, hipStream_t s)
+Launch reduce_op_launch(const VmProgram& prog, const Box& B, int blocks)
 {
-  hipLaunchKernelGGL((vm_reduce_kernel<NCLS, kVmV, OP, MATH>), dim3(blocks), dim3(kBlock), 0, s, prog, B,
-                     buf.partial);
+  return [prog, B, blocks](void* s, const CallPtrs& p) {
+    hipLaunchKernelGGL((vm_reduce_kernel<NCLS, kVmV, OP, MATH>), dim3(blocks), dim3(kBlock), 0, (hipStream_t)s, prog,
+                       B, p.partial);
+    return check_hip(hipGetLastError(), "Nebo Reduction");
+  };
 }
 
 template <int NCLS, bool MATH>
-void launch_reduce_any(const VmProgram& prog, const Box& B, int op, int blocks, const RedBuf& buf, hipStream_t s)
+Launch reduce_any_launch(const VmProgram& prog, const Box& B, int op, int blocks)
 {
   switch (op) {
-  case NEBO_RED_MAX: launch_reduce_op<NCLS, NEBO_RED_MAX, MATH>(prog, B, blocks, buf, s); break;
-  case NEBO_RED_MIN: launch_reduce_op<NCLS, NEBO_RED_MIN, MATH>(prog, B, blocks, buf, s); break;
-  case NEBO_RED_SUM: launch_reduce_op<NCLS, NEBO_RED_SUM, MATH>(prog, B, blocks, buf, s); break;
-  default: launch_reduce_op<NCLS, NEBO_RED_NORM, MATH>(prog, B, blocks, buf, s); break;
+  case NEBO_RED_MAX: return reduce_op_launch<NCLS, NEBO_RED_MAX, MATH>(prog, B, blocks);
+  case NEBO_RED_MIN: return reduce_op_launch<NCLS, NEBO_RED_MIN, MATH>(prog, B, blocks);
+  case NEBO_RED_SUM: return reduce_op_launch<NCLS, NEBO_RED_SUM, MATH>(prog, B, blocks);
+  default: return reduce_op_launch<NCLS, NEBO_RED_NORM, MATH>(prog, B, blocks);
   }
 }
 
-void launch_reduce_final(int op, const double* partial, int blocks, double* out_dev, hipStream_t s)
+// The fold of `blocks` partials (p.partial) into the result slot p.out.
+Launch reduce_final_launch(int op, int blocks)
 {
-  switch (op) {
-  case NEBO_RED_MAX: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MAX>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  case NEBO_RED_MIN: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MIN>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  case NEBO_RED_SUM: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_SUM>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  default: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_NORM>), dim3(1), dim3(kBlock), 0, s, partial, blocks, out_dev); break;
-  }
+  return [op, blocks](void* sv, const CallPtrs& p) {
+    hipStream_t s = (hipStream_t)sv;
+    switch (op) {
+    case NEBO_RED_MAX: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MAX>), dim3(1), dim3(kBlock), 0, s, p.partial, blocks, p.out); break;
+    case NEBO_RED_MIN: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_MIN>), dim3(1), dim3(kBlock), 0, s, p.partial, blocks, p.out); break;
+    case NEBO_RED_SUM: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_SUM>), dim3(1), dim3(kBlock), 0, s, p.partial, blocks, p.out); break;
+    default: hipLaunchKernelGGL((reduce_final_kernel<NEBO_RED_NORM>), dim3(1), dim3(kBlock), 0, s, p.partial, blocks, p.out); break;
+    }
+    return check_hip(hipGetLastError(), "Nebo Reduction");
+  };
 }
 
 // Two launches: the kernel compiled for the tree's shape (compiled now if
 // `compile`, else only when a previous call compiled it) or the interpreter's
-// kernel writes one partial per block into buf; the final kernel folds them.
-int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, double* out_dev, const RedBuf& buf,
-                  bool compile, hipStream_t s)
+// kernel writes one partial per block into ptrs.partial; the final kernel folds
+// them into ptrs.out.
+int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, const CallPtrs& ptrs, bool compile,
+                  hipStream_t s)
 {
   const Box B = make_box(lim, kVmV);
   const long items = (long)B.nplane * B.zchunks;
@@ -668,20 +710,19 @@ int launch_reduce(const VmProgram& prog, const nebo_limits_t& lim, int op, doubl
   static_assert(kRedBufBytes >= sizeof(double) * (2 * kRedMaxBlocks + 1), "partials and result fit the buffer");
   bool done = false;
   int jit_blocks = 0;
-  int rc = jit_reduce(prog, lim, op, buf.partial, kRedMaxBlocks, compile, s, &jit_blocks, &done);
+  int rc = jit_reduce(prog, lim, op, ptrs, kRedMaxBlocks, compile, s, &jit_blocks, &done);
   if (rc) return rc;
-  if (done) {
-    launch_reduce_final(op, buf.partial, jit_blocks, out_dev, s);
-    return check_hip(hipGetLastError(), "Nebo Reduction");
-  }
+  if (done) return emit(s, ptrs, reduce_final_launch(op, jit_blocks));
   const bool one = prog.n_classes == 1, math = uses_math(prog);
-  BackendScope bs("stack-program interpreter reduction (" + std::to_string(blocks) + " blocks)");
-  if (one && !math) launch_reduce_any<1, false>(prog, B, op, blocks, buf, s);
-  else if (one) launch_reduce_any<1, true>(prog, B, op, blocks, buf, s);
-  else if (!math) launch_reduce_any<kVmMaxClasses, false>(prog, B, op, blocks, buf, s);
-  else launch_reduce_any<kVmMaxClasses, true>(prog, B, op, blocks, buf, s);
-  launch_reduce_final(op, buf.partial, blocks, out_dev, s);
-  return check_hip(hipGetLastError(), "Nebo Reduction");
+  {
+    BackendScope bs("stack-program interpreter reduction (" + std::to_string(blocks) + " blocks)");
+    Launch l = one && !math ? reduce_any_launch<1, false>(prog, B, op, blocks)
+               : one        ? reduce_any_launch<1, true>(prog, B, op, blocks)
+               : !math      ? reduce_any_launch<kVmMaxClasses, false>(prog, B, op, blocks)
+                            : reduce_any_launch<kVmMaxClasses, true>(prog, B, op, blocks);
+    if ((rc = emit(s, ptrs, std::move(l)))) return rc;
+  }
+  return emit(s, ptrs, reduce_final_launch(op, blocks));
 }
 
 // Reduction buffers: one per (device, stream), created by the first reduction
@@ -717,11 +758,10 @@ int stream_reduce_buffer(void* stream, StreamRedBuf* out)
 
 __global__ void fill_value_kernel(double* out, double v) { *out = v; }
 
-// Reduction of e into out_dev (device memory) on stream, with buf as working
-// memory; `compile`: a tree shape met for the first time is compiled now (else
-// it runs on the interpreter's kernel until an allocating call compiles it).
-int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void* stream, const RedBuf& buf,
-              bool compile)
+// Reduction of e into ptrs.out (device memory) on stream, with ptrs.partial as
+// working memory; `compile`: a tree shape met for the first time is compiled now
+// (else it runs on the interpreter's kernel until an allocating call compiles it).
+int do_reduce(const nebo_expr_t* e, int op, int use_ghost, const CallPtrs& ptrs, void* stream, bool compile)
 {
   static const char* ctx = "Nebo Reduction";
   nebo_limits_t lim;
@@ -731,13 +771,40 @@ int do_reduce(const nebo_expr_t* e, int op, int use_ghost, double* out_dev, void
   hipStream_t s = (hipStream_t)stream;
   if (empty) {
     const double v = op == NEBO_RED_MAX ? -INFINITY : op == NEBO_RED_MIN ? INFINITY : 0.0;
-    hipLaunchKernelGGL(fill_value_kernel, dim3(1), dim3(1), 0, s, out_dev, v);
-    return check_hip(hipGetLastError(), ctx);
+    return emit(s, ptrs, [v](void* sv, const CallPtrs& p) {
+      hipLaunchKernelGGL(fill_value_kernel, dim3(1), dim3(1), 0, (hipStream_t)sv, p.out, v);
+      return check_hip(hipGetLastError(), ctx);
+    });
   }
   VmProgram prog;
   int depth = 0;
   if ((rc = lower_program(e, &prog, &depth))) return rc;
-  return launch_reduce(prog, lim, op, out_dev, buf, compile, s);
+  return launch_reduce(prog, lim, op, ptrs, compile, s);
+}
+
+// do_reduce, its launches recorded under *key when key is non-null.
+int recorded_reduce(const CallKey* key, const nebo_expr_t* e, int op, int use_ghost, const CallPtrs& ptrs,
+                    void* stream, bool compile)
+{
+  if (!key) return do_reduce(e, op, use_ghost, ptrs, stream, compile);
+  MemoRecorder rec(*key);
+  const int rc = do_reduce(e, op, use_ghost, ptrs, stream, compile);
+  rec.commit(rc);
+  return rc;
+}
+
+// Key of a reduction; false (not recorded) for operands on the host or when no
+// GPU is current.
+bool reduce_key(CallKey* key, const nebo_expr_t* e, int op, int use_ghost, bool compile)
+{
+  if (!e || e->n_fields < 0 || (e->n_fields > 0 && (!e->fields || e->fields[0].device < 0))) return false;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  key->begin('R', device, op, 2 * use_ghost + (compile ? 1 : 0));
+  return key->expr(e);
 }
 
 // Argument checks shared by the reduction entry points.
@@ -832,12 +899,22 @@ extern "C" int nebo_reduce_async(const nebo_expr_t* expr, int op, int use_ghost,
                                  double* out_dev, void* stream)
 {
   clear_error();
-  int rc = check_reduce(expr, op, use_ghost);
-  if (rc) return rc;
+  thread_local CallKey key;
+  const bool keyed = out_dev && reduce_key(&key, expr, op, use_ghost, true);
+  int rc;
+  if (keyed) {
+    if (auto r = memo_find(key)) {
+      StreamRedBuf b;
+      if ((rc = stream_reduce_buffer(stream, &b))) return rc;
+      return memo_run(*r, stream, CallPtrs{out_dev, b.buf.partial});
+    }
+  }
+  if ((rc = check_reduce(expr, op, use_ghost))) return rc;
   if (!out_dev) return set_error(NEBO_ERR_INVALID, "Nebo Reduction", "null argument");
   StreamRedBuf b;
   if ((rc = stream_reduce_buffer(stream, &b))) return rc;
-  return do_reduce(expr, op, use_ghost, out_dev, stream, b.buf, true);
+  return recorded_reduce(keyed ? &key : nullptr, expr, op, use_ghost, CallPtrs{out_dev, b.buf.partial}, stream,
+                         true);
 }
 
 namespace {
@@ -870,17 +947,27 @@ extern "C" int nebo_reduce_async_noalloc(const nebo_expr_t* expr, int op, int us
 {
   static const char* ctx = "Nebo Reduction";
   clear_error();
-  int rc = check_reduce(expr, op, use_ghost);
-  if (rc) return rc;
-  if (!out_dev) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
+  thread_local CallKey key;
+  const bool keyed = out_dev && reduce_key(&key, expr, op, use_ghost, false);
+  int rc;
   int device = 0;
+  char* p = nullptr;
+  if (keyed) {
+    if (auto r = memo_find(key)) {
+      if ((rc = check_hip(hipGetDevice(&device), ctx))) return rc;
+      if ((rc = scratch_on_device(device, &p))) return rc;
+      return memo_run(*r, stream, CallPtrs{out_dev, red_buf_at(p).partial});
+    }
+  }
+  if ((rc = check_reduce(expr, op, use_ghost))) return rc;
+  if (!out_dev) return set_error(NEBO_ERR_INVALID, ctx, "null argument");
   if ((rc = check_hip(hipGetDevice(&device), ctx))) return rc;
   for (int i = 0; i < expr->n_fields; ++i)
     if (expr->fields[i].device != device)
       return set_error(NEBO_ERR_DEVICE, ctx, "operands must be resident on the current GPU");
-  char* p = nullptr;
   if ((rc = scratch_on_device(device, &p))) return rc;
-  return do_reduce(expr, op, use_ghost, out_dev, stream, red_buf_at(p), false);
+  return recorded_reduce(keyed ? &key : nullptr, expr, op, use_ghost, CallPtrs{out_dev, red_buf_at(p).partial},
+                         stream, false);
 }
 
 extern "C" int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, double* out,
@@ -894,11 +981,15 @@ extern "C" int nebo_reduce(const nebo_expr_t* expr, int op, int use_ghost, doubl
   hipStream_t s = (hipStream_t)stream;
   StreamRedBuf b;
   if ((rc = stream_reduce_buffer(stream, &b))) return rc;
+  thread_local CallKey key;
+  const bool keyed = reduce_key(&key, expr, op, use_ghost, true);
   {
     // the result slot is the stream's: no other nebo_reduce on this stream may
     // write it between this reduction and the copy of its result
     std::lock_guard<std::mutex> lk(*b.mu);
-    rc = do_reduce(expr, op, use_ghost, b.buf.result, stream, b.buf, true);
+    const CallPtrs ptrs{b.buf.result, b.buf.partial};
+    std::shared_ptr<const MemoRecord> r = keyed ? memo_find(key) : nullptr;
+    rc = r ? memo_run(*r, stream, ptrs) : recorded_reduce(keyed ? &key : nullptr, expr, op, use_ghost, ptrs, stream, true);
     if (!rc) rc = check_hip(hipMemcpyAsync(out, b.buf.result, sizeof(double), hipMemcpyDeviceToHost, s), ctx);
   }
   if (!rc) rc = check_hip(hipStreamSynchronize(s), ctx);

@@ -4,6 +4,8 @@
 #include "../../include/nebo_hip.h"
 
 #include <cstdint>
+#include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -25,6 +27,59 @@ struct BackendScope {
   ~BackendScope();
   BackendScope(const BackendScope&) = delete;
   BackendScope& operator=(const BackendScope&) = delete;
+};
+
+// ---------------------------------------------------------------------------
+// call memo: an assignment or reduction whose descriptors are byte for byte
+// those of an earlier successful call (same kind, flags, current device and
+// planning settings) replays that call's launches instead of checking, lowering
+// and planning the tree again, so a time loop that repeats its expressions pays
+// one key comparison per call on the host before its launches.
+// ---------------------------------------------------------------------------
+// Pointers of one call that its descriptors do not hold: a reduction's result
+// slot and its block partials.
+struct CallPtrs {
+  double* out = nullptr;
+  double* partial = nullptr;
+};
+// One launch of a recorded call, on the stream given at replay.
+using Launch = std::function<int(void* stream, const CallPtrs& p)>;
+// Runs `launch` now and, while the calling thread records a call, keeps it for
+// replay with the backend name in effect.  Every launch of an assignment or
+// reduction goes through here, or its replay would miss it.
+int emit(void* stream, const CallPtrs& p, Launch launch);
+// Drops every recorded call.  Called by whatever changes what planning reads:
+// pinned and tuned tiles, the run-time compiler's settings, a kernel compiled.
+void bump_plan_epoch();
+
+// Key of a call: its kind, flags and current device, then the bytes of its
+// descriptors (only the points a stencil defines).  expr() is false for a
+// descriptor it cannot read safely (negative counts, null tables): such calls
+// are not recorded, and planning reports their error.
+struct CallKey {
+  std::string bytes;
+  void begin(char kind, int device, int flag_a, int flag_b);
+  void field(const nebo_field_t& f);
+  bool expr(const nebo_expr_t* e);
+};
+struct MemoRecord;
+// The call recorded under key for the current settings, or null.
+std::shared_ptr<const MemoRecord> memo_find(const CallKey& key);
+// Replays a recorded call's launches on stream.
+int memo_run(const MemoRecord& r, void* stream, const CallPtrs& p);
+// Records the launches of the calling thread from construction to commit(),
+// which keeps them under the key when the call succeeded.
+class MemoRecorder {
+ public:
+  explicit MemoRecorder(const CallKey& key);
+  ~MemoRecorder();
+  void commit(int rc);
+  MemoRecorder(const MemoRecorder&) = delete;
+  MemoRecorder& operator=(const MemoRecorder&) = delete;
+ private:
+  std::string key_;
+  MemoRecord* rec_;
+  MemoRecord* prev_;
 };
 
 // ---------------------------------------------------------------------------
@@ -120,12 +175,12 @@ int device_cu_count(int device);
 // compilation is off or failed for this program (the interpreter then runs it).
 bool jit_enabled();
 int jit_assign(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t& lim, void* stream, bool* done);
-// Reduction: writes one (hi, lo) partial per block into partial (room for
+// Reduction: writes one (hi, lo) partial per block into ptrs.partial (room for
 // max_blocks) and the block count into *blocks; the caller folds the partials
 // with reduce_final_kernel.  Sets *done, or leaves it false when run-time
 // compilation is off, failed for this program, or (compile false) the program's
 // kernel has not been compiled by an earlier call.
-int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, double* partial, int max_blocks, bool compile,
+int jit_reduce(const VmProgram& P, const nebo_limits_t& lim, int op, const CallPtrs& ptrs, int max_blocks, bool compile,
                void* stream, int* blocks, bool* done);
 
 // One (field, offset) read of an expression; edgeless when an edgeless stencil

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <sstream>
 #include <unordered_map>
@@ -57,8 +58,13 @@ bool report_backend()
   return r == 1;
 }
 
+namespace {
+thread_local unsigned t_scope_serial = 0;   // BackendScopes this thread has opened
+}
+
 BackendScope::BackendScope(const std::string& name)
 {
+  ++t_scope_serial;
   t_backend = name;
   if (report_backend()) {
     std::printf("Starting Nebo HIP %s\n", name.c_str());
@@ -72,6 +78,137 @@ BackendScope::~BackendScope()
     std::printf("Finished Nebo HIP %s\n", t_backend.c_str());
     std::fflush(stdout);
   }
+}
+
+// ---- call memo (see nebo_internal.h) ----
+struct MemoRecord {
+  struct Step {
+    bool scope;            // opened a backend scope when recorded (replayed inside one)
+    std::string backend;   // the backend name in effect at the launch
+    Launch launch;
+    unsigned serial;       // scope serial at the launch, while recording
+  };
+  uint64_t epoch = 0;
+  unsigned serial0 = 0;    // scope serial when recording started
+  bool ok = true;
+  std::vector<Step> steps;
+};
+
+namespace {
+std::atomic<uint64_t> g_plan_epoch{1};
+thread_local MemoRecord* t_record = nullptr;
+std::mutex g_memo_mu;
+std::unordered_map<std::string, std::shared_ptr<const MemoRecord>> g_memo;
+constexpr size_t kMemoMaxCalls = 1024;   // a full memo is cleared and refilled
+
+template <class T>
+void put_bytes(std::string& k, const T& v)
+{
+  k.append(reinterpret_cast<const char*>(&v), sizeof v);
+}
+}  // namespace
+
+void bump_plan_epoch() { g_plan_epoch.fetch_add(1); }
+
+int emit(void* stream, const CallPtrs& p, Launch launch)
+{
+  const int rc = launch(stream, p);
+  MemoRecord* r = t_record;
+  if (r) {
+    if (rc) {
+      r->ok = false;
+    } else {
+      const unsigned prev = r->steps.empty() ? r->serial0 : r->steps.back().serial;
+      r->steps.push_back(MemoRecord::Step{t_scope_serial != prev, t_backend, std::move(launch), t_scope_serial});
+    }
+  }
+  return rc;
+}
+
+void CallKey::begin(char kind, int device, int flag_a, int flag_b)
+{
+  bytes.clear();
+  bytes.push_back(kind);
+  put_bytes(bytes, device);
+  put_bytes(bytes, flag_a);
+  put_bytes(bytes, flag_b);
+}
+
+void CallKey::field(const nebo_field_t& f) { put_bytes(bytes, f); }
+
+bool CallKey::expr(const nebo_expr_t* e)
+{
+  if (!e || e->n_nodes < 0 || e->n_fields < 0 || e->n_stencils < 0 || e->n_masks < 0) return false;
+  if ((e->n_nodes && !e->nodes) || (e->n_fields && !e->fields) || (e->n_stencils && !e->stencils) ||
+      (e->n_masks && !e->masks))
+    return false;
+  put_bytes(bytes, e->n_nodes);
+  put_bytes(bytes, e->n_fields);
+  put_bytes(bytes, e->n_stencils);
+  put_bytes(bytes, e->n_masks);
+  bytes.append(reinterpret_cast<const char*>(e->nodes), sizeof(nebo_node_t) * (size_t)e->n_nodes);
+  bytes.append(reinterpret_cast<const char*>(e->fields), sizeof(nebo_field_t) * (size_t)e->n_fields);
+  for (int i = 0; i < e->n_stencils; ++i) {
+    const nebo_stencil_t& st = e->stencils[i];
+    const int n = st.npts < 0 ? 0 : st.npts > NEBO_MAX_STENCIL_POINTS ? NEBO_MAX_STENCIL_POINTS : st.npts;
+    put_bytes(bytes, st.kind);
+    put_bytes(bytes, st.npts);
+    bytes.append(reinterpret_cast<const char*>(st.pts), sizeof st.pts[0] * (size_t)n);
+    bytes.append(reinterpret_cast<const char*>(st.coefs), sizeof st.coefs[0] * (size_t)n);
+  }
+  bytes.append(reinterpret_cast<const char*>(e->masks), sizeof(nebo_mask_t) * (size_t)e->n_masks);
+  return true;
+}
+
+std::shared_ptr<const MemoRecord> memo_find(const CallKey& key)
+{
+  std::lock_guard<std::mutex> lk(g_memo_mu);
+  auto it = g_memo.find(key.bytes);
+  if (it == g_memo.end()) return nullptr;
+  if (it->second->epoch != g_plan_epoch.load()) {
+    g_memo.erase(it);
+    return nullptr;
+  }
+  return it->second;
+}
+
+int memo_run(const MemoRecord& r, void* stream, const CallPtrs& p)
+{
+  for (const MemoRecord::Step& st : r.steps) {
+    int rc;
+    if (st.scope) {
+      BackendScope bs(st.backend);
+      rc = st.launch(stream, p);
+    } else {
+      rc = st.launch(stream, p);
+    }
+    if (rc) return rc;
+  }
+  return NEBO_OK;
+}
+
+MemoRecorder::MemoRecorder(const CallKey& key) : key_(key.bytes), rec_(new MemoRecord), prev_(t_record)
+{
+  rec_->epoch = g_plan_epoch.load();
+  rec_->serial0 = t_scope_serial;
+  t_record = rec_;
+}
+
+void MemoRecorder::commit(int rc)
+{
+  t_record = prev_;
+  if (rc != NEBO_OK || !rec_->ok) return;
+  std::shared_ptr<const MemoRecord> r(rec_);
+  rec_ = nullptr;
+  std::lock_guard<std::mutex> lk(g_memo_mu);
+  if (g_memo.size() >= kMemoMaxCalls) g_memo.clear();
+  g_memo[key_] = std::move(r);
+}
+
+MemoRecorder::~MemoRecorder()
+{
+  t_record = prev_;
+  delete rec_;
 }
 
 int check_hip(int st, const char* ctx)

@@ -932,9 +932,13 @@ int launch_divk_shape(int shape, int form, const nebo_field_t* r, const nebo_fie
   const int nb = T.nx * T.ny * T.nz;
   const int X = r->glob[0], XY = r->glob[0] * r->glob[1];
   BackendScope bs(tile_backend(form == 2 ? 1 : 0, shape));
-  hipLaunchKernelGGL(divk_table(form, shape), dim3(nb), dim3(geo.threads), 0, (hipStream_t)stream,
-                     frame_origin(r), frame_origin(k), frame_origin(phi), T, X, XY, c, dt);
-  return check_hip(hipGetLastError(), "Nebo Assignment");
+  const auto fn = divk_table(form, shape);
+  const unsigned threads = geo.threads;
+  double *rp = frame_origin(r), *kp = frame_origin(k), *pp = frame_origin(phi);
+  return emit(stream, CallPtrs{}, [=](void* s, const CallPtrs&) {
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(threads), 0, (hipStream_t)s, rp, kp, pp, T, X, XY, c, dt);
+    return check_hip(hipGetLastError(), "Nebo Assignment");
+  });
 }
 
 int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* phi,
@@ -947,9 +951,13 @@ int launch_lapl_shape(int shape, const nebo_field_t* out, const nebo_field_t* ph
   const int nb = T.nx * T.ny * T.nz;
   const int X = out->glob[0], XY = out->glob[0] * out->glob[1];
   BackendScope bs(tile_backend(2, shape));
-  hipLaunchKernelGGL(lapl_table(shape), dim3(nb), dim3(geo.threads), 0, (hipStream_t)stream,
-                     frame_origin(out), frame_origin(phi), T, X, XY, c, sc);
-  return check_hip(hipGetLastError(), "Nebo Assignment");
+  const auto fn = lapl_table(shape);
+  const unsigned threads = geo.threads;
+  double *op = frame_origin(out), *pp = frame_origin(phi);
+  return emit(stream, CallPtrs{}, [=](void* s, const CallPtrs&) {
+    hipLaunchKernelGGL(fn, dim3(nb), dim3(threads), 0, (hipStream_t)s, op, pp, T, X, XY, c, sc);
+    return check_hip(hipGetLastError(), "Nebo Assignment");
+  });
 }
 
 // Periodic fill of the ghost slab of axis d: every ghost cell takes the value of
@@ -1138,6 +1146,7 @@ extern "C" int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t
     // unpin: tuned tiles and the size rule apply again
     g_pinned[kernel].store(-1);
     g_last[kernel].store(kDefaultShape);
+    bump_plan_epoch();
     return NEBO_OK;
   }
   const int s = find_shape(tx, ty, zchunk, variant);
@@ -1149,6 +1158,7 @@ extern "C" int nebo_stencil_set_tile(int kernel, int32_t tx, int32_t ty, int32_t
   }
   g_pinned[kernel].store(s);
   g_last[kernel].store(s);
+  bump_plan_epoch();
   return NEBO_OK;
 }
 
@@ -1244,6 +1254,7 @@ extern "C" int nebo_stencil_autotune(int kernel, const nebo_field_t* r, const ne
   }
   g_pinned[kernel].store(-1);   // the tuned tile takes effect for its box size
   g_last[kernel].store(best);
+  bump_plan_epoch();
   if (tx) *tx = kShapes[best].tx;
   if (ty) *ty = kShapes[best].ty;
   if (zchunk) *zchunk = kShapes[best].zc;

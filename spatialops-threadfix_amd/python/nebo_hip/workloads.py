@@ -10,8 +10,11 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import inputs
-from .expr import E, Stencil, cond
+import ctypes as C
+
+from . import _abi, inputs
+from .api import _stream
+from .expr import E, Stencil, cond, serialize
 from .fields import Field, Layout, Mask
 from .ops import basic_op_types, build_stencils
 
@@ -70,7 +73,6 @@ class HeatStepper:
 
     def __init__(self, n=64, slots=100):
         import torch
-        from .expr import serialize
         L, ops, s, mask = heat_problem(n)
         mask.add_consumer(0)
         self.n, self.layout = n, L
@@ -82,8 +84,12 @@ class HeatStepper:
         for src in (0, 1):
             e1, cond_of = heat_step_exprs(ops, self.fields[src], s, mask)
             dst = self.fields[1 - src]
-            self._plans.append((dst.desc("gpu"), serialize(e1, "gpu"), serialize(cond_of(dst), "gpu"),
-                                serialize(E(dst), "gpu")))
+            d, s1, s2, sf = dst.desc("gpu"), serialize(e1, "gpu"), serialize(cond_of(dst), "gpu"), serialize(E(dst), "gpu")
+            # ctypes arguments made once: a time loop only passes them again
+            self._plans.append((C.byref(d), s1.ref, s2.ref, sf.ref, (d, s1, s2, sf)))
+        self._red0 = self.red.data_ptr()
+        lib = _abi.lib()
+        self._assign, self._reduce = lib.nebo_assign, lib.nebo_reduce_async
         self._keep = (ops, mask)
 
     def reset(self):
@@ -91,17 +97,12 @@ class HeatStepper:
         self.fields[0].gpu.copy_(torch.from_numpy(self.phi0))
 
     def step(self, i, stream=None):
-        import ctypes as C
-        from . import _abi
-        from .api import _stream
-        lib = _abi.lib()
         sp = _stream(stream)
-        d, e1, e2, ef = self._plans[i & 1]
-        slot = i % self.slots
-        for rc in (lib.nebo_assign(C.byref(d), e1.ref, 1, sp), lib.nebo_assign(C.byref(d), e2.ref, 1, sp),
-                   lib.nebo_reduce_async(ef.ref, _abi.RED_MAX, 1, C.c_void_p(self.red[2 * slot].data_ptr()), sp),
-                   lib.nebo_reduce_async(ef.ref, _abi.RED_SUM, 1, C.c_void_p(self.red[2 * slot + 1].data_ptr()),
-                                         sp)):
+        d, e1, e2, ef, _ = self._plans[i & 1]
+        slot = self._red0 + 16 * (i % self.slots)   # red[2 * slot], then red[2 * slot + 1]
+        rc = (self._assign(d, e1, 1, sp) or self._assign(d, e2, 1, sp)
+              or self._reduce(ef, _abi.RED_MAX, 1, slot, sp) or self._reduce(ef, _abi.RED_SUM, 1, slot + 8, sp))
+        if rc:
             _abi.check(rc)
 
 
