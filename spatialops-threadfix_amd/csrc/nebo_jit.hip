@@ -338,11 +338,12 @@ void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int
 // raw buffer loads bounded by the family's allocation, so lanes and rows past
 // the box read 0 past it instead of faulting; their results are not stored.
 struct ColumnPlan {
-  int nf = 0, RY = 4;
+  int nf = 0, RY = 4, PF = 1;   // rows per lane; planes loaded ahead of the last plane in use
   int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
   int fam[kVmMaxLoads], dx[kVmMaxLoads], dy[kVmMaxLoads], dz[kVmMaxLoads];
 };
-constexpr int kColumnRegs = 96;   // ring doubles per lane, over all families
+constexpr int kColumnRegs = 96;    // ring doubles per lane, over all families
+constexpr int kColumnPrefetch = 1;  // planes loaded ahead, when the ring allows it
 // Defaults from a sweep at 256^3 (div(k grad phi) and Laplacian trees): one wave
 // per block measured as fast as 2-4 waves and gives small boxes the most
 // blocks; g_column_zc 0 picks the planes per block from the box size.
@@ -381,7 +382,6 @@ bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
   }
   if (cp->nf == 0) return false;
   bool stencil = false;
-  int regs1 = 0, regs0 = 0;   // ring doubles: per row of RY, and fixed
   for (int f = 0; f < cp->nf; ++f) {
     int lo[3] = {2, 2, 2}, hi[3] = {-2, -2, -2};
     for (int s = 0; s < kVmMaxLoads; ++s)
@@ -392,15 +392,25 @@ bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
     cp->cx[f] = lo[0] + (hi[0] - lo[0] == 2 ? 1 : 0);
     cp->ylo[f] = lo[1], cp->yhi[f] = hi[1], cp->zlo[f] = lo[2], cp->zhi[f] = hi[2];
     stencil = stencil || hi[1] > lo[1] || hi[2] > lo[2];
-    regs1 += hi[2] - lo[2] + 2;   // planes in use and the one loading
-    regs0 += (hi[1] - lo[1]) * (hi[2] - lo[2] + 2);
     for (int s = 0; s < kVmMaxLoads; ++s)
       if (cp->fam[s] == f) cp->dx[s] = off[s][0] - cp->cx[f], cp->dy[s] = off[s][1], cp->dz[s] = off[s][2];
   }
   if (!stencil) return false;
+  // ring doubles per lane at ry rows and pf planes ahead: per family, its rows
+  // (and its edge value when it reads x neighbours) times its planes in use + pf
+  auto ring_regs = [&](int ry, int pf) {
+    int n = 0;
+    for (int f = 0; f < cp->nf; ++f) {
+      bool edge = false;
+      for (int s = 0; s < kVmMaxLoads; ++s) edge = edge || (cp->fam[s] == f && cp->dx[s] != 0);
+      n += (ry + cp->yhi[f] - cp->ylo[f] + (edge ? 1 : 0)) * (cp->zhi[f] - cp->zlo[f] + 1 + pf);
+    }
+    return n;
+  };
   cp->RY = 0;
-  for (int ry = 4; ry >= 1 && !cp->RY; ry /= 2)
-    if (regs1 * ry + regs0 <= kColumnRegs) cp->RY = ry;
+  for (int pf = kColumnPrefetch; pf >= 1 && !cp->RY; --pf)
+    for (int ry = 4; ry >= 1 && !cp->RY; ry /= 2)
+      if (ring_regs(ry, pf) <= kColumnRegs) cp->RY = ry, cp->PF = pf;
   return cp->RY > 0;
 }
 
@@ -620,15 +630,18 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     if (has_edge(f)) o << ind << "  " << edge_reg(f, k) << " = ld_(rs_, eo" << f << "_);\n";
     o << ind << "}\n";
   };
-  // slots 0 .. np-1 hold planes z + zlo .. z + zhi; slot np receives plane
-  // z + zhi + 1 while step z computes (past the chunk end it is not used)
+  // nslots(f) = np + PF register slots per family (np planes in use): slots
+  // 0 .. np-1 hold planes z + zlo .. z + zhi, the next PF - 1 the planes after
+  // them, and the last receives plane z + zlo + nslots - 1 while step z computes
+  // (loads past the chunk end are not used; past the allocation they return 0)
+  auto nslots = [&](int f) { return cp.zhi[f] - cp.zlo[f] + 1 + cp.PF; };
   for (int f = 0; f < cp.nf; ++f) {
-    const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
+    const int rows = RY + cp.yhi[f] - cp.ylo[f], ns = nslots(f);
     for (int j = 0; j < rows; ++j)
-      for (int k = 0; k <= np; ++k) o << "  double " << ring(f, j, k) << " = 0.0;\n";
+      for (int k = 0; k < ns; ++k) o << "  double " << ring(f, j, k) << " = 0.0;\n";
     if (has_edge(f))
-      for (int k = 0; k <= np; ++k) o << "  double " << edge_reg(f, k) << " = 0.0;\n";
-    for (int k = 0; k < np; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
+      for (int k = 0; k < ns; ++k) o << "  double " << edge_reg(f, k) << " = 0.0;\n";
+    for (int k = 0; k + 1 < ns; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
   }
   // Cells: the program as a tree, with the subtrees of shift_plan taken from the
   // row above or the plane below instead of recomputed.  Carries are written
@@ -678,16 +691,15 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     for (const auto& w : writes) o << ind << w.first << " = " << w.second << ";\n";
     return res;
   };
-  // One z step at plane zq: the next plane's loads, then RY cells.  Ring slot k
-  // of family f (plane z + zlo + k; k = np is the plane loading) lives in
-  // register (k + u) mod (np + 1), so the unrolled steps u = 0 .. U-1 rename
-  // registers instead of copying them; the remainder loop (u = 0) copies.
+  // One z step at plane zq: the loads of the plane PF planes ahead, then RY
+  // cells.  Ring slot k of family f (plane z + zlo + k; the last slot is the
+  // plane loading) lives in register (k + u) mod nslots(f), so the unrolled
+  // steps u = 0 .. U-1 rename registers instead of copying them; the remainder
+  // loop (u = 0) copies.
   auto step = [&](const std::string& in, int u) {
-    auto slot = [&](int f, int k) { return (k + u) % (cp.zhi[f] - cp.zlo[f] + 2); };
-    for (int f = 0; f < cp.nf; ++f) {
-      const int np = cp.zhi[f] - cp.zlo[f] + 1;
-      load_plane(in, f, slot(f, np), "zq + " + std::to_string(cp.zhi[f] + 1));
-    }
+    auto slot = [&](int f, int k) { return (k + u) % nslots(f); };
+    for (int f = 0; f < cp.nf; ++f)
+      load_plane(in, f, slot(f, nslots(f) - 1), "zq + " + std::to_string(cp.zlo[f] + nslots(f) - 1));
     for (size_t c = 0; c < sp.cdir.size(); ++c)
       if (sp.cdir[c] == 1)
         for (int r = 0; r + 1 < RY; ++r) o << in << "double " << cy_name((int)c, r) << ";\n";
@@ -717,7 +729,7 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   };
   int U = 1;
   for (int f = 0; f < cp.nf; ++f) {
-    const int n = cp.zhi[f] - cp.zlo[f] + 2;
+    const int n = nslots(f);
     int a = U, b = n;
     while (b) { const int t = a % b; a = b; b = t; }
     U = U / a * n;
@@ -739,8 +751,8 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   o << "  for (; z < zend; ++z) {\n    const int zq = z;\n";
   step("    ", 0);
   for (int f = 0; f < cp.nf; ++f) {
-    const int rows = RY + cp.yhi[f] - cp.ylo[f], np = cp.zhi[f] - cp.zlo[f] + 1;
-    for (int k = 0; k < np; ++k) {
+    const int rows = RY + cp.yhi[f] - cp.ylo[f];
+    for (int k = 0; k + 1 < nslots(f); ++k) {
       for (int j = 0; j < rows; ++j) o << "    " << ring(f, j, k) << " = " << ring(f, j, k + 1) << ";\n";
       if (has_edge(f)) o << "    " << edge_reg(f, k) << " = " << edge_reg(f, k + 1) << ";\n";
     }
@@ -969,14 +981,15 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
   cs->zc = g_column_zc.load();
   cs->xcd = g_xcd.load();
   // Indices of the kernel's stores and per-lane offsets are 32-bit: lanes, rows
-  // and planes past the box reach at most 66 + X (RY waves + 2) + 3 XY past a
-  // slot's last valid index, which must not overflow (fields hold fewer than
-  // 2^31 points, so only fields within that margin of the limit keep the
+  // and planes past the box reach at most 66 + X (RY waves + 2) + (PF + 2) XY
+  // past a slot's last valid index, which must not overflow (fields hold fewer
+  // than 2^31 points, so only fields within that margin of the limit keep the
   // flattened schedule).
   for (int s = 0; s < kVmMaxLoads; ++s) {
     if (cp->fam[s] < 0) continue;
     const int k = P.loads[s].cls;
-    const long reach = (long)P.loads[s].ghi + 66 + (long)P.cls_X[k] * (cp->RY * cs->waves + 2) + 3L * P.cls_XY[k];
+    const long reach = (long)P.loads[s].ghi + 66 + (long)P.cls_X[k] * (cp->RY * cs->waves + 2) +
+                       (long)(cp->PF + 2) * P.cls_XY[k];
     if (reach >= 2147483647L) return false;
   }
   if (cs->zc == 0) {
@@ -999,6 +1012,7 @@ std::string column_key(const VmProgram& P, const ColumnPlan& cp, const ColumnSha
   std::string k = structural_key(P, 1, Shape{64 * cs.waves, 0, cs.xcd}, -2);
   auto put = [&k](int v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
   put(cp.RY);
+  put(cp.PF);
   put(cs.zc);
   put(cp.nf);
   for (int f = 0; f < cp.nf; ++f) {
