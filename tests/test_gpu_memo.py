@@ -187,3 +187,32 @@ def test_replays_run_on_the_stream_they_name(capfd):
     printed = capfd.readouterr().out
     assert printed.count("Starting Nebo HIP compiled kernel") == 1
     assert printed.count("Finished Nebo HIP compiled kernel") == 1
+
+
+def test_failed_calls_are_not_recorded():
+    """A call that fails is not recorded: the identical call fails again with the
+    same error (an in-place read by a stencil, an unbounded reduction), and a
+    valid call on the same fields afterwards runs as a freshly planned one."""
+    n = (24, 20, 16)
+    L = nh.Layout.build(n, 1)
+    rng = np.random.default_rng(63)
+    a = both(L, rng.standard_normal(L.npts))
+    ops = nh.basic_op_types(nh.build_stencils(*n, 1.0, 1.0, 1.0))
+    bad = a + ops["DivX"](ops["GradX"](a))
+    for _ in range(3):
+        with pytest.raises(nh.NeboError) as ei:
+            nh.assign(a, bad)
+        assert "read by a stencil" in str(ei.value)
+    c = nh.E(2.0) + 1.0
+    for _ in range(3):
+        with pytest.raises(nh.NeboError) as ei:
+            nh.field_max(c)
+        assert ei.value.code == _abi.NEBO_ERR_INVALID and "unbounded" in str(ei.value)
+    out = nh.Field.empty(L, "gpu")
+    good = ops["DivX"](ops["GradX"](a))
+    for _ in range(2):
+        out.gpu.zero_()
+        nh.assign(out, good)
+        ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+        orc.assign(ref, good)
+        assert np.array_equal(fetch(out), ref.host)
