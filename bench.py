@@ -422,12 +422,28 @@ def heat_leg(steps=100, cpu_steps=5, per_graph=10):
         orc.reduce(dst, nh._abi.RED_SUM)
     cpu_ms = (time.perf_counter() - t0) * 1e3 / cpu_steps
     return {"n": n, "steps": steps, "gpu_ms_per_step": round(gpu_ms, 4),
+            "cpp_front_end": cpp_heat_timing(n),
             "graph_ms_per_step": round(graph_ms, 4), "steps_per_graph": per_graph,
             "graph_equals_eager": same,
             "kernels_per_step": "Laplacian+Euler, cond mask (one kernel each), max, sum (block partials, then a fold: two kernels each)",
             "max_first_last": [float(maxes[0]), float(maxes[-1])],
             "max_decreasing": bool(np.all(np.diff(maxes) <= 0)),
             "oracle_cpu_ms_per_step_serial": round(cpu_ms, 3), "cpu_steps_timed": cpu_steps}
+
+
+def cpp_heat_timing(n, steps=200):
+    """The time loop of examples/heat_equation.cpp (three assignments per step,
+    the z terms added) at n^3 through the C++ front end: cpp/bin/heat_timing, run
+    as a child process; its JSON line, or a note when it is not built or fails."""
+    import subprocess
+    exe = os.path.join(ROOT, "spatialops-threadfix_amd", "cpp", "bin", "heat_timing")
+    if not os.path.exists(exe):
+        return {"skipped": "cpp/bin/heat_timing not built"}
+    r = subprocess.run([exe, str(n), str(steps)], capture_output=True, text=True, timeout=120)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"failed": r.returncode, "stderr": r.stderr[-300:]}
+    return json.loads(lines[-1])
 
 
 def particles_leg(n=128, count=4_000_000, reps=5):
