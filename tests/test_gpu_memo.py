@@ -216,3 +216,51 @@ def test_failed_calls_are_not_recorded():
         ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
         orc.assign(ref, good)
         assert np.array_equal(fetch(out), ref.host)
+
+
+def test_concurrent_threads_replay_their_own_calls():
+    """Two host threads, each on its own stream, repeat their own assignment and
+    reduction 200 times while the other does the same (the memo is shared, the
+    recording is per thread): every result equals the oracle's."""
+    import threading
+    n = (48, 40, 36)
+    L = nh.Layout.build(n, 1)
+    ops = nh.basic_op_types(nh.build_stencils(*n, 1.0, 1.0, 1.0))
+    lib = _abi.lib()
+    jobs = []
+    for j in range(2):
+        rng = np.random.default_rng(70 + j)
+        k = both(L, rng.uniform(0.5, 2.0, L.npts))
+        phi = both(L, rng.standard_normal(L.npts))
+        tree = ops["DivX"](ops["InterpC2FX"](k) * ops["GradX"](phi)) + ops["DivY"](ops["InterpC2FY"](k) * ops["GradY"](phi)) \
+            + ops["DivZ"](ops["InterpC2FZ"](k) * ops["GradZ"](phi))
+        tree = tree if j == 0 else phi * k + 3.0
+        out = nh.Field.empty(L, "gpu")
+        out.gpu.zero_()
+        res = torch.zeros(1, dtype=torch.float64, device="cuda")
+        jobs.append(dict(tree=tree, out=out, desc=out.desc("gpu"), s=serialize(E(tree), "gpu"),
+                         red=serialize(E(phi * k), "gpu"), res=res, stream=torch.cuda.Stream(), err=[],
+                         k=k, phi=phi))
+    torch.cuda.synchronize()
+
+    def run(job):
+        try:
+            sp = _stream(job["stream"])
+            for _ in range(200):
+                _abi.check(lib.nebo_assign(C.byref(job["desc"]), job["s"].ref, 1, sp))
+                _abi.check(lib.nebo_reduce_async(job["red"].ref, _abi.RED_MAX, 1,
+                                                 C.c_void_p(job["res"].data_ptr()), sp))
+            job["stream"].synchronize()
+        except Exception as ex:   # reported by the main thread
+            job["err"].append(repr(ex))
+    threads = [threading.Thread(target=run, args=(job,)) for job in jobs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    for j, job in enumerate(jobs):
+        assert not job["err"], job["err"]
+        ref = nh.Field.from_numpy(L, np.zeros(L.npts), where="cpu")
+        orc.assign(ref, job["tree"])
+        assert np.array_equal(fetch(job["out"]), ref.host), j
+        assert job["res"].item() == orc.reduce(job["phi"] * job["k"], _abi.RED_MAX), j
