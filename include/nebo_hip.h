@@ -268,14 +268,18 @@ int nebo_fused_jit_tune(int32_t block_threads, int32_t tile_x, int32_t max_cells
                         int32_t load_budget, int32_t xcd_remap);
 /* Column schedule of the compiled assignment kernels.  It applies to a tree
  * without masks whose unguarded reads of each field lie within a 3 x 3 x 3
- * neighbourhood and that reads some y or z neighbour.  A wave covers 64
- * consecutive x points; each lane holds 1, 2 or 4 rows (as many as fit 80
- * register doubles), walks zchunk planes and loads each plane of each field
- * once, one plane ahead of its use.  x neighbours come from the adjacent lanes.
- * Other trees keep the shape of nebo_fused_jit_tune.  mode 1 uses it where it
- * applies (the default), 0 never; waves per block in [1, 16], zchunk in
- * [1, 4096]; waves 0 keeps the default (1); zchunk 0 (the default) takes the
- * most of 128, 64, 32 or 16 planes that still gives 4 waves per compute unit, else 8.
+ * neighbourhood and that reads some y or z neighbour.  Each lane holds one
+ * column, or two adjacent columns (16-byte loads and stores; the x face its two
+ * cells share is formed once) for a tree reading two or more fields on a box
+ * wider than 64 columns where the register ring of 96 doubles fits 2 or more
+ * rows that way; a wave covers 64 or 128 consecutive x points, each lane 1, 2
+ * or 4 rows, walks zchunk planes and loads each plane of each field once, one
+ * plane ahead of its use.  x neighbours come from the adjacent lanes.  Other
+ * trees keep the shape of nebo_fused_jit_tune.  mode 1 uses it where it
+ * applies (the default), 2 the same with one column per lane, 0 never; waves
+ * per block (stacked in y) in [1, 16], 0 (the default) for 4 with two columns
+ * per lane and 1 with one; zchunk in [1, 4096], 0 (the default) for the most of
+ * 128, 64, 32 or 16 planes that still gives 4 waves per compute unit, else 8.
  * Results are the same bits either way. */
 int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk);
 int nebo_fused_jit_stats(int32_t* compiled, int32_t* cache_hits, int32_t* failures);

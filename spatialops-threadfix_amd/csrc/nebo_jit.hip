@@ -339,20 +339,31 @@ void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int
 // the box read 0 past it instead of faulting; their results are not stored.
 struct ColumnPlan {
   int nf = 0, RY = 4, PF = 1;   // rows per lane; planes loaded ahead of the last plane in use
+  bool pair = false;            // two adjacent columns per lane (16-byte loads and stores)
   int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
   int fam[kVmMaxLoads], dx[kVmMaxLoads], dy[kVmMaxLoads], dz[kVmMaxLoads];
 };
 constexpr int kColumnRegs = 96;    // ring doubles per lane, over all families
 constexpr int kColumnPrefetch = 1;  // planes loaded ahead, when the ring allows it
-// Defaults from a sweep at 256^3 (div(k grad phi) and Laplacian trees): one wave
-// per block measured as fast as 2-4 waves and gives small boxes the most
-// blocks; g_column_zc 0 picks the planes per block from the box size.
-constexpr int kDefaultColumnWaves = 1, kDefaultColumnZc = 8;
-std::atomic<int> g_column{1}, g_column_waves{kDefaultColumnWaves}, g_column_zc{0};
+// Two columns per lane where the ring fits 2 or more rows that way, for trees of
+// two or more families on boxes wider than 64 columns.  At 256^3 (one process,
+// clocks settled): div(k grad phi) 67.5 us with two columns and 4 stacked waves
+// against 71.5 with one column and one wave (the hand-written kernel 67.2); the
+// one-family Laplacian 43.1 us with one column against 44.6 with two.
+constexpr bool kColumnPairs = true;
+// Defaults from sweeps at 256^3 (div(k grad phi) and Laplacian trees): with one
+// column per lane, one wave per block measured as fast as 2-4 waves and gives
+// small boxes the most blocks; two columns per lane run 4 waves stacked in y
+// (div(k grad phi) 67.5 us against 69.9 with one).  g_column_waves 0 takes
+// these; g_column_zc 0 picks the planes per block from the box size.
+constexpr int kDefaultColumnWaves = 1, kDefaultPairWaves = 4, kDefaultColumnZc = 8;
+std::atomic<int> g_column{1}, g_column_waves{0}, g_column_zc{0};
 
 long round_div(long a, long b) { return a >= 0 ? (a + b / 2) / b : -((-a + b / 2) / b); }
 
-bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
+// pairs: two columns per lane allowed (column mode 1 on a box wider than 64
+// columns; mode 2 keeps one)
+bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp, bool pairs)
 {
   for (int m = 0; m < kVmMaxMasks; ++m)
     if (u.mused[m]) return false;   // mask bits past the box are not addressable
@@ -397,20 +408,25 @@ bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
   }
   if (!stencil) return false;
   // ring doubles per lane at ry rows and pf planes ahead: per family, its rows
-  // (and its edge value when it reads x neighbours) times its planes in use + pf
-  auto ring_regs = [&](int ry, int pf) {
+  // (two values each with two columns per lane, and its edge value when it
+  // reads x neighbours) times its planes in use + pf
+  auto ring_regs = [&](int ry, int pf, bool pair) {
     int n = 0;
     for (int f = 0; f < cp->nf; ++f) {
       bool edge = false;
       for (int s = 0; s < kVmMaxLoads; ++s) edge = edge || (cp->fam[s] == f && cp->dx[s] != 0);
-      n += (ry + cp->yhi[f] - cp->ylo[f] + (edge ? 1 : 0)) * (cp->zhi[f] - cp->zlo[f] + 1 + pf);
+      n += ((ry + cp->yhi[f] - cp->ylo[f]) * (pair ? 2 : 1) + (edge ? 1 : 0)) * (cp->zhi[f] - cp->zlo[f] + 1 + pf);
     }
     return n;
   };
+  // two columns per lane where 4 or 2 rows fit, else one column with 4, 2 or 1
   cp->RY = 0;
-  for (int pf = kColumnPrefetch; pf >= 1 && !cp->RY; --pf)
+  for (int pf = kColumnPrefetch; pf >= 1 && !cp->RY; --pf) {
+    for (int ry = 4; pairs && cp->nf >= 2 && ry >= 2 && !cp->RY; ry /= 2)
+      if (ring_regs(ry, pf, true) <= kColumnRegs) cp->RY = ry, cp->PF = pf, cp->pair = true;
     for (int ry = 4; ry >= 1 && !cp->RY; ry /= 2)
-      if (ring_regs(ry, pf) <= kColumnRegs) cp->RY = ry, cp->PF = pf;
+      if (ring_regs(ry, pf, false) <= kColumnRegs) cp->RY = ry, cp->PF = pf;
+  }
   return cp->RY > 0;
 }
 
@@ -448,9 +464,9 @@ std::vector<TNode> program_tree(const VmProgram& P)
 // reused: exchanging a value between lanes costs as many instructions as a face.
 struct ShiftPlan {
   std::vector<int> from;    // node -> the node B it is taken from, or -1
-  std::vector<int> dir;     // 1: B one row down (y), 2: B one plane down (z)
+  std::vector<int> dir;     // 1: B one row down (y), 2: B one plane down (z), 3: B the lane's first column (x)
   std::vector<int> carry;   // node B -> its carry index, or -1
-  std::vector<int> cdir;    // carry index -> 1 (row to row) or 2 (plane to plane)
+  std::vector<int> cdir;    // carry index -> 1 (row to row), 2 (plane to plane), 3 (column to column)
 };
 
 ShiftPlan shift_plan(const VmProgram& P, const ColumnPlan& cp, const std::vector<TNode>& t)
@@ -512,7 +528,9 @@ ShiftPlan shift_plan(const VmProgram& P, const ColumnPlan& cp, const std::vector
       if (a == b || claimed[a] || sig[a] != sig[b]) continue;
       if ((a >= first[b] && a <= b) || (b >= first[a] && b <= a)) continue;
       const int dx = lo[3 * b] - lo[3 * a], dy = lo[3 * b + 1] - lo[3 * a + 1], dz = lo[3 * b + 2] - lo[3 * a + 2];
-      const int d = dx == 0 && dy == 1 && dz == 0 ? 1 : dx == 0 && dy == 0 && dz == 1 ? 2 : 0;
+      const int d = dx == 0 && dy == 1 && dz == 0   ? 1
+                    : dx == 0 && dy == 0 && dz == 1 ? 2
+                    : dx == 1 && dy == 0 && dz == 0 && cp.pair ? 3 : 0;
       if (!d) continue;
       sp.from[a] = b;
       sp.dir[a] = d;
@@ -554,6 +572,17 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
        "  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);\n"
        "  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);\n"
        "  return __hiloint2double(hi, lo);\n}\n";
+  const int PW = cp.pair ? 2 : 1;   // columns per lane
+  if (cp.pair) {
+    // two adjacent doubles in one 16-byte access (8-byte aligned)
+    o << "#ifndef NEBO_JIT_D2\ntypedef double nebo_jit_d2_ __attribute__((ext_vector_type(2), aligned(8)));\n"
+         "#define NEBO_JIT_D2 nebo_jit_d2_\n#endif\n";
+    o << "__device__ __forceinline__ void ld2_(__amdgpu_buffer_rsrc_t r, unsigned off, double& a, double& b)\n{\n"
+         "  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);\n"
+         "  a = __hiloint2double((int)v[1], (int)v[0]);\n  b = __hiloint2double((int)v[3], (int)v[2]);\n}\n";
+    o << "__device__ __forceinline__ void st2_(double* p, double a, double b)\n{\n"
+         "  NEBO_JIT_D2 v;\n  v[0] = a;\n  v[1] = b;\n  __builtin_nontemporal_store(v, (NEBO_JIT_D2*)p);\n}\n";
+  }
   o << "extern \"C\" __global__ __launch_bounds__(" << 64 * NW << ") void nebo_fused(const JitArgs A)\n{\n";
   if (cs.xcd) {
     o << "  const int L = (int)blockIdx.x, d = L % " << kXcds << ";\n";
@@ -563,21 +592,26 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     o << "  const int b = (int)blockIdx.x;\n";
   }
   o << "  const int lane_ = (int)threadIdx.x & 63, w_ = (int)threadIdx.x >> 6;\n";
-  o << "  const int ncx = (A.n0 + 63) >> 6, ncy = (A.n1 + " << RY * NW - 1 << ") / " << RY * NW << ";\n";
+  o << "  const int ncx = (A.n0 + " << 64 * PW - 1 << ") >> " << (cp.pair ? 7 : 6) << ", ncy = (A.n1 + "
+    << RY * NW - 1 << ") / " << RY * NW << ";\n";
   o << "  const int bz = b / (ncx * ncy), bxy = b - bz * (ncx * ncy), by = bxy / ncx, bx = bxy - by * ncx;\n";
-  o << "  const int x = A.lo0 + bx * 64 + lane_;\n";
+  if (cp.pair) o << "  const int x = A.lo0 + bx * 128 + 2 * lane_, xc_ = x;\n";
+  else o << "  const int x = A.lo0 + bx * 64 + lane_;\n";
   o << "  const int y0 = A.lo1 + (by * " << NW << " + w_) * " << RY << ";\n";
   o << "  const int zs = A.lo2 + bz * " << cs.zc << ";\n";
   o << "  const int zend = zs + " << cs.zc << " < A.lo2 + A.n2 ? zs + " << cs.zc << " : A.lo2 + A.n2;\n";
   o << "  const bool xin_ = x < A.lo0 + A.n0;\n  const int ye_ = A.lo1 + A.n1;\n";
-  auto ring = [](int f, int j, int k) {
-    return "R" + std::to_string(f) + "_" + std::to_string(j) + "_" + std::to_string(k);
+  if (cp.pair) o << "  const bool xin1_ = x + 1 < A.lo0 + A.n0;\n";
+  // ring value of family f, row j, plane slot k (column h of the lane's two)
+  auto ring = [&](int f, int j, int k, int h = 0) {
+    return "R" + std::to_string(f) + "_" + std::to_string(j) + "_" + std::to_string(k) +
+           (cp.pair ? "_" + std::to_string(h) : std::string());
   };
   // Edge reads: per family, one more load per plane serves every x-shifted read
   // of that plane: lanes 0 .. nE-1 read column x0 - 1 of rows ey .. ey+nE-1 (ey
   // the lowest dy of the family's x-shifted reads, nE = RY + their dy span) when
-  // the family reads x - 1, lanes 32 .. 32+nE-1 column x0 + 64 when it reads
-  // x + 1; other lanes re-read a ring row.  The value travels with its plane in
+  // the family reads x - 1, lanes 32 .. 32+nE-1 column x0 + 64 (x0 + 128 with
+  // two columns per lane) when it reads x + 1; other lanes re-read a ring row.  The value travels with its plane in
   // the register ring, loaded the same number of planes ahead as the rows.  A
   // side the family does not read may lie outside the region its reads cover,
   // below the allocation's start where the range check does not reach (a
@@ -604,14 +638,15 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
         << "] * (y0 - A.lo1 + " << j << ")) * 8u;\n";
     if (!has_edge(f)) continue;
     const int nE = RY + ef[f].eyhi - ef[f].ey, dyb = ef[f].ey - cp.ylo[f];
-    // x of the wave's first lane is x - lane_; lane i holds row y0 + ey + i
+    // x of the wave's first lane is x - PW lane_; lane i holds row y0 + ey + i
+    const std::string x0 = cp.pair ? "x - 2 * lane_" : "x - lane_";
     o << "  const unsigned eo" << f << "_ = ";
     if (ef[f].left)
-      o << "lane_ < " << nE << " ? (unsigned)((x - lane_ - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ + "
+      o << "lane_ < " << nE << " ? (unsigned)((" << x0 << " - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ + "
         << dyb << ")) * 8u : ";
     if (ef[f].right)
-      o << "(lane_ >= 32 && lane_ < " << 32 + nE << ") ? (unsigned)((x - lane_ + 65 - A.lo0) + A.clsX[" << cls
-        << "] * (y0 - A.lo1 + lane_ - 32 + " << dyb << ")) * 8u : ";
+      o << "(lane_ >= 32 && lane_ < " << 32 + nE << ") ? (unsigned)((" << x0 << " + " << 64 * PW + 1
+        << " - A.lo0) + A.clsX[" << cls << "] * (y0 - A.lo1 + lane_ - 32 + " << dyb << ")) * 8u : ";
     o << "o" << f << "_" << -cp.ylo[f] << ";\n";
   }
   // the resource of family f at plane zexpr
@@ -625,8 +660,12 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   auto load_plane = [&](const std::string& ind, int f, int k, const std::string& zexpr) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f];
     o << ind << "{\n" << ind << "  const __amdgpu_buffer_rsrc_t rs_ = " << rsrc(f, zexpr) << ";\n";
-    for (int j = 0; j < rows; ++j)
-      o << ind << "  " << ring(f, j, k) << " = ld_(rs_, o" << f << "_" << j << ");\n";
+    for (int j = 0; j < rows; ++j) {
+      if (cp.pair)
+        o << ind << "  ld2_(rs_, o" << f << "_" << j << ", " << ring(f, j, k, 0) << ", " << ring(f, j, k, 1) << ");\n";
+      else
+        o << ind << "  " << ring(f, j, k) << " = ld_(rs_, o" << f << "_" << j << ");\n";
+    }
     if (has_edge(f)) o << ind << "  " << edge_reg(f, k) << " = ld_(rs_, eo" << f << "_);\n";
     o << ind << "}\n";
   };
@@ -638,7 +677,8 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   for (int f = 0; f < cp.nf; ++f) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f], ns = nslots(f);
     for (int j = 0; j < rows; ++j)
-      for (int k = 0; k < ns; ++k) o << "  double " << ring(f, j, k) << " = 0.0;\n";
+      for (int k = 0; k < ns; ++k)
+        for (int h = 0; h < PW; ++h) o << "  double " << ring(f, j, k, h) << " = 0.0;\n";
     if (has_edge(f))
       for (int k = 0; k < ns; ++k) o << "  double " << edge_reg(f, k) << " = 0.0;\n";
     for (int k = 0; k + 1 < ns; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
@@ -649,9 +689,15 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   // the previous plane's value, whichever operand order the tree has.
   const std::vector<TNode> tree = program_tree(P);
   const ShiftPlan sp = shift_plan(P, cp, tree);
-  auto cy_name = [](int c, int r) { return "cy" + std::to_string(c) + "_" + std::to_string(r); };
-  auto cz_name = [](int c, int r) { return "cz" + std::to_string(c) + "_" + std::to_string(r); };
-  auto emit_row = [&](const std::string& ind, int r) {
+  // carries of cell (row r, column h): per column with two columns per lane
+  auto cy_name = [&](int c, int r, int h = 0) {
+    return "cy" + std::to_string(c) + "_" + std::to_string(r) + (cp.pair ? "_" + std::to_string(h) : std::string());
+  };
+  auto cz_name = [&](int c, int r, int h = 0) {
+    return "cz" + std::to_string(c) + "_" + std::to_string(r) + (cp.pair ? "_" + std::to_string(h) : std::string());
+  };
+  auto cx_name = [](int c, int r) { return "cx" + std::to_string(c) + "_" + std::to_string(r); };
+  auto emit_row = [&](const std::string& ind, int r, int h) {
     int tmp = 0;
     std::vector<std::pair<std::string, std::string>> writes;
     std::function<std::string(int, const std::string&, bool)> ev = [&](int i, const std::string& in,
@@ -659,14 +705,15 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       const TNode& a = tree[i];
       if (!computed && sp.from[i] >= 0) {
         const int c = sp.carry[sp.from[i]];
-        if (sp.dir[i] == 1 && r > 0) return cy_name(c, r - 1);
+        if (sp.dir[i] == 1 && r > 0) return cy_name(c, r - 1, h);
+        if (sp.dir[i] == 3 && h == 1) return cx_name(c, r);   // the lane's first column formed it
         if (sp.dir[i] == 2) {
           // the chunk's first plane computes it; later planes take the carry
           const std::string v = "zc" + std::to_string(tmp++) + "_";
           o << in << "double " << v << ";\n" << in << "if (zq == zs) {\n";
           const std::string w = ev(i, in + "  ", true);
           o << in << "  " << v << " = " << w << ";\n" << in << "} else {\n"
-            << in << "  " << v << " = " << cz_name(c, r) << ";\n" << in << "}\n";
+            << in << "  " << v << " = " << cz_name(c, r, h) << ";\n" << in << "}\n";
           return v;
         }
       }
@@ -682,8 +729,12 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       }
       if (sp.carry[i] >= 0) {
         const int c = sp.carry[i];
-        if (sp.cdir[c] == 2) writes.emplace_back(cz_name(c, r), name);
-        else if (r + 1 < RY) writes.emplace_back(cy_name(c, r), name);
+        if (sp.cdir[c] == 2) writes.emplace_back(cz_name(c, r, h), name);
+        else if (sp.cdir[c] == 3) {
+          if (h == 0) writes.emplace_back(cx_name(c, r), name);
+        } else if (r + 1 < RY) {
+          writes.emplace_back(cy_name(c, r, h), name);
+        }
       }
       return name;
     };
@@ -702,29 +753,57 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       load_plane(in, f, slot(f, nslots(f) - 1), "zq + " + std::to_string(cp.zlo[f] + nslots(f) - 1));
     for (size_t c = 0; c < sp.cdir.size(); ++c)
       if (sp.cdir[c] == 1)
-        for (int r = 0; r + 1 < RY; ++r) o << in << "double " << cy_name((int)c, r) << ";\n";
-    for (int r = 0; r < RY; ++r) {
-      const std::string ind = in + "  ";
-      o << in << "{\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = zq;\n";
+        for (int r = 0; r + 1 < RY; ++r)
+          for (int h = 0; h < PW; ++h) o << in << "double " << cy_name((int)c, r, h) << ";\n";
+    // the reads of cell (row r, column h) as L0_<slot>
+    auto bind_reads = [&](const std::string& ind, int r, int h) {
       for (int s = 0; s < kVmMaxLoads; ++s) {
         const int f = cp.fam[s];
         if (f < 0) continue;
-        const std::string R = ring(f, r + cp.dy[s] - cp.ylo[f], slot(f, cp.dz[s] - cp.zlo[f]));
+        const int j = r + cp.dy[s] - cp.ylo[f], k = slot(f, cp.dz[s] - cp.zlo[f]);
+        const int e = (cp.dx[s] > 0 ? 32 : 0) + r + cp.dy[s] - ef[f].ey;   // lane holding the edge value
         if (cp.dx[s] == 0) {
-          o << ind << "const double L0_" << s << " = " << R << ";\n";
+          o << ind << "const double L0_" << s << " = " << ring(f, j, k, h) << ";\n";
+          continue;
+        }
+        if (cp.pair && (cp.dx[s] < 0) == (h == 1)) {   // the neighbour is the lane's other column
+          o << ind << "const double L0_" << s << " = " << ring(f, j, k, 1 - h) << ";\n";
           continue;
         }
         // the exchange is taken by every lane, outside any branch
-        o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << R << ", "
-          << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
+        o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << ring(f, j, k, cp.dx[s] < 0 ? PW - 1 : 0)
+          << ", " << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
         o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? rdlane_("
-          << edge_reg(f, slot(f, cp.dz[s] - cp.zlo[f])) << ", " << (cp.dx[s] > 0 ? 32 : 0) + r + cp.dy[s] - ef[f].ey
-          << ") : D0_" << s << ";\n";
+          << edge_reg(f, k) << ", " << e << ") : D0_" << s << ";\n";
       }
-      emit_reads(o, P, direct, 1, true, ind);
-      const std::string res = emit_row(ind, r);
-      o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res
-        << ", &A.out[x + A.oX * y + A.oXY * zq]);\n" << in << "}\n";
+    };
+    for (int r = 0; r < RY; ++r) {
+      const std::string ind = in + "  ";
+      o << in << "{\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = zq;\n";
+      if (!cp.pair) {
+        bind_reads(ind, r, 0);
+        emit_reads(o, P, direct, 1, true, ind);
+        const std::string res = emit_row(ind, r, 0);
+        o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res
+          << ", &A.out[x + A.oX * y + A.oXY * zq]);\n" << in << "}\n";
+        continue;
+      }
+      // two columns: the first forms the x faces the second takes over
+      for (size_t c = 0; c < sp.cdir.size(); ++c)
+        if (sp.cdir[c] == 3) o << ind << "double " << cx_name((int)c, r) << ";\n";
+      o << ind << "double v0_, v1_;\n";
+      for (int h = 0; h < 2; ++h) {
+        const std::string in2 = ind + "  ";
+        o << ind << "{\n" << in2 << "const int x = xc_ + " << h << ";\n";
+        bind_reads(in2, r, h);
+        emit_reads(o, P, direct, 1, true, in2);
+        const std::string res = emit_row(in2, r, h);
+        o << in2 << "v" << h << "_ = " << res << ";\n" << ind << "}\n";
+      }
+      o << ind << "if (y < ye_) {\n"
+        << ind << "  if (xin1_) st2_(&A.out[x + A.oX * y + A.oXY * zq], v0_, v1_);\n"
+        << ind << "  else if (xin_) __builtin_nontemporal_store(v0_, &A.out[x + A.oX * y + A.oXY * zq]);\n"
+        << ind << "}\n" << in << "}\n";
     }
   };
   int U = 1;
@@ -737,7 +816,8 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   if (U > 6) U = 1;   // families of different depths: copies rather than a long unroll
   for (size_t c = 0; c < sp.cdir.size(); ++c)
     if (sp.cdir[c] == 2)
-      for (int r = 0; r < RY; ++r) o << "  double " << cz_name((int)c, r) << " = 0.0;\n";
+      for (int r = 0; r < RY; ++r)
+        for (int h = 0; h < PW; ++h) o << "  double " << cz_name((int)c, r, h) << " = 0.0;\n";
   o << "  int z = zs;   // column schedule\n";
   if (U > 1) {
     o << "  for (; z + " << U - 1 << " < zend; z += " << U << ") {\n";
@@ -753,7 +833,8 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   for (int f = 0; f < cp.nf; ++f) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f];
     for (int k = 0; k + 1 < nslots(f); ++k) {
-      for (int j = 0; j < rows; ++j) o << "    " << ring(f, j, k) << " = " << ring(f, j, k + 1) << ";\n";
+      for (int j = 0; j < rows; ++j)
+        for (int h = 0; h < PW; ++h) o << "    " << ring(f, j, k, h) << " = " << ring(f, j, k + 1, h) << ";\n";
       if (has_edge(f)) o << "    " << edge_reg(f, k) << " = " << edge_reg(f, k + 1) << ";\n";
     }
   }
@@ -976,19 +1057,22 @@ std::string structural_key(const VmProgram& P, int V, const Shape& sh, int kind)
 // transport stages 1.51-1.60 ms against 1.56-1.68 at 64), else 8.
 bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp, ColumnShape* cs)
 {
-  if (!g_column.load() || !column_plan(P, slot_use(P), cp)) return false;
+  const int mode = g_column.load();
+  const bool wide = !lim || lim->hi[0] - lim->lo[0] > 64;
+  if (!mode || !column_plan(P, slot_use(P), cp, kColumnPairs && mode == 1 && wide)) return false;
   cs->waves = g_column_waves.load();
+  if (cs->waves == 0) cs->waves = cp->pair ? kDefaultPairWaves : kDefaultColumnWaves;
   cs->zc = g_column_zc.load();
   cs->xcd = g_xcd.load();
   // Indices of the kernel's stores and per-lane offsets are 32-bit: lanes, rows
-  // and planes past the box reach at most 66 + X (RY waves + 2) + (PF + 2) XY
-  // past a slot's last valid index, which must not overflow (fields hold fewer
-  // than 2^31 points, so only fields within that margin of the limit keep the
-  // flattened schedule).
+  // and planes past the box reach at most 64 PW + 2 + X (RY waves + 2) +
+  // (PF + 2) XY past a slot's last valid index (PW columns per lane), which must
+  // not overflow (fields hold fewer than 2^31 points, so only fields within that
+  // margin of the limit keep the flattened schedule).
   for (int s = 0; s < kVmMaxLoads; ++s) {
     if (cp->fam[s] < 0) continue;
     const int k = P.loads[s].cls;
-    const long reach = (long)P.loads[s].ghi + 66 + (long)P.cls_X[k] * (cp->RY * cs->waves + 2) +
+    const long reach = (long)P.loads[s].ghi + (cp->pair ? 130 : 66) + (long)P.cls_X[k] * (cp->RY * cs->waves + 2) +
                        (long)(cp->PF + 2) * P.cls_XY[k];
     if (reach >= 2147483647L) return false;
   }
@@ -996,7 +1080,8 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
     cs->zc = kDefaultColumnZc;
     int device = 0;
     if (lim && hipGetDevice(&device) == hipSuccess) {
-      const long cols = (long)((lim->hi[0] - lim->lo[0] + 63) / 64) *
+      const int xw = cp->pair ? 128 : 64;   // columns per wave
+      const long cols = (long)((lim->hi[0] - lim->lo[0] + xw - 1) / xw) *
                         ((lim->hi[1] - lim->lo[1] + cp->RY * cs->waves - 1) / (cp->RY * cs->waves)) * cs->waves;
       const long want = 4L * device_cu_count(device);
       const int nz = lim->hi[2] - lim->lo[2];
@@ -1013,6 +1098,7 @@ std::string column_key(const VmProgram& P, const ColumnPlan& cp, const ColumnSha
   auto put = [&k](int v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
   put(cp.RY);
   put(cp.PF);
+  put(cp.pair ? 1 : 0);
   put(cs.zc);
   put(cp.nf);
   for (int f = 0; f < cp.nf; ++f) {
@@ -1166,7 +1252,8 @@ int column_args(const nebo_field_t* lhs, const VmProgram& P, const nebo_limits_t
   A->out = frame_origin(lhs);
   A->oX = lhs->glob[0];
   A->oXY = lhs->glob[0] * lhs->glob[1];
-  *total = (((long)A->n0 + 63) / 64) * (((long)A->n1 + cp.RY * cs.waves - 1) / (cp.RY * cs.waves)) *
+  const long xw = cp.pair ? 128 : 64;   // columns per wave
+  *total = (((long)A->n0 + xw - 1) / xw) * (((long)A->n1 + cp.RY * cs.waves - 1) / (cp.RY * cs.waves)) *
            (((long)A->n2 + cs.zc - 1) / cs.zc);
   if (*total >= (1L << 31)) return set_error(NEBO_ERR_TOO_LARGE, "Nebo Fused Kernel", "more than 2^31 blocks");
   A->nblocks = (int)*total;
@@ -1270,10 +1357,10 @@ extern "C" int nebo_fused_jit_column(int32_t mode, int32_t waves, int32_t zchunk
 {
   nebo::clear_error();
   static const char* ctx = "Nebo Fused Kernel";
-  const int w = waves ? waves : nebo::kDefaultColumnWaves;
+  const int w = waves;   // 0: 4 with two columns per lane, else 1
   const int z = zchunk;   // 0: from the box size
-  if (mode != 0 && mode != 1) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column mode must be 0 or 1");
-  if (w < 1 || w > 16) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column waves must be in [1, 16]");
+  if (mode < 0 || mode > 2) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column mode must be 0, 1 or 2");
+  if (w < 0 || w > 16) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column waves must be in [1, 16] or 0");
   if (z < 0 || z > 4096) return nebo::set_error(NEBO_ERR_INVALID, ctx, "column z chunk must be in [1, 4096] or 0");
   nebo::g_column.store(mode);
   nebo::g_column_waves.store(w);

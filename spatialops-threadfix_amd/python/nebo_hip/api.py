@@ -135,10 +135,13 @@ def fused_jit_stats():
 def fused_jit_column(enable=True, waves=0, zchunk=0):
     """Column schedule of the compiled assignment kernels (on by default): for
     trees without masks whose reads of each field stay within one point in x, y
-    and z, a wave covers 64 x points, each lane 1-4 rows and `zchunk` planes, and
-    each plane of each field is loaded once.  0 keeps the defaults (1 wave per
-    block; planes per block from the box size: 128, 64, 32, 16 or 8)."""
-    _abi.call("nebo_fused_jit_column", int(bool(enable)), int(waves), int(zchunk))
+    and z, a lane holds one column (or two adjacent ones where the register ring
+    allows), each lane 1-4 rows and `zchunk` planes, and each plane of each field
+    is loaded once.  enable: True/1 on, 2 on with one column per lane, False/0
+    off.  0 keeps the defaults (1 wave per block; planes per block from the box
+    size: 128, 64, 32, 16 or 8)."""
+    mode = 2 if enable == 2 else int(bool(enable))
+    _abi.call("nebo_fused_jit_column", mode, int(waves), int(zchunk))
 
 
 def fused_jit_tune(block_threads=0, tile_x=-1, max_cells=0, load_budget=0, xcd_remap=-1):

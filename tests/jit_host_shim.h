@@ -70,6 +70,33 @@ static inline ShimU2 shim_load_b64(ShimRsrc r, unsigned off, int, int)
 }
 #define __builtin_amdgcn_raw_buffer_load_b64 shim_load_b64
 
+// A 16-byte load (two adjacent doubles): each 8-byte half inside the range must
+// address registered memory (a half past the range reads nothing).
+typedef unsigned ShimU4 __attribute__((vector_size(16)));
+static inline ShimU4 shim_load_b128(ShimRsrc r, unsigned off, int, int)
+{
+  ShimU4 v = {0u, 0u, 0u, 0u};
+  ++g_loads;
+  for (int q = 0; q < 2; ++q) {
+    const long o = (long)off + 8 * q;
+    if (o + 8 > r.n) continue;
+    const char* a = r.base + o;
+    if (!shim_inside(a, 8)) {
+      shim_bad(a);
+      continue;
+    }
+    unsigned w[2];
+    std::memcpy(w, a, 8);
+    v[2 * q] = w[0];
+    v[2 * q + 1] = w[1];
+  }
+  return v;
+}
+#define __builtin_amdgcn_raw_buffer_load_b128 shim_load_b128
+// the generated kernels' 16-byte value type (two doubles, 8-byte aligned)
+typedef double ShimD2 __attribute__((vector_size(16), aligned(8)));
+#define NEBO_JIT_D2 ShimD2
+
 static inline int shim_readlane(int v, int) { return v; }
 #define __builtin_amdgcn_readlane shim_readlane
 static inline double shim_dpp(double v, int, int, int, bool) { return v; }

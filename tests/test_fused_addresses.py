@@ -72,7 +72,7 @@ static void shim_store_hit(const void* p, size_t bytes)
     ++g_bad_stores;
     return;
   }
-  ++g_hits[(c - g_out_lo) / 8];
+  for (size_t b = 0; b < bytes; b += 8) ++g_hits[(c - g_out_lo + b) / 8];
 }
 '''
 
@@ -170,9 +170,13 @@ def stencil_tree(rng, leaves, stencils, depth):
 def cases():
     """(layout, use_ghost, column shape, seed) combinations."""
     out = []
+    # mode 1: two columns per lane where the register ring allows, 2: one column
     specs = [((13, 9, 11), 2, (1, 1, 0)), ((13, 9, 11), 1, (1, 2, 1)), ((70, 6, 5), 2, (1, 1, 3)),
              ((130, 3, 17), 1, (1, 4, 0)), ((64, 4, 8), ((1, 2, 1), (2, 1, 2)), (1, 2, 5)),
-             ((5, 13, 7), 2, (1, 1, 2)), ((66, 10, 4), "valid", (1, 1, 0))]
+             ((5, 13, 7), 2, (1, 1, 2)), ((66, 10, 4), "valid", (1, 1, 0)),
+             ((13, 9, 11), 2, (2, 1, 0)), ((70, 6, 5), 2, (2, 1, 3)), ((130, 3, 17), 1, (2, 4, 0)),
+             ((66, 10, 4), "valid", (2, 1, 0)), ((129, 5, 6), 1, (1, 0, 0)), ((127, 4, 9), 2, (1, 1, 4)),
+             ((200, 11, 7), 2, (1, 0, 3))]
     for i, (n, ghost, shape) in enumerate(specs):
         for use_ghost in (True, False):
             out.append((n, ghost, use_ghost, shape, 100 + 2 * i + int(use_ghost)))

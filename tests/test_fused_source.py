@@ -186,8 +186,9 @@ def test_z_chain_reads_are_shared():
 
 
 def test_column_schedule_source():
-    """div(k grad phi) runs on the column schedule by default: k and phi are one
-    family each (reads within one point in x, y and z), each lane holds 4 rows,
+    """div(k grad phi) on the column schedule with one column per lane (mode 2):
+    k and phi are one family each (reads within one point in x, y and z), each
+    lane holds 4 rows,
     so a family keeps rows -1 .. 4 of 3 planes plus the plane loading: 2 x 6 x 3
     loads before the z loop and 2 x 6 per step, each a raw buffer load at a
     per-lane offset from a per-plane resource.  The four x-neighbour reads per
@@ -199,8 +200,13 @@ def test_column_schedule_source():
     keep the flattened-plane schedule, as does every tree with the column schedule
     switched off."""
     t = _divkgrad_tree()
-    src = nh.fused_source(t)
+    try:
+        nh.fused_jit_column(2)
+        src = nh.fused_source(t)
+    finally:
+        nh.fused_jit_column()
     assert "__launch_bounds__(64)" in src and "int z = zs;   // column schedule" in src
+    assert "ld2_(" not in src
     pro, loops = src.split("int z = zs;   // column schedule")
     main, rest = loops.split("for (; z < zend; ++z)")
     assert "for (; z + 3 < zend; z += 4)" in main
@@ -234,7 +240,7 @@ def test_column_schedule_source():
 
 
 def test_column_bounds():
-    for bad in ((2, 0, 0), (1, 17, 0), (1, -1, 0), (1, 0, 5000), (1, 0, -3)):
+    for bad in ((3, 0, 0), (-1, 0, 0), (1, 17, 0), (1, -1, 0), (1, 0, 5000), (1, 0, -3)):
         with pytest.raises(nh.NeboError) as ei:
             nh._abi.call("nebo_fused_jit_column", *bad)
         assert ei.value.code == _abi.NEBO_ERR_INVALID
@@ -276,8 +282,13 @@ def test_column_faces_reused_across_rows_and_planes():
     kernel forms each once, carrying y faces from row to row (3 carries for 4
     rows) and z faces from plane to plane (one per row), with the chunk's first
     plane computing its low faces.  A subtree that differs from its shifted
-    partner in a constant is never taken for it."""
-    src = nh.fused_source(_divkgrad_tree())
+    partner in a constant is never taken for it.  (One column per lane here;
+    test_pair_column_source covers two.)"""
+    try:
+        nh.fused_jit_column(2)
+        src = nh.fused_source(_divkgrad_tree())
+    finally:
+        nh.fused_jit_column()
     assert len(set(re.findall(r"\bcy\d+_\d+", src))) == 3
     assert len(set(re.findall(r"\bcz\d+_\d+", src))) == 4
     assert "if (zq == zs) {" in src
@@ -288,6 +299,45 @@ def test_column_faces_reused_across_rows_and_planes():
     same = nh.Stencil([(0, -1, 0), (0, 0, 0)], [1.0, 2.0])
     other = nh.Stencil([(0, -1, 0), (0, 0, 0)], [3.0, 4.0])
     for lo_, pairs in ((same, True), (other, False)):
-        src = nh.fused_source(hi_(f) * 0.5 + lo_(f) * 0.25)
+        try:
+            nh.fused_jit_column(2)
+            src = nh.fused_source(hi_(f) * 0.5 + lo_(f) * 0.25)
+        finally:
+            nh.fused_jit_column()
         assert "// column schedule" in src
         assert (len(set(re.findall(r"\bcy\d+_\d+", src))) > 0) == pairs
+
+
+def test_pair_column_source():
+    """By default div(k grad phi) runs with two adjacent columns per lane and 4
+    waves stacked per block: the ring of two families at 4 rows would need 104
+    doubles, so a lane holds 2 rows of both columns (72 doubles).  Every ring load and store moves 16
+    bytes (ld2_ / st2_; a lane whose second column lies past the box stores its
+    first alone); the x face between a lane's two cells is formed by the first
+    and carried to the second, so each row takes 2 lane exchanges per family
+    instead of 4.  A tree of three such families stays on one column per lane
+    (4 rows: 84 doubles)."""
+    src = nh.fused_source(_divkgrad_tree())
+    assert "// column schedule" in src and "ncy = (A.n1 + 7) / 8" in src and "__launch_bounds__(256)" in src
+    assert "const int x = A.lo0 + bx * 128 + 2 * lane_, xc_ = x;" in src
+    pro, loops = src.split("int z = zs;   // column schedule")
+    main, rest = loops.split("for (; z < zend; ++z)")
+    assert "for (; z + 3 < zend; z += 4)" in main
+    ring = lambda text: len(re.findall(r"ld2_\(rs_, o\d+_\d+, R\d+_\d+_\d+_0, R\d+_\d+_\d+_1\)", text))
+    assert ring(pro) == 2 * 4 * 3 and ring(main) == 4 * 2 * 4 and ring(rest) == 2 * 4
+    assert not re.search(r"R\d+_\d+_\d+ = ld_\(", src)
+    assert main.count("__builtin_amdgcn_mov_dpp") == 4 * 2 * 4 and rest.count("__builtin_amdgcn_mov_dpp") == 2 * 4
+    assert main.count("st2_(&A.out") == 4 * 2 and rest.count("st2_(&A.out") == 2
+    assert len(set(re.findall(r"\bcx\d+_\d+", src))) == 2   # one x face per row
+    assert len(set(re.findall(r"\bcy\d+_\d+_\d", src))) == 2   # one y face per column
+    n = (6, 5, 4)
+    L = nh.Layout.build(n, 1)
+    f, g, h = host(L), host(L), host(L)
+    db = nh.build_stencils(*n, 1, 1, 1)
+    lap = lambda q: sum((db[("Divergence", f"SSurf{a}", "SVol")](db[("Gradient", "SVol", f"SSurf{a}")](q))
+                         for a in "YZ"), db[("Divergence", "SSurfX", "SVol")](db[("Gradient", "SVol", "SSurfX")](q)))
+    src3 = nh.fused_source(lap(f) + lap(g) + lap(h))
+    assert "// column schedule" in src3 and "ld2_(" not in src3 and "ncy = (A.n1 + 3) / 4" in src3
+    # one family (the Laplacian of one field) keeps one column and one wave
+    src1 = nh.fused_source(lap(f))
+    assert "ld2_(" not in src1 and "__launch_bounds__(64)" in src1

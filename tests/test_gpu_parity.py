@@ -241,9 +241,9 @@ def test_random_trees_bit_exact(seed, fused_mode):
 @pytest.fixture(params=[((0, -1, 0, 0, -1), (1, 0, 0)), ((0, -1, 0, 0, -1), (0, 0, 0)),
                         ((1024, 0, 8, 64, 1), (0, 0, 0)), ((64, -1, 16, 256, 0), (0, 0, 0)),
                         ((512, 64, 4, 48, 1), (0, 0, 0)), ((0, -1, 0, 0, -1), (1, 2, 7)),
-                        ((0, -1, 0, 0, 0), (1, 4, 1))],
+                        ((0, -1, 0, 0, 0), (1, 4, 1)), ((0, -1, 0, 0, -1), (2, 1, 3))],
                 ids=["default", "flat", "b1024-v8-xcd", "b64-v16", "tile64x8-xcd", "column-w2-z7",
-                     "column-w4-z1"])
+                     "column-w4-z1", "one-column-z3"])
 def jit_shape(request):
     """Compiled kernels of several launch shapes: the column schedule (rows and
     planes past the box end inside a block, 1-4 waves, 1-8 planes per block) and,
@@ -258,15 +258,17 @@ def jit_shape(request):
     nh.fused_jit_column()
 
 
-@pytest.mark.parametrize("column", [(1, 0, 0), (1, 1, 3), (1, 3, 5), (1, 2, 64)],
-                         ids=["default", "w1-z3", "w3-z5", "w2-z64"])
-def test_column_schedule_matches_oracle(column):
-    """Generic stencil trees on a 100 x 37 x 21 box with 2 ghost layers run on the
-    column schedule: two wave columns in x (the second 36 lanes past the box), row
-    groups and z chunks cut by the box end, 1-3 waves per block.  Results equal
-    the oracle's bit for bit, with the interior assigned (ghost cells untouched)
-    and with the ghost-inclusive region."""
-    n = (100, 37, 21)
+@pytest.mark.parametrize("column", [(1, 0, 0), (1, 1, 3), (1, 3, 5), (1, 2, 64), (2, 0, 0), (2, 3, 5)],
+                         ids=["default", "w1-z3", "w3-z5", "w2-z64", "one-column", "one-column-w3-z5"])
+@pytest.mark.parametrize("n", [(100, 37, 21), (131, 9, 10)])
+def test_column_schedule_matches_oracle(column, n):
+    """Generic stencil trees on boxes with 2 ghost layers run on the column
+    schedule, two columns per lane where the ring allows (mode 1) or one (mode
+    2): wave columns in x partly past the box (100 or 131 wide: a lane whose
+    second column lies past the box stores its first alone; ghost-inclusive
+    widths 102 and 133), row groups and z chunks cut by the box end, 1-3 waves
+    per block.  Results equal the oracle's bit for bit, with the interior
+    assigned (ghost cells untouched) and with the ghost-inclusive region."""
     L = nh.Layout.build(n, 2)
     rng = np.random.default_rng(11)
     k = both(L, rng.uniform(0.5, 2.0, L.npts))
