@@ -83,6 +83,10 @@ __device__ __forceinline__ double from_upper_lane(double v)
 
 
 // Lane j's value of v (v_readlane; j is uniform), as seen by every lane.
+// Lane j's value of v, for every lane.  Take it outside any branch or select on
+// the lane: inside one, it runs with only some lanes active, and a value the
+// compiler keeps in other registers (a spill) is brought back for those lanes
+// only, so lane j may read a register nobody refilled.
 __device__ __forceinline__ double lane_value(double v, int j)
 {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);

@@ -340,6 +340,7 @@ void emit_chunk(std::ostringstream& o, const VmProgram& P, const SlotUse& u, int
 struct ColumnPlan {
   int nf = 0, RY = 4, PF = 1;   // rows per lane; planes loaded ahead of the last plane in use
   bool pair = false;            // two adjacent columns per lane (16-byte loads and stores)
+  bool drop[kVmMaxLoads] = {};  // per family: its lowest plane is read only through z carries
   int root[kVmMaxLoads], cx[kVmMaxLoads], ylo[kVmMaxLoads], yhi[kVmMaxLoads], zlo[kVmMaxLoads], zhi[kVmMaxLoads];
   int fam[kVmMaxLoads], dx[kVmMaxLoads], dy[kVmMaxLoads], dz[kVmMaxLoads];
 };
@@ -351,6 +352,9 @@ constexpr int kColumnPrefetch = 1;  // planes loaded ahead, when the ring allows
 // against 71.5 with one column and one wave (the hand-written kernel 67.2); the
 // one-family Laplacian 43.1 us with one column against 44.6 with two.
 constexpr bool kColumnPairs = true;
+// The plane below a family's ring, when only z carries read it, may be loaded
+// once for the chunk's first plane instead of riding in the ring (select_ring).
+constexpr bool kColumnDropPlanes = true;
 // Defaults from sweeps at 256^3 (div(k grad phi) and Laplacian trees): with one
 // column per lane, one wave per block measured as fast as 2-4 waves and gives
 // small boxes the most blocks; two columns per lane run 4 waves stacked in y
@@ -361,9 +365,9 @@ std::atomic<int> g_column{1}, g_column_waves{0}, g_column_zc{0};
 
 long round_div(long a, long b) { return a >= 0 ? (a + b / 2) / b : -((-a + b / 2) / b); }
 
-// pairs: two columns per lane allowed (column mode 1 on a box wider than 64
-// columns; mode 2 keeps one)
-bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp, bool pairs)
+// The families of a tree's reads and their offsets (the ring's shape is chosen
+// by select_ring); false for a tree the column schedule does not take.
+bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp)
 {
   for (int m = 0; m < kVmMaxMasks; ++m)
     if (u.mused[m]) return false;   // mask bits past the box are not addressable
@@ -406,28 +410,7 @@ bool column_plan(const VmProgram& P, const SlotUse& u, ColumnPlan* cp, bool pair
     for (int s = 0; s < kVmMaxLoads; ++s)
       if (cp->fam[s] == f) cp->dx[s] = off[s][0] - cp->cx[f], cp->dy[s] = off[s][1], cp->dz[s] = off[s][2];
   }
-  if (!stencil) return false;
-  // ring doubles per lane at ry rows and pf planes ahead: per family, its rows
-  // (two values each with two columns per lane, and its edge value when it
-  // reads x neighbours) times its planes in use + pf
-  auto ring_regs = [&](int ry, int pf, bool pair) {
-    int n = 0;
-    for (int f = 0; f < cp->nf; ++f) {
-      bool edge = false;
-      for (int s = 0; s < kVmMaxLoads; ++s) edge = edge || (cp->fam[s] == f && cp->dx[s] != 0);
-      n += ((ry + cp->yhi[f] - cp->ylo[f]) * (pair ? 2 : 1) + (edge ? 1 : 0)) * (cp->zhi[f] - cp->zlo[f] + 1 + pf);
-    }
-    return n;
-  };
-  // two columns per lane where 4 or 2 rows fit, else one column with 4, 2 or 1
-  cp->RY = 0;
-  for (int pf = kColumnPrefetch; pf >= 1 && !cp->RY; --pf) {
-    for (int ry = 4; pairs && cp->nf >= 2 && ry >= 2 && !cp->RY; ry /= 2)
-      if (ring_regs(ry, pf, true) <= kColumnRegs) cp->RY = ry, cp->PF = pf, cp->pair = true;
-    for (int ry = 4; ry >= 1 && !cp->RY; ry /= 2)
-      if (ring_regs(ry, pf, false) <= kColumnRegs) cp->RY = ry, cp->PF = pf;
-  }
-  return cp->RY > 0;
+  return stencil;
 }
 
 // Tree form of a stack program: post-order (children before their parent, the
@@ -542,6 +525,74 @@ ShiftPlan shift_plan(const VmProgram& P, const ColumnPlan& cp, const std::vector
     }
   }
   return sp;
+}
+
+// Families whose lowest plane is read only inside subtrees that take their value
+// from the plane below (z carries): the chunk's first plane forms those values
+// before the z loop from one load of that plane, so the ring holds one plane
+// less of them.
+void drop_planes(const ColumnPlan& cp, const std::vector<TNode>& t, const ShiftPlan& sp, bool* drop)
+{
+  const int n = (int)t.size();
+  std::vector<int> first(n);
+  std::vector<char> carried(n, 0);
+  for (int i = 0; i < n; ++i) first[i] = t[i].nk ? first[t[i].kid[0]] : i;
+  for (int a = 0; a < n; ++a)
+    if (sp.from[a] >= 0 && sp.dir[a] == 2)
+      for (int j = first[a]; j <= a; ++j) carried[j] = 1;
+  for (int f = 0; f < cp.nf; ++f) drop[f] = kColumnDropPlanes && cp.zhi[f] > cp.zlo[f];
+  for (int i = 0; i < n; ++i) {
+    if (t[i].op != VM_LOAD) continue;
+    const int f = cp.fam[t[i].imm];
+    if (f >= 0 && cp.dz[t[i].imm] == cp.zlo[f] && !carried[i]) drop[f] = false;
+  }
+}
+
+// The ring's shape: rows per lane, planes ahead, one or two columns per lane
+// (pairs: allowed), and the planes each family keeps.  Two columns per lane
+// where 4 or 2 rows fit, for trees of two or more families, else one column
+// with 4, 2 or 1 rows; false when not even one row fits.
+bool select_ring(const VmProgram& P, ColumnPlan* cp, bool pairs)
+{
+  const std::vector<TNode> tree = program_tree(P);
+  // ring doubles per lane at ry rows and pf planes ahead: per family, its rows
+  // (two values each with two columns per lane, and its edge value when it
+  // reads x neighbours) times its planes in use + pf
+  auto ring_regs = [&](int ry, int pf, bool pair, const bool* drop) {
+    int n = 0;
+    for (int f = 0; f < cp->nf; ++f) {
+      bool edge = false;
+      for (int s = 0; s < kVmMaxLoads; ++s) edge = edge || (cp->fam[s] == f && cp->dx[s] != 0);
+      n += ((ry + cp->yhi[f] - cp->ylo[f]) * (pair ? 2 : 1) + (edge ? 1 : 0)) *
+           (cp->zhi[f] - cp->zlo[f] + 1 - (drop[f] ? 1 : 0) + pf);
+    }
+    return n;
+  };
+  // Preference: two columns per lane, then one; for each, the ring as it is,
+  // then with the planes below dropped (which only pays where it lets the
+  // layout fit: the 512^3 transport stages 4.31-4.37 ms with two columns per
+  // lane and dropped planes against 4.52-4.55 with one column; div(k grad phi)
+  // at 256^3 67.5 us with two columns and 2 rows as they are against 72.3-74.0
+  // with its planes dropped).
+  cp->RY = 0;
+  for (int pair = (pairs && cp->nf >= 2) ? 1 : 0; pair >= 0 && !cp->RY; --pair) {
+    cp->pair = pair == 1;
+    const ShiftPlan sp = shift_plan(P, *cp, tree);   // x carries depend on the layout
+    bool drop[kVmMaxLoads] = {};
+    drop_planes(*cp, tree, sp, drop);
+    const bool none[kVmMaxLoads] = {};
+    for (int use = 0; use <= 1 && !cp->RY; ++use) {
+      const bool* d = use ? drop : none;
+      for (int pf = kColumnPrefetch; pf >= 1 && !cp->RY; --pf)
+        for (int ry = 4; ry >= (pair ? 2 : 1) && !cp->RY; ry /= 2)
+          if (ring_regs(ry, pf, cp->pair, d) <= kColumnRegs) {
+            cp->RY = ry, cp->PF = pf;
+            for (int f = 0; f < cp->nf; ++f) cp->drop[f] = d[f];
+          }
+    }
+  }
+  if (!cp->RY) cp->pair = false;
+  return cp->RY > 0;
 }
 
 // Launch shape of the column schedule: waves per block (stacked in y) and z
@@ -669,11 +720,14 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     if (has_edge(f)) o << ind << "  " << edge_reg(f, k) << " = ld_(rs_, eo" << f << "_);\n";
     o << ind << "}\n";
   };
-  // nslots(f) = np + PF register slots per family (np planes in use): slots
-  // 0 .. np-1 hold planes z + zlo .. z + zhi, the next PF - 1 the planes after
-  // them, and the last receives plane z + zlo + nslots - 1 while step z computes
-  // (loads past the chunk end are not used; past the allocation they return 0)
-  auto nslots = [&](int f) { return cp.zhi[f] - cp.zlo[f] + 1 + cp.PF; };
+  // zl(f): the lowest plane the ring keeps (zlo, or zlo + 1 when the plane below
+  // is read only for the chunk's first plane); nslots(f) = np + PF register
+  // slots per family (np planes z + zl .. z + zhi in use, the next PF - 1 the
+  // planes after them, the last receiving plane z + zl + nslots - 1 while step z
+  // computes; loads past the chunk end are not used; past the allocation they
+  // return 0)
+  auto zl = [&](int f) { return cp.zlo[f] + (cp.drop[f] ? 1 : 0); };
+  auto nslots = [&](int f) { return cp.zhi[f] - zl(f) + 1 + cp.PF; };
   for (int f = 0; f < cp.nf; ++f) {
     const int rows = RY + cp.yhi[f] - cp.ylo[f], ns = nslots(f);
     for (int j = 0; j < rows; ++j)
@@ -681,7 +735,7 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
         for (int h = 0; h < PW; ++h) o << "  double " << ring(f, j, k, h) << " = 0.0;\n";
     if (has_edge(f))
       for (int k = 0; k < ns; ++k) o << "  double " << edge_reg(f, k) << " = 0.0;\n";
-    for (int k = 0; k + 1 < ns; ++k) load_plane("  ", f, k, "zs + " + std::to_string(cp.zlo[f] + k));
+    for (int k = 0; k + 1 < ns; ++k) load_plane("  ", f, k, "zs + " + std::to_string(zl(f) + k));
   }
   // Cells: the program as a tree, with the subtrees of shift_plan taken from the
   // row above or the plane below instead of recomputed.  Carries are written
@@ -697,7 +751,8 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     return "cz" + std::to_string(c) + "_" + std::to_string(r) + (cp.pair ? "_" + std::to_string(h) : std::string());
   };
   auto cx_name = [](int c, int r) { return "cx" + std::to_string(c) + "_" + std::to_string(r); };
-  auto emit_row = [&](const std::string& ind, int r, int h) {
+  // cell (row r, column h) of the tree, or (root >= 0) that subtree alone
+  auto emit_row = [&](const std::string& ind, int r, int h, int root = -1) {
     int tmp = 0;
     std::vector<std::pair<std::string, std::string>> writes;
     std::function<std::string(int, const std::string&, bool)> ev = [&](int i, const std::string& in,
@@ -707,15 +762,7 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
         const int c = sp.carry[sp.from[i]];
         if (sp.dir[i] == 1 && r > 0) return cy_name(c, r - 1, h);
         if (sp.dir[i] == 3 && h == 1) return cx_name(c, r);   // the lane's first column formed it
-        if (sp.dir[i] == 2) {
-          // the chunk's first plane computes it; later planes take the carry
-          const std::string v = "zc" + std::to_string(tmp++) + "_";
-          o << in << "double " << v << ";\n" << in << "if (zq == zs) {\n";
-          const std::string w = ev(i, in + "  ", true);
-          o << in << "  " << v << " = " << w << ";\n" << in << "} else {\n"
-            << in << "  " << v << " = " << cz_name(c, r, h) << ";\n" << in << "}\n";
-          return v;
-        }
+        if (sp.dir[i] == 2) return cz_name(c, r, h);   // the first plane's: formed before the z loop
       }
       std::string name;
       if (a.op == VM_LOAD || a.op == VM_LOADG) {
@@ -738,9 +785,49 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       }
       return name;
     };
-    const std::string res = ev((int)tree.size() - 1, ind, false);
+    const std::string res = root >= 0 ? ev(root, ind, true) : ev((int)tree.size() - 1, ind, false);
     for (const auto& w : writes) o << ind << w.first << " = " << w.second << ";\n";
     return res;
+  };
+  // the plane below's values of a family whose ring starts one plane higher,
+  // loaded once for the chunk's first plane
+  auto low_reg = [&](int f, int j, int h) {
+    return "P" + std::to_string(f) + "_" + std::to_string(j) + (cp.pair ? "_" + std::to_string(h) : std::string());
+  };
+  // The reads of cell (row r, column h) as L0_<slot>, at step u (ring slots
+  // renamed by u) and plane zq + zshift: -1 for the chunk's first plane's carried
+  // subtrees, formed at plane zs - 1 before the loop.  A read of a plane the
+  // ring does not keep is bound only there (from the plane-below registers).
+  auto bind_reads = [&](const std::string& ind, int r, int h, int u, int zshift) {
+    auto slot = [&](int f, int k) { return (k + u) % nslots(f); };
+    for (int s = 0; s < kVmMaxLoads; ++s) {
+      const int f = cp.fam[s];
+      if (f < 0) continue;
+      const int dz = cp.dz[s] + zshift;
+      if (dz < zl(f) && !(zshift < 0 && cp.drop[f] && dz == cp.zlo[f])) continue;
+      const bool low = dz < zl(f);
+      const int j = r + cp.dy[s] - cp.ylo[f], k = low ? 0 : slot(f, dz - zl(f));
+      auto val = [&](int hh) { return low ? low_reg(f, j, hh) : ring(f, j, k, hh); };
+      const std::string edge = low ? "PV" + std::to_string(f) : edge_reg(f, k);
+      const int e = (cp.dx[s] > 0 ? 32 : 0) + r + cp.dy[s] - ef[f].ey;   // lane holding the edge value
+      if (cp.dx[s] == 0) {
+        o << ind << "const double L0_" << s << " = " << val(h) << ";\n";
+        continue;
+      }
+      if (cp.pair && (cp.dx[s] < 0) == (h == 1)) {   // the neighbour is the lane's other column
+        o << ind << "const double L0_" << s << " = " << val(1 - h) << ";\n";
+        continue;
+      }
+      // the exchange and the edge lane's read are taken by every lane, outside
+      // any branch: a lane read inside the select would run with only the edge
+      // lane active, and a value the compiler keeps elsewhere (a spill) would be
+      // brought back for that lane alone, not for the lane read
+      o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << val(cp.dx[s] < 0 ? PW - 1 : 0)
+        << ", " << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
+      o << ind << "const double E0_" << s << " = rdlane_(" << edge << ", " << e << ");\n";
+      o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? E0_" << s << " : D0_"
+        << s << ";\n";
+    }
   };
   // One z step at plane zq: the loads of the plane PF planes ahead, then RY
   // cells.  Ring slot k of family f (plane z + zlo + k; the last slot is the
@@ -750,38 +837,16 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
   auto step = [&](const std::string& in, int u) {
     auto slot = [&](int f, int k) { return (k + u) % nslots(f); };
     for (int f = 0; f < cp.nf; ++f)
-      load_plane(in, f, slot(f, nslots(f) - 1), "zq + " + std::to_string(cp.zlo[f] + nslots(f) - 1));
+      load_plane(in, f, slot(f, nslots(f) - 1), "zq + " + std::to_string(zl(f) + nslots(f) - 1));
     for (size_t c = 0; c < sp.cdir.size(); ++c)
       if (sp.cdir[c] == 1)
         for (int r = 0; r + 1 < RY; ++r)
           for (int h = 0; h < PW; ++h) o << in << "double " << cy_name((int)c, r, h) << ";\n";
-    // the reads of cell (row r, column h) as L0_<slot>
-    auto bind_reads = [&](const std::string& ind, int r, int h) {
-      for (int s = 0; s < kVmMaxLoads; ++s) {
-        const int f = cp.fam[s];
-        if (f < 0) continue;
-        const int j = r + cp.dy[s] - cp.ylo[f], k = slot(f, cp.dz[s] - cp.zlo[f]);
-        const int e = (cp.dx[s] > 0 ? 32 : 0) + r + cp.dy[s] - ef[f].ey;   // lane holding the edge value
-        if (cp.dx[s] == 0) {
-          o << ind << "const double L0_" << s << " = " << ring(f, j, k, h) << ";\n";
-          continue;
-        }
-        if (cp.pair && (cp.dx[s] < 0) == (h == 1)) {   // the neighbour is the lane's other column
-          o << ind << "const double L0_" << s << " = " << ring(f, j, k, 1 - h) << ";\n";
-          continue;
-        }
-        // the exchange is taken by every lane, outside any branch
-        o << ind << "const double D0_" << s << " = __builtin_amdgcn_mov_dpp(" << ring(f, j, k, cp.dx[s] < 0 ? PW - 1 : 0)
-          << ", " << (cp.dx[s] > 0 ? "0x130" : "0x138") << ", 0xf, 0xf, true);\n";
-        o << ind << "const double L0_" << s << " = lane_ == " << (cp.dx[s] > 0 ? 63 : 0) << " ? rdlane_("
-          << edge_reg(f, k) << ", " << e << ") : D0_" << s << ";\n";
-      }
-    };
     for (int r = 0; r < RY; ++r) {
       const std::string ind = in + "  ";
       o << in << "{\n" << ind << "const int y = y0 + " << r << ";\n" << ind << "const int z0 = zq;\n";
       if (!cp.pair) {
-        bind_reads(ind, r, 0);
+        bind_reads(ind, r, 0, u, 0);
         emit_reads(o, P, direct, 1, true, ind);
         const std::string res = emit_row(ind, r, 0);
         o << ind << "if (xin_ && y < ye_) __builtin_nontemporal_store(" << res
@@ -795,7 +860,7 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
       for (int h = 0; h < 2; ++h) {
         const std::string in2 = ind + "  ";
         o << ind << "{\n" << in2 << "const int x = xc_ + " << h << ";\n";
-        bind_reads(in2, r, h);
+        bind_reads(in2, r, h, u, 0);
         emit_reads(o, P, direct, 1, true, in2);
         const std::string res = emit_row(in2, r, h);
         o << in2 << "v" << h << "_ = " << res << ";\n" << ind << "}\n";
@@ -818,6 +883,43 @@ std::string jit_column_source(const VmProgram& P, const SlotUse& u, const Column
     if (sp.cdir[c] == 2)
       for (int r = 0; r < RY; ++r)
         for (int h = 0; h < PW; ++h) o << "  double " << cz_name((int)c, r, h) << " = 0.0;\n";
+  // The chunk's first plane: its subtrees taken from the plane below are formed
+  // at plane zs - 1, from the ring and from one load of each dropped plane.
+  bool zcarry = false;
+  for (size_t c = 0; c < sp.cdir.size(); ++c) zcarry = zcarry || sp.cdir[c] == 2;
+  if (zcarry) {
+    o << "  {\n";
+    for (int f = 0; f < cp.nf; ++f) {
+      if (!cp.drop[f]) continue;
+      const int rows = RY + cp.yhi[f] - cp.ylo[f];
+      for (int j = 0; j < rows; ++j)
+        for (int h = 0; h < PW; ++h) o << "    double " << low_reg(f, j, h) << ";\n";
+      if (has_edge(f)) o << "    double PV" << f << ";\n";
+      o << "    {\n      const __amdgpu_buffer_rsrc_t rs_ = " << rsrc(f, "zs + " + std::to_string(cp.zlo[f])) << ";\n";
+      for (int j = 0; j < rows; ++j) {
+        if (cp.pair)
+          o << "      ld2_(rs_, o" << f << "_" << j << ", " << low_reg(f, j, 0) << ", " << low_reg(f, j, 1) << ");\n";
+        else
+          o << "      " << low_reg(f, j, 0) << " = ld_(rs_, o" << f << "_" << j << ");\n";
+      }
+      if (has_edge(f)) o << "      PV" << f << " = ld_(rs_, eo" << f << "_);\n";
+      o << "    }\n";
+    }
+    for (int r = 0; r < RY; ++r)
+      for (int h = 0; h < PW; ++h) {
+        o << "    {\n      const int y = y0 + " << r << ";\n      const int z0 = zs - 1;\n";
+        if (cp.pair) o << "      const int x = xc_ + " << h << ";\n";
+        bind_reads("      ", r, h, 0, -1);
+        for (int i = 0; i < (int)tree.size(); ++i)
+          if (sp.carry[i] >= 0 && sp.cdir[sp.carry[i]] == 2) {   // each subtree in a scope of its own
+            o << "      {\n";
+            emit_row("        ", r, h, i);
+            o << "      }\n";
+          }
+        o << "    }\n";
+      }
+    o << "  }\n";
+  }
   o << "  int z = zs;   // column schedule\n";
   if (U > 1) {
     o << "  for (; z + " << U - 1 << " < zend; z += " << U << ") {\n";
@@ -1059,7 +1161,8 @@ bool column_choice(const VmProgram& P, const nebo_limits_t* lim, ColumnPlan* cp,
 {
   const int mode = g_column.load();
   const bool wide = !lim || lim->hi[0] - lim->lo[0] > 64;
-  if (!mode || !column_plan(P, slot_use(P), cp, kColumnPairs && mode == 1 && wide)) return false;
+  if (!mode || !column_plan(P, slot_use(P), cp) || !select_ring(P, cp, kColumnPairs && mode == 1 && wide))
+    return false;
   cs->waves = g_column_waves.load();
   if (cs->waves == 0) cs->waves = cp->pair ? kDefaultPairWaves : kDefaultColumnWaves;
   cs->zc = g_column_zc.load();
@@ -1107,6 +1210,7 @@ std::string column_key(const VmProgram& P, const ColumnPlan& cp, const ColumnSha
     put(cp.ylo[f]);
     put(cp.yhi[f]);
     put(cp.zlo[f]);
+    put(cp.drop[f] ? 1 : 0);
     put(cp.zhi[f]);
   }
   for (int s = 0; s < kVmMaxLoads; ++s)

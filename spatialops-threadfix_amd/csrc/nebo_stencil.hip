@@ -326,9 +326,11 @@ __global__ __launch_bounds__(kTx * W * NW) void divkgrad_column_kernel(
         if constexpr (XR) {
           const double pu = from_upper_lane(pc), ku = from_upper_lane(kc);
           const double own = face_flux(kc, ku, pc, pu, c.ix0, c.ix1, c.gx0, c.gx1);
-          fx_hi = lane == kTx - 1 ? lane_value(fend, 32 + j) : own;
+          // lane reads outside the selects: every lane takes them (see lane_value)
+          const double end_hi = lane_value(fend, 32 + j), end_lo = lane_value(fend, j);
+          fx_hi = lane == kTx - 1 ? end_hi : own;
           const double below = from_lower_lane(fx_hi);
-          fx_lo = lane == 0 ? lane_value(fend, j) : below;
+          fx_lo = lane == 0 ? end_lo : below;
         } else {
           const double pl = from_lower_lane(pc), pu = from_upper_lane(pc);
           const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);

@@ -213,11 +213,15 @@ void transport_column_kernel(
         const double sl = from_lower_lane(sc), su = from_upper_lane(sc);
         const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
         const double uu = from_upper_lane(C0.ux[j]);
-        const double sxm = lane == 0 ? lane_value(C0.se, j) : sl;
-        const double sxp = lane == kLanes - 1 ? lane_value(C0.se, 32 + j) : su;
-        const double kxm = lane == 0 ? lane_value(C0.ke, j) : kl;
-        const double kxp = lane == kLanes - 1 ? lane_value(C0.ke, 32 + j) : ku;
-        const double uxh = lane == kLanes - 1 ? lane_value(C0.uxe, 32 + j) : uu;
+        // lane reads outside the selects: every lane takes them (see lane_value)
+        const double sel = lane_value(C0.se, j), ser = lane_value(C0.se, 32 + j);
+        const double kel = lane_value(C0.ke, j), ker = lane_value(C0.ke, 32 + j);
+        const double uxe = lane_value(C0.uxe, 32 + j);
+        const double sxm = lane == 0 ? sel : sl;
+        const double sxp = lane == kLanes - 1 ? ser : su;
+        const double kxm = lane == 0 ? kel : kl;
+        const double kxp = lane == kLanes - 1 ? ker : ku;
+        const double uxh = lane == kLanes - 1 ? uxe : uu;
         const double sym = C0.s[j], syp = C0.s[j + 2], kym = C0.k[j], kyp = C0.k[j + 2];
         const double szp = C1.s[j + 1], kzp = C1.k[j + 1];
         const double fx = face_flux(kxm, kc, sxm, sc, cx.i0, cx.i1, cx.g0, cx.g1) * cx.d0 +
@@ -422,10 +426,15 @@ void transport_adjoint_column_kernel(
         const double kl = from_lower_lane(kc), ku = from_upper_lane(kc);
         const double uu = from_upper_lane(C0.ux[j]);
         const bool first = lane == 0, last = lane == kLanes - 1;
-        const double axm = first ? lane_value(C0.ae, j) : al, axp = last ? lane_value(C0.ae, 32 + j) : au;
-        const double sxm = first ? lane_value(C0.se, j) : sl, sxp = last ? lane_value(C0.se, 32 + j) : su;
-        const double kxm = first ? lane_value(C0.ke, j) : kl, kxp = last ? lane_value(C0.ke, 32 + j) : ku;
-        const double uxh = last ? lane_value(C0.uxe, 32 + j) : uu;
+        // lane reads outside the selects: every lane takes them (see lane_value)
+        const double ael = lane_value(C0.ae, j), aer = lane_value(C0.ae, 32 + j);
+        const double sel = lane_value(C0.se, j), ser = lane_value(C0.se, 32 + j);
+        const double kel = lane_value(C0.ke, j), ker = lane_value(C0.ke, 32 + j);
+        const double uxe = lane_value(C0.uxe, 32 + j);
+        const double axm = first ? ael : al, axp = last ? aer : au;
+        const double sxm = first ? sel : sl, sxp = last ? ser : su;
+        const double kxm = first ? kel : kl, kxp = last ? ker : ku;
+        const double uxh = last ? uxe : uu;
         const AdjFace xlo = adj_face(axm, ac, sxm, sc, kxm, kc, C0.ux[j], cx);
         const AdjFace xhi = adj_face(ac, axp, sc, sxp, kc, kxp, uxh, cx);
         const AdjFace ylo = adj_face(C0.a[j], ac, C0.s[j], sc, C0.k[j], kc, C0.uy[j], cy);

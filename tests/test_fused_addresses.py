@@ -276,3 +276,45 @@ def test_column_launch_refuses_other_trees():
     assert column_launch(out, f * g + 1.0, True) is None
     m = nh.Mask(L, [(0, 0, 0)])
     assert column_launch(out, nh.cond(m, f)(g), True) is None
+
+
+def test_dropped_planes_stay_in_bounds():
+    """Kernels whose ring drops the plane below (the chunk's first plane formed
+    by a prologue from one load of that plane per family): three Laplacians on
+    a box 130 wide with two ghost layers (two columns per lane, ghost-inclusive
+    and interior regions) and a transport stage at 66^3 (six fields, face
+    velocities on their own layouts), audited like the others."""
+    from nebo_hip import workloads as W
+    jobs = []
+    n = (130, 6, 9)
+    L = nh.Layout.build(n, 2)
+    rng = np.random.default_rng(31)
+    f, g, h = host_field(L, "f", rng), host_field(L, "g", rng), host_field(L, "h", rng)
+    out = nh.Field.empty(L, "cpu", name="out")
+    db = nh.build_stencils(*n, 1.0, 1.0, 1.0)
+    lap = lambda q: sum((db[("Divergence", f"SSurf{a}", "SVol")](db[("Gradient", "SVol", f"SSurf{a}")](q))
+                         for a in "YZ"), db[("Divergence", "SSurfX", "SVol")](db[("Gradient", "SVol", "SSurfX")](q)))
+    t = lap(f) + lap(g) + lap(h)
+    for use_ghost in (True, False):
+        lim = nh.assign_limits(out, t, use_ghost, where="host")
+        launch = column_launch(out, t, use_ghost)
+        assert launch is not None and "double P" in launch[0]
+        jobs.append(([f, g, h, out], out, launch, lim))
+    m = 66
+    Lt, Ls, ops, k, phi, u, dt = W.transport_problem(m)
+    K = nh.Field.from_numpy(Lt, k, where="cpu", name="k")
+    U = [nh.Field.from_numpy(l, x.reshape(-1), where="cpu", name=f"u{d}") for d, (l, x) in enumerate(zip(Ls, u))]
+    PN = nh.Field.from_numpy(Lt, phi, where="cpu", name="phin")
+    S = nh.Field.from_numpy(Lt, phi.copy(), where="cpu", name="s")
+    lo_, hi_ = W.upwind_ops()
+    s1, s2, s3 = W.rk3_stage_exprs(ops, PN, K, U, dt, lo_, hi_)
+    st = s2(S)
+    out_t = nh.Field.empty(Lt, "cpu", name="out")
+    lim = nh.assign_limits(out_t, st, True, where="host")
+    launch = column_launch(out_t, st, True)
+    assert launch is not None and "double P" in launch[0] and "ld2_(rs_" in launch[0]
+    jobs.append(([K, PN, S, out_t] + U, out_t, launch, lim))
+    with tempfile.TemporaryDirectory() as tmp:
+        libs = build([j[2][0] for j in jobs], tmp)
+        for so, (fields, lhs, launch, lim) in zip(libs, jobs):
+            audit(so, fields, lhs, launch, lim)

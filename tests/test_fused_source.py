@@ -280,8 +280,9 @@ def test_column_faces_reused_across_rows_and_planes():
     """In div(k grad phi) the low y face of row r is the high y face of row r - 1,
     and the low z face of plane z the high z face of plane z - 1: the column
     kernel forms each once, carrying y faces from row to row (3 carries for 4
-    rows) and z faces from plane to plane (one per row), with the chunk's first
-    plane computing its low faces.  A subtree that differs from its shifted
+    rows) and z faces from plane to plane (one per row); the chunk's first
+    plane takes its low faces from a prologue that forms them at plane zs - 1
+    before the z loop, which has no branch on the plane.  A subtree that differs from its shifted
     partner in a constant is never taken for it.  (One column per lane here;
     test_pair_column_source covers two.)"""
     try:
@@ -291,7 +292,7 @@ def test_column_faces_reused_across_rows_and_planes():
         nh.fused_jit_column()
     assert len(set(re.findall(r"\bcy\d+_\d+", src))) == 3
     assert len(set(re.findall(r"\bcz\d+_\d+", src))) == 4
-    assert "if (zq == zs) {" in src
+    assert "if (zq == zs)" not in src and src.count("const int z0 = zs - 1;") == 4
     n = (8, 8, 8)
     L = nh.Layout.build(n, 1)
     f = host(L)
@@ -315,8 +316,9 @@ def test_pair_column_source():
     bytes (ld2_ / st2_; a lane whose second column lies past the box stores its
     first alone); the x face between a lane's two cells is formed by the first
     and carried to the second, so each row takes 2 lane exchanges per family
-    instead of 4.  A tree of three such families stays on one column per lane
-    (4 rows: 84 doubles)."""
+    instead of 4.  A tree of three such families fits two columns per lane (2
+    rows) only with the plane below each family's ring dropped (81 doubles
+    instead of 108), which it then is."""
     src = nh.fused_source(_divkgrad_tree())
     assert "// column schedule" in src and "ncy = (A.n1 + 7) / 8" in src and "__launch_bounds__(256)" in src
     assert "const int x = A.lo0 + bx * 128 + 2 * lane_, xc_ = x;" in src
@@ -337,7 +339,33 @@ def test_pair_column_source():
     lap = lambda q: sum((db[("Divergence", f"SSurf{a}", "SVol")](db[("Gradient", "SVol", f"SSurf{a}")](q))
                          for a in "YZ"), db[("Divergence", "SSurfX", "SVol")](db[("Gradient", "SVol", "SSurfX")](q)))
     src3 = nh.fused_source(lap(f) + lap(g) + lap(h))
-    assert "// column schedule" in src3 and "ld2_(" not in src3 and "ncy = (A.n1 + 3) / 4" in src3
+    assert "// column schedule" in src3 and "ld2_(rs_" in src3 and "ncy = (A.n1 + 7) / 8" in src3
+    assert re.search(r"ld2_\(rs_, o\d+_\d+, P\d+_\d+_0, P\d+_\d+_1\)", src3)
     # one family (the Laplacian of one field) keeps one column and one wave
     src1 = nh.fused_source(lap(f))
     assert "ld2_(" not in src1 and "__launch_bounds__(64)" in src1
+
+
+def test_transport_stage_drops_the_plane_below():
+    """A transport stage (six fields: upwinded convective and diffusive fluxes)
+    fits two columns per lane only with the ring one plane shorter: its z faces
+    are carried from plane to plane, so the plane below is read only for the
+    chunk's first plane, which a prologue forms from one load of that plane per
+    family (P... registers) before the z loop.  div(k grad phi), which fits as
+    it is, keeps the plane in its ring."""
+    from nebo_hip import workloads as W
+    n = 16
+    L, Ls, ops, k, phi, u, dt = W.transport_problem(n)
+    K = nh.Field.from_numpy(L, k, where="cpu")
+    U = [nh.Field.from_numpy(l, x.reshape(-1), where="cpu") for l, x in zip(Ls, u)]
+    P = [nh.Field.from_numpy(L, phi, where="cpu") for _ in range(3)]
+    lo, hi = W.upwind_ops()
+    s1, s2, s3 = W.rk3_stage_exprs(ops, P[0], K, U, dt, lo, hi)
+    src = nh.fused_source(s2(P[1]))
+    pro, loops = src.split("int z = zs;   // column schedule")
+    assert "ld2_(" in src and "(zs + -1)" in pro and re.search(r"\bP\d+_\d+_0 = |ld2_\(rs_, o\d+_\d+, P\d+_", pro)
+    assert "(zs + -1)" not in loops and "if (zq == zs)" not in src
+    # the wave-edge values are read by every lane, then selected
+    assert re.search(r"const double E0_\d+ = rdlane_\(", loops) and "? rdlane_(" not in src
+    div = nh.fused_source(_divkgrad_tree())
+    assert "ld2_(rs_" in div and not re.search(r"\bP\d+_\d+_\d+", div)
