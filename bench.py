@@ -40,6 +40,10 @@ def parse():
                    help="skip the batched, adjoint, transport, generic and heat legs")
     p.add_argument("--pmc-probe", action="store_true",
                    help="only launch 3 known-size copies and 3 kernels (for rocprofv3 --pmc runs)")
+    p.add_argument("--pmc-probe-batched", action="store_true",
+                   help="the same for the batched step: 3 copies of the patch blocks' size, 3 steps")
+    p.add_argument("--batch-tile", default="0,2,16",
+                   help="item tile of --pmc-probe-batched (x,y,z; x = 0: column items, y rows per lane)")
     return p.parse_args()
 
 
@@ -127,17 +131,23 @@ def cpu_baseline(L, kh, ph, coefs, budget_s):
             "oracle_build": "gcc -O3 -DNDEBUG -ffp-contract=off"}
 
 
+def batched_dims(count=512):
+    """Interior sizes of the batched workload: each axis drawn from {16,24,32,48,64}
+    by a seeded splitmix64 stream."""
+    import numpy as np
+    from nebo_hip import inputs
+    sizes = np.array([16, 24, 32, 48, 64])
+    idx = (inputs.splitmix64(4, 3 * count) % np.uint64(5)).astype(np.int64).reshape(count, 3)
+    return [tuple(int(v) for v in sizes[i]) for i in idx]
+
+
 def batched_leg(steps=10, count=512):
     """512 patches with interior sizes drawn from {16,24,32,48,64} per axis (seeded),
     Laplacian + Euler per step: one persistent launch per step captured 10 at a
     time in a hipGraph, against one launch per patch per step."""
-    import numpy as np
     import torch
     import nebo_hip as nh
-    from nebo_hip import inputs
-    sizes = np.array([16, 24, 32, 48, 64])
-    idx = (inputs.splitmix64(4, 3 * count) % np.uint64(5)).astype(np.int64).reshape(count, 3)
-    dims = [tuple(int(v) for v in sizes[i]) for i in idx]
+    dims = batched_dims(count)
     cells = sum(d[0] * d[1] * d[2] for d in dims)
     db = nh.build_stencils(64, 64, 64, 1.0, 1.0, 1.0)
     coefs = nh.axis_coefs(db)
@@ -611,6 +621,34 @@ def pmc_probe(n, tile=None):
                       "tile": _tile_name(nh.stencil_select_tile(0, L.interior))}))
 
 
+def pmc_probe_batched(tile, count=512):
+    """Three copies of a buffer the size of the patch blocks (known bytes) and three
+    batched steps (alternating a -> b -> a) with the given item tile, nothing else."""
+    import torch
+    import nebo_hip as nh
+    dims = batched_dims(count)
+    ghosted = sum((d[0] + 2) * (d[1] + 2) * (d[2] + 2) for d in dims)
+    cells = sum(d[0] * d[1] * d[2] for d in dims)
+    db = nh.build_stencils(64, 64, 64, 1.0, 1.0, 1.0)
+    a, b = nh.alloc_patches(dims)
+    for x in a:
+        x.normal_()
+    B = nh.Batch(a, b, dims, nh.axis_coefs(db), 0.1 / 64.0 ** 2, tile=tile)
+    x = torch.randn(ghosted, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    for i in range(3):
+        B.step(i & 1)
+    torch.cuda.synchronize()
+    info = B.info()
+    B.close()
+    print(json.dumps({"pmc_probe_batched": True, "copy_bytes_read": 8 * ghosted, "copy_bytes_written": 8 * ghosted,
+                      "step_bytes_read_min": 8 * ghosted, "step_bytes_written_min": 8 * cells,
+                      "tile": list(info[0]), "items": info[1]}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -620,6 +658,9 @@ def main():
     tile_arg = tuple(int(v) for v in a.tile.split(",")) if a.tile else None
     if a.pmc_probe:
         pmc_probe(a.n, tile_arg)
+        return
+    if a.pmc_probe_batched:
+        pmc_probe_batched(tuple(int(v) for v in a.batch_tile.split(",")))
         return
     import numpy as np
     import torch
