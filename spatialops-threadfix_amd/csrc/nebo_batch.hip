@@ -225,7 +225,9 @@ __global__ __launch_bounds__(256) void batch_column_kernel(const DevPatch* __res
       gz[j] = gz_hi;
       if (active && yb + j < p.ny) store_nt_at(ro, row[j + 1], pc + s * ((Lx + Ly) + Lz));
     }
-    S[jj] = load(min(z + NS, z1));
+    // the slot's next plane is read by step z + NS - 1 at the earliest: past the
+    // item's last plane (z1) no step reads it (z is uniform over the wave)
+    if (z + NS <= z1) S[jj] = load(z + NS);
   };
   const int full = (z1 - z0) / NS, rem = (z1 - z0) - NS * full;
   int z = z0;
@@ -358,12 +360,26 @@ int check_tile(const int32_t* t)
   return NEBO_OK;
 }
 
+// The column kernel for ry rows per lane and items of zc planes.  Items of up to
+// 4 or 8 planes load every plane they read as they start (planes ahead >= item
+// length), so a wave has all its loads in flight at once; longer items keep 2-3
+// planes ahead.  512 patches of 16-64^3 (hipGraph of 10 steps, one box): items of
+// 4 planes with 2 rows per lane 0.115 ms per step, of 8 planes 0.117, against 0.131
+// for 16-plane items 3 planes ahead; 4-plane items took 0.151 ms 3 planes ahead and
+// 0.121-0.155 ms 5-9 ahead (whose first loads repeat the item's last plane).
+using ColumnKernel = void (*)(const DevPatch*, const int4*, int, int, Coef6, double, int);
+ColumnKernel column_kernel(int ry, int zc)
+{
+  if (zc <= 4) return ry == 1 ? batch_column_kernel<1, 4> : ry == 2 ? batch_column_kernel<2, 4> : batch_column_kernel<4, 4>;
+  if (zc <= 8) return ry == 1 ? batch_column_kernel<1, 8> : ry == 2 ? batch_column_kernel<2, 8> : batch_column_kernel<4, 8>;
+  return ry == 1 ? batch_column_kernel<1, 3> : ry == 2 ? batch_column_kernel<2, 3> : batch_column_kernel<4, 2>;
+}
+
 int launch_items(nebo_batch* b, int first, int count, int grid, int swap, hipStream_t s)
 {
   if (count <= 0) return NEBO_OK;
   if (b->tile[0] == 0) {
-    const int ry = b->tile[1];
-    auto fn = ry == 1 ? batch_column_kernel<1, 3> : ry == 2 ? batch_column_kernel<2, 3> : batch_column_kernel<4, 2>;
+    const ColumnKernel fn = column_kernel(b->tile[1], b->tile[2]);
     hipLaunchKernelGGL(fn, dim3((count + 3) / 4), dim3(256), 0, s, b->d_patches, b->d_items + first, count,
                        b->tile[2], coef6(b->coefs), b->s, swap);
   } else if (b->tile[1] == 0) {
@@ -513,7 +529,7 @@ extern "C" int nebo_batch_autotune(nebo_batch_t* b, int32_t* tile_out, void* str
       {512, 0, 8}, {1024, 0, 4}, {1024, 0, 8},
       // column schedule (x = 0): rows per lane, planes per item
       {0, 1, 16}, {0, 1, 32}, {0, 1, 64}, {0, 2, 16}, {0, 2, 32}, {0, 2, 64}, {0, 4, 16}, {0, 4, 32},
-      {0, 4, 64}, {0, 2, 8}};
+      {0, 4, 64}, {0, 1, 8}, {0, 2, 8}, {0, 4, 8}, {0, 1, 4}, {0, 2, 4}, {0, 4, 4}};
   hipEvent_t e0, e1;
   int rc = check_hip(hipEventCreate(&e0), ctx);
   if (!rc) rc = check_hip(hipEventCreate(&e1), ctx);

@@ -40,9 +40,10 @@ def test_batch_matches_oracle_and_per_patch(count):
     # the default tile, a 2-D tile run one launch per patch, flattened-plane
     # tiles: chunk lengths without (5) and with a compiled full-chunk path (4, 8,
     # 16; patch sides 16..64 also leave cut chunks), and column tiles (x = 0: 1, 2
-    # or 4 rows per lane)
+    # or 4 rows per lane; items of up to 4 and 8 planes load all their planes as
+    # they start)
     tiles = [None, (16, 8, 4), (128, 0, 5), (64, 0, 4), (256, 0, 8), (1024, 0, 16), (0, 1, 16), (0, 2, 5),
-             (0, 4, 64)]
+             (0, 4, 64), (0, 2, 4), (0, 4, 8), (0, 1, 3)]
     blocks = []
     for _ in tiles:
         a, b = nh.alloc_patches(dims)
@@ -67,7 +68,7 @@ def test_batch_matches_oracle_and_per_patch(count):
         B.close()
 
 
-@pytest.mark.parametrize("tile", [(0, 1, 7), (0, 2, 16), (0, 4, 3)])
+@pytest.mark.parametrize("tile", [(0, 1, 7), (0, 2, 16), (0, 4, 3), (0, 2, 4), (0, 4, 8), (0, 2, 6)])
 def test_batch_column_tiles_odd_patches(tile):
     """Column tiles on patches narrower than a wave with widths that leave lanes
     idle (7, 24, 63), two columns wide (32 row groups per wave), exactly one wave
