@@ -487,8 +487,8 @@ typedef struct nebo_batch nebo_batch_t;   /* opaque */
  * idle on patches narrower than the tile, or with x = 0 the column schedule: one
  * wave per item covering up to 64 columns of a patch (64/nx row groups side by
  * side on patches narrower than 64), y = 1, 2 or 4 rows per lane, z planes per
- * item, x neighbours exchanged between lanes; tile == NULL starts from
- * (64, 4, 16).  Coefficients and s are shared by all patches:
+ * item (items of up to 8 planes load all their planes as they start), x
+ * neighbours exchanged between lanes; tile == NULL starts from (64, 4, 16).  Coefficients and s are shared by all patches:
  * out = in + s*((Lx + Ly) + Lz) as in nebo_lapl_euler. */
 int nebo_batch_create(const nebo_patch_t* patches, int32_t n,
                       const nebo_axis_coefs_t* c, double s,
