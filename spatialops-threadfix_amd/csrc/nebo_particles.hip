@@ -201,7 +201,26 @@ __global__ void run_sum_kernel(double* __restrict__ dest, const uint32_t* __rest
   const uint32_t cell = keys[s];
   if (s > 0 && keys[s - 1] == cell) return;
   double acc = dest[cell];
-  for (long t = s; t < total && keys[t] == cell; ++t) acc += vals[t];
+  // The run is read U entries at a time, all loads issued before the adds, which
+  // still go one by one in entry order up to the first entry of another cell
+  // (cells are below 2^31, so the padding key 0xffffffff ends a run).
+  constexpr int U = 8;
+  for (long t = s;; t += U) {
+    uint32_t k[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      k[u] = t + u < total ? keys[t + u] : 0xffffffffu;
+      v[u] = t + u < total ? vals[t + u] : 0.0;
+    }
+    bool end = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      end = end || k[u] != cell;
+      if (!end) acc += v[u];
+    }
+    if (end) break;
+  }
   dest[cell] = acc;
 }
 
